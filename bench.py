@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Throughput of the batched ion-trajectory integrator on one MI355X.
+
+One step = one run_batch of the whole workload (inject, integrate every ion to
+termination, summarize), inputs resident in HBM. The default workload is the
+batched mass scan: m/z 475..523 step 2 (25 masses) x 64 RF frequencies
+(450..550 kHz) x 1000 ions = 1.6M trajectories on a 128x128x512 grid at
+0.05 mm, Fig-4 drive (100 V RF, 0.5 V AC at 100 kHz, 5 V DC gate), half-kick
+(1/2 a dt) push, thermal N2 at 1 mTorr / 300 K, Gaussian 0.1 mm spot with a
+300 K Maxwell-Boltzmann start, dt = 100 ns, 20,000-step cap.
+
+Prints one JSON line: value = FP32 batched ion-steps/s; plus the plain FP64
+form on the same inputs, the outcome agreement between them, the roofline
+object of the batched worker kernel and the CPU baseline (the FP64 oracle on
+the host's cores over a bounded sample of the same jobs).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "multiq-workbench_amd"))
+
+import numpy as np  # noqa: E402
+
+import mqhip  # noqa: E402
+import synth  # noqa: E402
+
+BYTES_PER_STEP_FP32 = 292.0   # 8 corners x 3 bases x 3 components x 4 B + 4 B mask word
+BYTES_PER_STEP_FP64 = 580.0
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload(name: str):
+    if name == "c2":
+        grid = synth.c2_grid()
+        jobs = synth.c2_jobs(mqhip, n=1000)
+        ph = mqhip.Physics.correct(0.133322).copy(dt=1e-7, max_steps=20000, injection=mqhip.INJ_GAUSSIAN,
+                                                  inj_sigma_mm=0.1, legacy_force=1)
+        desc = ("mass scan: 25 m/z (475-523) x 64 f_rf (450-550 kHz) x 1000 ions = 1.6M trajectories; "
+                "128x128x512 grid @ 0.05 mm (FP32 cell tables); 100 V RF, 0.5 V AC @ 100 kHz, 5 V DC gate; "
+                "half-kick push; thermal N2 1 mTorr 300 K; Gaussian 0.1 mm spot, 300 K MB; dt 100 ns; cap 20000 steps")
+        return grid, jobs, [0.0, 5.0, 0.0, 0.0], ph, desc
+    if name == "c3":
+        grid = synth.c3_grid()
+        jobs = synth.c3_jobs(mqhip)
+        ph = mqhip.Physics.correct(0.133322).copy(dt=1e-7, max_steps=20000, injection=mqhip.INJ_GAUSSIAN,
+                                                  inj_sigma_mm=0.1, legacy_force=1)
+        desc = ("operating-setting sweep: 4096 LHS settings (V_rf 50-150 V, f_exc 80-120 kHz, V_exc 0.1-1 V, "
+                "V_dc 0-10 V, 0.5-2 mTorr) x 256 ions = 1M trajectories; 192x192x768 grid @ 1/30 mm")
+        return grid, jobs, [0.0, 1.0, 0.0, 0.0], ph, desc
+    if name == "c1":
+        grid = synth.c1_grid()
+        jobs = synth.c1_jobs(mqhip, n=1000)
+        ph = mqhip.Physics.correct(0.133322).copy(dt=1e-7, max_steps=20000, injection=mqhip.INJ_GAUSSIAN,
+                                                  inj_sigma_mm=0.1, legacy_force=1)
+        return grid, jobs, [0.0, 5.0, 0.0, 0.0], ph, "Fig-4 single channel, 1000 ions, 64x64x256 @ 0.1 mm"
+    raise SystemExit(f"unknown config {name}")
+
+
+def pulls(a, b):
+    """Per-job, per-class binomial pulls between two outcome lists."""
+    out = []
+    for x, y in zip(a, b):
+        for k in ("crashed", "returned", "transmitted", "left", "timeout"):
+            p1, p2 = getattr(x, k) / max(x.n, 1), getattr(y, k) / max(y.n, 1)
+            se = np.sqrt(p1 * (1 - p1) / max(x.n, 1) + p2 * (1 - p2) / max(y.n, 1))
+            out.append(0.0 if se == 0 and p1 == p2 else (p1 - p2) / max(se, 1e-12))
+    return np.array(out)
+
+
+def cpu_baseline(grid, jobs, dc, ph, seconds: float):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the FP64 oracle: CPU baseline only
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    threads = max(1, min(16, ncpu))
+    # calibrate on one job, then size the sample to about `seconds`
+    idx = np.linspace(0, len(jobs) - 1, len(jobs)).astype(int)
+    t0 = time.perf_counter()
+    r = oracle.run_batch(grid, [jobs[idx[len(idx) // 2]]], dc, ph, threads)
+    t1 = time.perf_counter() - t0
+    rate0 = r["steps"].sum() / max(t1, 1e-9)
+    per_job = r["steps"].sum()
+    njobs = int(max(1, min(len(jobs), seconds * rate0 / max(per_job, 1))))
+    pick = [jobs[i] for i in np.linspace(0, len(jobs) - 1, njobs).astype(int)]
+    log(f"cpu baseline: oracle on {njobs} jobs x {pick[0].n} ions with {threads} threads")
+    t0 = time.perf_counter()
+    r = oracle.run_batch(grid, pick, dc, ph, threads)
+    dt = time.perf_counter() - t0
+    steps = int(r["steps"].sum())
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": steps / dt, "unit": "ion-steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle (C restatement of the reference step) on {njobs} evenly spaced jobs of the "
+                      f"workload ({njobs * pick[0].n} trajectories, {steps} ion-steps, {dt:.1f} s); host: {model}, "
+                      f"{ncpu} CPUs visible"}
+
+
+def read_traffic(name: str):
+    """HBM bytes per worker launch from the committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"k_batched_{name}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--skip-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        sys.exit("bench.py runs on one GPU only: the integrator has no multi-GPU path in this build "
+                 "(ions are independent; see the sharding design). Use --gpus 1.")
+
+    log(f"building the {a.config} workload")
+    grid, jobs, dc, ph, desc = workload(a.config)
+    n_traj = int(sum(j.n for j in jobs))
+    eng = mqhip.Engine(grid)
+    log(f"engine ready: {grid['nx']}x{grid['ny']}x{grid['nz']} grid, {len(jobs)} jobs, {n_traj} trajectories")
+    for w in range(a.warmup):
+        eng.run_batch(jobs, dc, ph, mqhip.PATH_BATCHED)
+        log(f"warmup {w + 1}/{a.warmup}: {eng.stats()['ion_steps']} ion-steps")
+    steps = kms = 0
+    launches = 0
+    outs = None
+    t0 = time.perf_counter()
+    for s in range(a.steps):
+        outs = eng.run_batch(jobs, dc, ph, mqhip.PATH_BATCHED)
+        st = eng.stats()
+        steps += st["ion_steps"]
+        kms += st["kernel_ms"]
+        launches += st["launches"]
+        log(f"step {s + 1}/{a.steps}: {st['ion_steps']} ion-steps, worker kernels {st['kernel_ms']:.1f} ms, "
+            f"wall {st['wall_ms']:.1f} ms")
+    wall = time.perf_counter() - t0
+    value = steps / wall
+
+    result = {
+        "metric": "ion-steps/s, batched FP32 vs plain FP64, matched outcome counts; grad evals/s",
+        "value": value, "unit": "ion-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / max(a.steps, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (FP64 position/time/phase)", "data": "synthetic",
+        "config": {"workload": desc, "trajectories": n_traj, "jobs": len(jobs), "grid": [grid["nx"], grid["ny"],
+                   grid["nz"]], "h_mm": grid["h"] * 1e3, "parallelism": "single GPU"},
+    }
+    achieved = steps * BYTES_PER_STEP_FP32 / (kms * 1e-3) / 1e9
+    result["roofline"] = {"bound": "hbm", "kernel": "k_batched", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                          "traffic": read_traffic(a.config),
+                          "bytes_per_ion_step": BYTES_PER_STEP_FP32,
+                          "ion_steps_per_launch": steps / max(launches, 1),
+                          "avg_launch_ms": kms / max(launches, 1)}
+
+    if a.plain_steps > 0:
+        psteps = pk = 0
+        pouts = None
+        t0 = time.perf_counter()
+        for s in range(a.plain_steps):
+            pouts = eng.run_batch(jobs, dc, ph, mqhip.PATH_PLAIN)
+            st = eng.stats()
+            psteps += st["ion_steps"]
+            pk += st["kernel_ms"]
+            log(f"plain FP64 step {s + 1}/{a.plain_steps}: {st['ion_steps']} ion-steps, kernels "
+                f"{st['kernel_ms']:.1f} ms, {st['launches']} launches")
+        pwall = time.perf_counter() - t0
+        pv = psteps / pwall
+        pl = pulls(outs, pouts)
+        tot = lambda os_, k: sum(getattr(o, k) for o in os_)
+        result["plain_fp64"] = {
+            "value": pv, "unit": "ion-steps/s", "ms_per_step": 1e3 * pwall / a.plain_steps,
+            "roofline_achieved_gbs": psteps * BYTES_PER_STEP_FP64 / (pk * 1e-3) / 1e9,
+            "outcomes_batched": {k: tot(outs, k) for k in ("crashed", "returned", "transmitted", "left", "timeout")},
+            "outcomes_plain": {k: tot(pouts, k) for k in ("crashed", "returned", "transmitted", "left", "timeout")},
+            "per_job_pulls_within_3sigma": float(np.mean(np.abs(pl) <= 3.0)),
+            "max_abs_pull": float(np.max(np.abs(pl))),
+        }
+        result["speedup_vs_plain_fp64"] = value / pv
+
+    if not a.skip_cpu:
+        result["cpu_baseline"] = cpu_baseline(grid, jobs, dc, ph, a.cpu_seconds)
+    eng.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,173 @@
+/*
+ * mq_engine.h -- C ABI of the MI355X batched ion-trajectory integrator.
+ *
+ * Drop-in boundary for chsim's batch/scan API (simulation/chsim/src/channel.rs,
+ * simulation/chsim/src/scan.rs). Plain pointers and sizes only; every call is
+ * synchronous on the engine's own HIP stream; an engine handle is not
+ * thread-safe (the reference Engine is !Sync, channel.rs:309-318).
+ *
+ * Return convention: 0 = ok, < 0 = error; mq_last_error() gives the message of
+ * the last failing call on the calling thread. Where the reference panics
+ * (channel.rs:328,331,444,446) these entry points return an error instead.
+ */
+#ifndef MQ_ENGINE_H
+#define MQ_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Ion status codes: ChIon p.w in channel.metal:32. */
+enum {
+    MQ_ALIVE = 0,
+    MQ_CRASHED = 1,
+    MQ_RETURNED = 2,
+    MQ_TRANSMITTED = 3,
+    MQ_LEFT = 4,
+    MQ_TIMEOUT = 5
+};
+
+/* Execution path: Path { Tiled, Direct } in channel.rs:304.
+ * BATCHED = FP32 fields/interpolation/push with FP64 position, time and phase,
+ *           persistent wavefront workers with compaction (tiled.rs:159-218).
+ * PLAIN   = FP64 everything, one thread per ion, relaunched until every ion
+ *           has finished (Engine::run_direct, channel.rs:478-493). */
+enum { MQ_PATH_BATCHED = 0, MQ_PATH_PLAIN = 1 };
+
+/* Injection model. REFERENCE follows Physics.legacy_injection exactly as
+ * Engine::inject (channel.rs:342-374): legacy 10-degree cone on axis, or a
+ * thermal disc with flux-weighted axial speed. GAUSSIAN: Gaussian spot of
+ * sigma inj_sigma_mm about the axis, Maxwell-Boltzmann velocity at temp_k. */
+enum { MQ_INJ_REFERENCE = 0, MQ_INJ_GAUSSIAN = 1 };
+
+/* Species (channel.rs:236): mass [Da], charge number, N2 CCS [A^2]. */
+typedef struct { double mass, z, ccs_n2; } mq_species;
+
+/* RF + AC drive of one job (channel.rs:247). Volts and Hz. */
+typedef struct { double v_rf, f_rf, v_ac, f_ac; } mq_rfac;
+
+/* One scan point (channel.rs:252). Extensions for setting sweeps: dc_scale
+ * multiplies the batch's combined DC field for this job only (1.0 reproduces
+ * the reference, which shares DC per batch); pressure_pa > 0 overrides the
+ * batch pressure for this job; per-job v_rf/f_rf are allowed (the reference
+ * asserts a shared RF, channel.rs:446). */
+typedef struct {
+    mq_species sp;
+    mq_rfac drive;
+    uint64_t n;
+    uint64_t seed;
+    double dc_scale;
+    double pressure_pa;
+} mq_job;
+
+/* Physics switches (channel.rs:255-259) plus the injection model and an
+ * explicit step cap. max_steps = 0 means round(t_max / dt). */
+typedef struct {
+    int32_t legacy_force;       /* 1: half kick 0.5*q/m (their 1/2 a dt) */
+    int32_t legacy_collisions;  /* 1: stationary-gas countdown collisions  */
+    int32_t legacy_fields;      /* 1: field of the lower-corner node, no interpolation */
+    int32_t legacy_injection;   /* 1: legacy cone injection (MQ_INJ_REFERENCE only) */
+    int32_t legacy_criteria;    /* 1: legacy return/exit planes and rod-only crash window */
+    int32_t injection;          /* MQ_INJ_* */
+    double pressure_pa, temp_k, dt, t_max, r_inject_mm;
+    double inj_sigma_mm;
+    uint64_t max_steps;
+} mq_physics;
+
+/* Per-job outcome counts (channel.rs:297). mean_time in s. */
+typedef struct {
+    uint64_t n, crashed, returned, transmitted, left, timeout;
+    double mean_coll, mean_time;
+} mq_outcome;
+
+/* Basis E-field grids, E per volt, node-major index i + nx*(j + ny*k), node
+ * (i,j,k) at (i*h, j*h, k*h) [m]. Fields (channel.rs:122-129): RF, AC, four DC
+ * bases (q1, plate, downstream, box) of which any may be NULL (zero), and the
+ * conductor mask (RF .w in the reference). Plane and window positions are the
+ * values channel.rs:397-410 derives from its geometry. */
+typedef struct {
+    int32_t nx, ny, nz;
+    double h;
+    double axis_x, axis_y, z_inject;
+    double z_return, z_exit;                 /* correct criteria */
+    double z_return_legacy, z_exit_legacy;   /* legacy criteria */
+    double crash_z0_legacy, crash_z1_legacy; /* legacy crash window (rods) */
+    const double* rf;      /* 3*N */
+    const double* ac;      /* 3*N */
+    const double* dc[4];   /* 3*N each, or NULL */
+    const uint8_t* mask;   /* N, nonzero = conductor node */
+} mq_fields;
+
+/* Final ion states, structure of arrays, caller-owned, n entries each. */
+typedef struct {
+    double *x, *y, *z;
+    double *vx, *vy, *vz;
+    uint32_t *steps;   /* steps executed (time = steps * dt) */
+    uint32_t *ncoll;   /* collisions */
+    uint8_t *status;   /* MQ_* status */
+} mq_ion_soa;
+
+/* Statistics of the last run on an engine. */
+typedef struct {
+    uint64_t ion_steps;      /* sum over ions of steps executed */
+    uint64_t n_ions;
+    uint32_t launches;       /* worker / direct kernel launches */
+    double kernel_ms;        /* device time of the worker kernels (HIP events) */
+    double wall_ms;          /* host wall time of the call */
+} mq_run_stats;
+
+/* Tuning of the batched path (TileCfg, tiled.rs:12-29). Zero fields keep
+ * the current value. */
+typedef struct {
+    int32_t block;           /* threads per workgroup (64..1024, multiple of 64) */
+    int32_t waves_per_cu;    /* persistent waves launched per CU */
+    int32_t refill_period;   /* steps between idle-lane refills (compaction period) */
+    int32_t chunk;           /* max steps per ion per launch (doubles per launch) */
+    int32_t chunk_cap;       /* chunk growth cap */
+    int32_t resort;          /* 1: sort the live list by grid cell between launches */
+} mq_tune;
+
+typedef struct mq_engine mq_engine;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* mq_last_error(void);
+
+/* Replaces Engine::new (channel.rs:327-340): selects the GPU, uploads the RF,
+ * AC and DC basis grids and the conductor mask once. */
+int mq_engine_create(const mq_fields* fields, mq_engine** out);
+
+/* Drops the engine and all device memory it owns. NULL is a no-op. */
+void mq_engine_destroy(mq_engine* eng);
+
+/* Replaces Engine::run_batch (channel.rs:437-440): runs every job of the batch
+ * to completion and writes one outcome per job into out[njobs]. dc[4] are the
+ * voltages of the four DC bases, shared by the batch. */
+int mq_run_batch(mq_engine* eng, const mq_job* jobs, size_t njobs, const double dc[4],
+                 const mq_physics* ph, int32_t path, mq_outcome* out);
+
+/* Replaces Engine::run_batch_ions (channel.rs:458-475): as mq_run_batch but
+ * returns the final state of every ion (concatenated in job order, sum of
+ * jobs[k].n entries) and the job index of each ion (out_job may be NULL). */
+int mq_run_batch_ions(mq_engine* eng, const mq_job* jobs, size_t njobs, const double dc[4],
+                      const mq_physics* ph, int32_t path, const mq_ion_soa* out, uint32_t* out_job);
+
+/* Replaces summarize (channel.rs:496-506): per-job counts and means from final
+ * states. dt converts steps to time. */
+int mq_summarize(const uint8_t* status, const uint32_t* ncoll, const uint32_t* steps,
+                 const uint32_t* job, size_t n, size_t njobs, double dt, mq_outcome* out);
+
+/* Statistics of the last run_batch / run_batch_ions call on this engine. */
+int mq_last_stats(const mq_engine* eng, mq_run_stats* out);
+
+/* Tuning of the batched path; replaces TileCfg::from_env (tiled.rs:24-28). */
+int mq_set_tune(mq_engine* eng, const mq_tune* t);
+int mq_get_tune(const mq_engine* eng, mq_tune* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MQ_ENGINE_H */

@@ -1,0 +1,549 @@
+// engine.cpp -- host side of the MI355X batched ion-trajectory integrator and
+// its C ABI (include/mq_engine.h).
+//
+// Mirrors chsim::channel::Engine (simulation/chsim/src/channel.rs:309-506) and
+// the tiled launch loop (simulation/chsim/src/tiled.rs:87-218): the engine owns
+// the device, one HIP stream, the uploaded basis grids, a cache of the last DC
+// configuration (combined FP64 field + FP32 cell table) and pooled per-call
+// buffers. Every call is synchronous; an engine is used from one thread.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mq_engine.h"
+#include "kernels.h"
+#include "mq_types.h"
+
+using namespace mq;
+
+namespace {
+
+constexpr double AMU = 1.6605390666e-27;
+constexpr double QE = 1.602176634e-19;
+constexpr double KB = 1.380649e-23;
+
+thread_local std::string g_err;
+
+struct Err : std::runtime_error {
+    int code;
+    Err(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                                     \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            throw Err(-3, std::string("HIP error ") + hipGetErrorString(e_) + " at " #expr);             \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t b)
+    {
+        if (b <= bytes) return;
+        release();
+        size_t nb = std::max<size_t>(b, 256);
+        HIPCHK(hipMalloc(&p, nb));
+        bytes = nb;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// channel.rs:376-381
+double legacy_mfp_mean(const mq_species& sp, double pressure_pa)
+{
+    double p_torr = pressure_pa / 133.322 + 1e-10;
+    double np = 2.68e5 * (p_torr / 760.0);
+    double mm = sp.mass;
+    double ccs_he = 2.81e-9 * mm * mm * mm - 3.55e-5 * mm * mm + 0.232 * mm + 41.91;
+    double ccs = ccs_he * 10.0 * std::sqrt(sp.z);
+    return 1.0 / (np * ccs);
+}
+
+// Engine::job_params (channel.rs:412-426), FP64.
+JobP job_params(const mq_job& j, const mq_physics& ph, uint64_t off)
+{
+    JobP J{};
+    double m = j.sp.mass * AMU, mg = 28.0 * AMU;
+    double qm = j.sp.z * QE / m;
+    double p = j.pressure_pa > 0.0 ? j.pressure_pa : ph.pressure_pa;
+    double n_gas = p / (KB * ph.temp_k);
+    double s_gas = std::sqrt(KB * ph.temp_k / mg);
+    J.m = m;
+    J.zc = j.sp.z;
+    J.kq = (ph.legacy_force ? 0.5 : 1.0) * qm * ph.dt;
+    J.v_rf = j.drive.v_rf;
+    J.w_rf = j.drive.f_rf * ph.dt;
+    J.v_ac = j.drive.v_ac;
+    J.w_ac = j.drive.f_ac * ph.dt;
+    J.s_dc = j.dc_scale;
+    J.nsig_dt = n_gas * j.sp.ccs_n2 * 1e-20 * ph.dt;
+    J.s_gas = s_gas;
+    J.v_bound = 6.0 * s_gas;
+    J.mr = mg / (m + mg);
+    J.one_m_mr = 1.0 - J.mr;
+    J.floor_v = ph.legacy_collisions ? std::sqrt(2.0 * 0.035 * 1.6e-19 / (j.sp.mass * 1.66e-27)) : -1.0;
+    J.mfp_mean = legacy_mfp_mean(j.sp, p);
+    J.key0 = (uint32_t)j.seed;
+    J.key1 = (uint32_t)(j.seed >> 32);
+    J.off = off;
+    return J;
+}
+
+uint32_t max_steps_of(const mq_physics& ph)
+{
+    uint64_t m = ph.max_steps ? ph.max_steps : (uint64_t)std::llround(ph.t_max / ph.dt);
+    if (m > 0xFFFFFFF0ull) m = 0xFFFFFFF0ull;
+    if (m < 1) m = 1;
+    return (uint32_t)m;
+}
+
+}  // namespace
+
+struct mq_engine {
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int nx = 0, ny = 0, nz = 0;
+    double h = 0;
+    mq_fields meta{};
+    DevBuf rf, ac, dcb[4], mask;
+    bool has_dc[4] = {false, false, false, false};
+    // cache of the last DC configuration
+    bool dc_valid = false;
+    double dc_key[4] = {0, 0, 0, 0};
+    DevBuf dcc;    // combined DC, FP64, 3N
+    bool cells_valid = false;
+    DevBuf cells;  // FP32 cell records
+    // per-call pools
+    DevBuf jobs, acc, list[2], ctr;
+    DevBuf sx, sy, sz, sphi, svx, svy, svz, sfx, sfy, sfz, ssteps, sncoll, scd, sjob, sstatus;
+    mq_tune tune{256, 8, 8, 20000, 200000, 0};
+    mq_run_stats stats{};
+
+    size_t nodes() const { return (size_t)nx * ny * nz; }
+
+    void create(const mq_fields* f)
+    {
+        if (!f) throw Err(-1, "fields is NULL");
+        if (f->nx < 2 || f->ny < 2 || f->nz < 2) throw Err(-1, "grid needs at least 2 nodes per axis");
+        if (!(f->h > 0.0)) throw Err(-1, "grid spacing h must be positive");
+        if (!f->rf || !f->ac) throw Err(-1, "RF and AC basis grids are required");
+        if (3ull * (size_t)f->nx * f->ny * f->nz >= (1ull << 31)) throw Err(-1, "grid too large (3*N must be < 2^31)");
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        if (ndev < 1) throw Err(-2, "no HIP device");
+        HIPCHK(hipGetDevice(&device));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        cus = prop.multiProcessorCount;
+        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&ev0));
+        HIPCHK(hipEventCreate(&ev1));
+        nx = f->nx; ny = f->ny; nz = f->nz; h = f->h;
+        meta = *f;
+        meta.rf = meta.ac = nullptr;
+        for (auto& p : meta.dc) p = nullptr;
+        meta.mask = nullptr;
+        const size_t n3 = 3 * nodes() * sizeof(double);
+        rf.ensure(n3);
+        HIPCHK(hipMemcpyAsync(rf.p, f->rf, n3, hipMemcpyHostToDevice, stream));
+        ac.ensure(n3);
+        HIPCHK(hipMemcpyAsync(ac.p, f->ac, n3, hipMemcpyHostToDevice, stream));
+        for (int b = 0; b < 4; ++b) {
+            if (!f->dc[b]) continue;
+            has_dc[b] = true;
+            dcb[b].ensure(n3);
+            HIPCHK(hipMemcpyAsync(dcb[b].p, f->dc[b], n3, hipMemcpyHostToDevice, stream));
+        }
+        mask.ensure(nodes());
+        if (f->mask) HIPCHK(hipMemcpyAsync(mask.p, f->mask, nodes(), hipMemcpyHostToDevice, stream));
+        else HIPCHK(hipMemsetAsync(mask.p, 0, nodes(), stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    void destroy()
+    {
+        for (DevBuf* b : {&rf, &ac, &dcb[0], &dcb[1], &dcb[2], &dcb[3], &mask, &dcc, &cells, &jobs, &acc, &list[0],
+                          &list[1], &ctr, &sx, &sy, &sz, &sphi, &svx, &svy, &svz, &sfx, &sfy, &sfz, &ssteps,
+                          &sncoll, &scd, &sjob, &sstatus})
+            b->release();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // Combined DC field (Engine::dc_buffer, channel.rs:384-395) and, for the
+    // batched path, its FP32 cell table (brick_buffer, tiled.rs:65-84).
+    void prepare_dc(const double dc[4], bool want_cells)
+    {
+        bool same = dc_valid;
+        for (int b = 0; b < 4; ++b) same = same && dc_key[b] == dc[b];
+        if (!same) {
+            dcc.ensure(3 * nodes() * sizeof(double));
+            const double* d[4];
+            double v[4];
+            for (int b = 0; b < 4; ++b) {
+                d[b] = has_dc[b] ? dcb[b].as<double>() : nullptr;
+                v[b] = dc[b];
+                dc_key[b] = dc[b];
+            }
+            launch_combine_dc(dcc.as<double>(), d, v, (uint32_t)(3 * nodes()), stream);
+            HIPCHK(hipGetLastError());
+            dc_valid = true;
+            cells_valid = false;
+        }
+        if (want_cells && !cells_valid) {
+            const size_t nc = (size_t)(nx - 1) * (ny - 1) * (nz - 1);
+            cells.ensure(nc * CELL_F4 * sizeof(float4));
+            launch_build_cells(cells.as<float4>(), rf.as<double>(), dcc.as<double>(), ac.as<double>(),
+                               mask.as<uint8_t>(), nx, ny, nz, stream);
+            HIPCHK(hipGetLastError());
+            cells_valid = true;
+        }
+    }
+
+    IonState state_buffers(size_t n)
+    {
+        sx.ensure(n * 8); sy.ensure(n * 8); sz.ensure(n * 8); sphi.ensure(n * 8);
+        svx.ensure(n * 8); svy.ensure(n * 8); svz.ensure(n * 8);
+        sfx.ensure(n * 4); sfy.ensure(n * 4); sfz.ensure(n * 4);
+        ssteps.ensure(n * 4); sncoll.ensure(n * 4); scd.ensure(n * 4); sjob.ensure(n * 4); sstatus.ensure(n);
+        IonState S;
+        S.x = sx.as<double>(); S.y = sy.as<double>(); S.z = sz.as<double>(); S.phi = sphi.as<double>();
+        S.vx = svx.as<double>(); S.vy = svy.as<double>(); S.vz = svz.as<double>();
+        S.fvx = sfx.as<float>(); S.fvy = sfy.as<float>(); S.fvz = sfz.as<float>();
+        S.steps = ssteps.as<uint32_t>(); S.ncoll = sncoll.as<uint32_t>(); S.cd = scd.as<uint32_t>();
+        S.job = sjob.as<uint32_t>(); S.status = sstatus.as<uint8_t>();
+        return S;
+    }
+
+    BatchP batch_params(const mq_physics& ph, uint32_t njobs) const
+    {
+        BatchP B{};
+        B.dt = ph.dt;
+        B.h = h;
+        B.inv_h = 1.0 / h;
+        B.lim0 = (double)(nx - 1) * h;
+        B.lim1 = (double)(ny - 1) * h;
+        B.lim2 = (double)(nz - 1) * h;
+        B.z_ret = ph.legacy_criteria ? meta.z_return_legacy : meta.z_return;
+        B.z_exit = ph.legacy_criteria ? meta.z_exit_legacy : meta.z_exit;
+        B.cz0 = ph.legacy_criteria ? meta.crash_z0_legacy : -1e30;
+        B.cz1 = ph.legacy_criteria ? meta.crash_z1_legacy : 1e30;
+        B.nx = nx; B.ny = ny; B.nz = nz;
+        B.nearest = ph.legacy_fields ? 1 : 0;
+        B.legacy_coll = ph.legacy_collisions ? 1 : 0;
+        B.max_steps = max_steps_of(ph);
+        B.njobs = njobs;
+        return B;
+    }
+
+    // One batch: inject, run to completion on `path`, summarize. Returns the
+    // device ion state (valid until the next call).
+    IonState run(const mq_job* jobs_h, size_t njobs, const double dc[4], const mq_physics& ph, int path,
+                 std::vector<unsigned long long>& acc_h, size_t& n_total)
+    {
+        auto t_wall = std::chrono::steady_clock::now();
+        if (!jobs_h || njobs == 0) throw Err(-1, "batch has no jobs");
+        if (!(ph.dt > 0.0)) throw Err(-1, "dt must be positive");
+        if (path != MQ_PATH_BATCHED && path != MQ_PATH_PLAIN) throw Err(-1, "unknown path");
+        std::vector<JobP> jp(njobs);
+        uint64_t off = 0;
+        for (size_t k = 0; k < njobs; ++k) {
+            jp[k] = job_params(jobs_h[k], ph, off);
+            off += jobs_h[k].n;
+        }
+        if (off >= 0xFFFFFF00ull) throw Err(-1, "too many ions in one batch (limit 2^32 - 256)");
+        const uint32_t n = (uint32_t)off;
+        n_total = n;
+        jobs.ensure(njobs * sizeof(JobP));
+        HIPCHK(hipMemcpyAsync(jobs.p, jp.data(), njobs * sizeof(JobP), hipMemcpyHostToDevice, stream));
+        acc.ensure(njobs * ACC * sizeof(unsigned long long));
+        launch_fill_u64(acc.as<unsigned long long>(), 0ull, (uint32_t)(njobs * ACC), stream);
+        prepare_dc(dc, path == MQ_PATH_BATCHED);
+        IonState S = state_buffers(std::max<size_t>(n, 1));
+        InjP ip{};
+        ip.axis_x = meta.axis_x; ip.axis_y = meta.axis_y; ip.z_inject = meta.z_inject;
+        ip.temp_k = ph.temp_k; ip.r_inject_mm = ph.r_inject_mm; ip.inj_sigma_mm = ph.inj_sigma_mm; ip.dt = ph.dt;
+        ip.model = ph.injection == MQ_INJ_GAUSSIAN ? 2 : (ph.legacy_injection ? 1 : 0);
+        launch_inject(S, jobs.as<JobP>(), (uint32_t)njobs, ip, n, stream);
+        HIPCHK(hipGetLastError());
+        const BatchP B = batch_params(ph, (uint32_t)njobs);
+        stats = mq_run_stats{};
+        stats.n_ions = n;
+        float kms = 0.0f;
+        if (n) {
+            if (path == MQ_PATH_PLAIN) run_plain(S, B, n, kms);
+            else run_batched(S, B, n, kms);
+        }
+        launch_summarize(S, acc.as<unsigned long long>(), n, stream);
+        HIPCHK(hipGetLastError());
+        acc_h.resize(njobs * ACC);
+        HIPCHK(hipMemcpyAsync(acc_h.data(), acc.p, njobs * ACC * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        uint64_t steps = 0;
+        for (size_t k = 0; k < njobs; ++k) steps += acc_h[k * ACC + 6];
+        stats.ion_steps = steps;
+        stats.kernel_ms = kms;
+        stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
+        return S;
+    }
+
+    void run_plain(const IonState& S, const BatchP& B, uint32_t n, float& kms)
+    {
+        uint32_t nsteps = (uint32_t)std::max(1000.0, std::floor(2e-4 / B.dt));
+        nsteps = (nsteps + 3u) & ~3u;
+        ctr.ensure(256);
+        uint32_t* live = ctr.as<uint32_t>();
+        kms = 0.0f;
+        for (;;) {
+            launch_fill_u32(live, 0u, 1, stream);
+            HIPCHK(hipEventRecord(ev0, stream));
+            launch_plain(S, jobs.as<JobP>(), B, rf.as<double>(), dcc.as<double>(), ac.as<double>(),
+                         mask.as<uint8_t>(), n, nsteps, live, stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(ev1, stream));
+            stats.launches += 1;
+            uint32_t alive = 0;
+            HIPCHK(hipMemcpyAsync(&alive, live, 4, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+            kms += ms;
+            if (alive == 0) break;
+        }
+    }
+
+    void run_batched(const IonState& S, const BatchP& B, uint32_t n, float& kms)
+    {
+        list[0].ensure((size_t)n * 4);
+        list[1].ensure((size_t)n * 4);
+        ctr.ensure(256);
+        uint32_t* c0 = ctr.as<uint32_t>();
+        uint32_t* c1 = c0 + 16;
+        launch_iota(list[0].as<uint32_t>(), n, stream);
+        launch_fill_u32(c0, n, 2, stream);  // [1] = count
+        const uint32_t block = (uint32_t)tune.block;
+        const uint32_t grid = std::max<uint32_t>(1, (uint32_t)((uint64_t)cus * tune.waves_per_cu * 64 / block));
+        uint32_t chunk = ((uint32_t)tune.chunk + 3u) & ~3u;
+        const uint32_t cap = std::max(chunk, ((uint32_t)tune.chunk_cap + 3u) & ~3u);
+        const uint32_t period = std::max(4u, ((uint32_t)tune.refill_period + 3u) & ~3u);
+        kms = 0.0f;
+        int cur = 0;
+        for (;;) {
+            WorkerArgs a;
+            a.S = S;
+            a.jobs = jobs.as<JobP>();
+            a.B = B;
+            a.cells = cells.as<float4>();
+            a.list_in = list[cur].as<uint32_t>();
+            a.list_out = list[cur ^ 1].as<uint32_t>();
+            a.ctr_in = cur ? c1 : c0;
+            a.ctr_out = cur ? c0 : c1;
+            a.chunk = chunk;
+            a.period = period;
+            launch_prepare(a, stream);
+            HIPCHK(hipEventRecord(ev0, stream));
+            launch_batched(a, grid, block, stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(ev1, stream));
+            stats.launches += 1;
+            uint32_t alive = 0;
+            HIPCHK(hipMemcpyAsync(&alive, a.ctr_out + 1, 4, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+            kms += ms;
+            if (alive == 0) break;
+            cur ^= 1;
+            chunk = std::min(cap, chunk * 2u);
+        }
+    }
+};
+
+// ------------------------------------------------------------------ C ABI
+
+#define MQ_GUARD(...)                                       \
+    try {                                                   \
+        g_err.clear();                                      \
+        __VA_ARGS__;                                        \
+        return 0;                                           \
+    } catch (const Err& e) {                                \
+        g_err = e.what();                                   \
+        return e.code;                                      \
+    } catch (const std::exception& e) {                     \
+        g_err = e.what();                                   \
+        return -9;                                          \
+    }
+
+extern "C" {
+
+const char* mq_last_error(void) { return g_err.c_str(); }
+
+int mq_engine_create(const mq_fields* fields, mq_engine** out)
+{
+    mq_engine* e = nullptr;
+    try {
+        g_err.clear();
+        if (!out) throw Err(-1, "out is NULL");
+        e = new mq_engine();
+        e->create(fields);
+        *out = e;
+        return 0;
+    } catch (const Err& x) {
+        g_err = x.what();
+        if (e) { e->destroy(); delete e; }
+        return x.code;
+    } catch (const std::exception& x) {
+        g_err = x.what();
+        if (e) { e->destroy(); delete e; }
+        return -9;
+    }
+}
+
+void mq_engine_destroy(mq_engine* eng)
+{
+    if (!eng) return;
+    eng->destroy();
+    delete eng;
+}
+
+static void fill_outcomes(const std::vector<unsigned long long>& a, size_t njobs, double dt, mq_outcome* out)
+{
+    for (size_t k = 0; k < njobs; ++k) {
+        const unsigned long long* p = &a[k * ACC];
+        mq_outcome o{};
+        o.n = p[0];
+        o.crashed = p[1]; o.returned = p[2]; o.transmitted = p[3]; o.left = p[4]; o.timeout = p[5];
+        o.mean_coll = o.n ? (double)p[7] / (double)o.n : 0.0;
+        o.mean_time = o.n ? (double)p[6] * dt / (double)o.n : 0.0;
+        out[k] = o;
+    }
+}
+
+int mq_run_batch(mq_engine* eng, const mq_job* jobs, size_t njobs, const double dc[4], const mq_physics* ph,
+                 int32_t path, mq_outcome* out)
+{
+    MQ_GUARD({
+        if (!eng || !ph || !dc || !out) throw Err(-1, "NULL argument");
+        std::vector<unsigned long long> acc;
+        size_t n = 0;
+        eng->run(jobs, njobs, dc, *ph, path, acc, n);
+        fill_outcomes(acc, njobs, ph->dt, out);
+    })
+}
+
+int mq_run_batch_ions(mq_engine* eng, const mq_job* jobs, size_t njobs, const double dc[4], const mq_physics* ph,
+                      int32_t path, const mq_ion_soa* out, uint32_t* out_job)
+{
+    MQ_GUARD({
+        if (!eng || !ph || !dc || !out) throw Err(-1, "NULL argument");
+        std::vector<unsigned long long> acc;
+        size_t n = 0;
+        IonState S = eng->run(jobs, njobs, dc, *ph, path, acc, n);
+        if (n == 0) return 0;
+        hipStream_t st = eng->stream;
+        auto d2h = [&](void* dst, const void* src, size_t bytes) {
+            if (dst) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        };
+        d2h(out->x, S.x, n * 8); d2h(out->y, S.y, n * 8); d2h(out->z, S.z, n * 8);
+        d2h(out->steps, S.steps, n * 4); d2h(out->ncoll, S.ncoll, n * 4); d2h(out->status, S.status, n);
+        if (out_job) d2h(out_job, S.job, n * 4);
+        if (path == MQ_PATH_PLAIN) {
+            d2h(out->vx, S.vx, n * 8); d2h(out->vy, S.vy, n * 8); d2h(out->vz, S.vz, n * 8);
+            HIPCHK(hipStreamSynchronize(st));
+        } else {
+            std::vector<float> f(n);
+            double* dsts[3] = {out->vx, out->vy, out->vz};
+            float* srcs[3] = {S.fvx, S.fvy, S.fvz};
+            for (int d = 0; d < 3; ++d) {
+                if (!dsts[d]) continue;
+                HIPCHK(hipMemcpyAsync(f.data(), srcs[d], n * 4, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                for (size_t i = 0; i < n; ++i) dsts[d][i] = (double)f[i];
+            }
+            HIPCHK(hipStreamSynchronize(st));
+        }
+    })
+}
+
+int mq_summarize(const uint8_t* status, const uint32_t* ncoll, const uint32_t* steps, const uint32_t* job, size_t n,
+                 size_t njobs, double dt, mq_outcome* out)
+{
+    MQ_GUARD({
+        if (!out || (n && (!status || !ncoll || !steps || !job))) throw Err(-1, "NULL argument");
+        std::vector<unsigned long long> a(njobs * ACC, 0ull);
+        for (size_t i = 0; i < n; ++i) {
+            if (job[i] >= njobs) throw Err(-1, "job index out of range");
+            unsigned long long* p = &a[(size_t)job[i] * ACC];
+            p[0] += 1;
+            uint8_t s = status[i];
+            p[(s >= 1 && s <= 5) ? s : 5] += 1;
+            p[6] += steps[i];
+            p[7] += ncoll[i];
+        }
+        fill_outcomes(a, njobs, dt, out);
+    })
+}
+
+int mq_last_stats(const mq_engine* eng, mq_run_stats* out)
+{
+    MQ_GUARD({
+        if (!eng || !out) throw Err(-1, "NULL argument");
+        *out = eng->stats;
+    })
+}
+
+int mq_set_tune(mq_engine* eng, const mq_tune* t)
+{
+    MQ_GUARD({
+        if (!eng || !t) throw Err(-1, "NULL argument");
+        mq_tune n = eng->tune;
+        if (t->block) {
+            if (t->block < 64 || t->block > 1024 || t->block % 64) throw Err(-1, "block must be a multiple of 64 in [64, 1024]");
+            n.block = t->block;
+        }
+        if (t->waves_per_cu) {
+            if (t->waves_per_cu < 1 || t->waves_per_cu > 32) throw Err(-1, "waves_per_cu must be in [1, 32]");
+            n.waves_per_cu = t->waves_per_cu;
+        }
+        if (t->refill_period) {
+            if (t->refill_period < 4 || t->refill_period > 4096) throw Err(-1, "refill_period must be in [4, 4096]");
+            n.refill_period = t->refill_period;
+        }
+        if (t->chunk) {
+            if (t->chunk < 4) throw Err(-1, "chunk must be >= 4");
+            n.chunk = t->chunk;
+        }
+        if (t->chunk_cap) n.chunk_cap = t->chunk_cap;
+        n.resort = t->resort;
+        eng->tune = n;
+    })
+}
+
+int mq_get_tune(const mq_engine* eng, mq_tune* out)
+{
+    MQ_GUARD({
+        if (!eng || !out) throw Err(-1, "NULL argument");
+        *out = eng->tune;
+    })
+}
+
+}  // extern "C"

@@ -1,0 +1,38 @@
+// kernels.h -- host launchers of the HIP kernels (one stream per call).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mq_types.h"
+
+namespace mq {
+
+void launch_inject(const IonState& S, const JobP* jobs, uint32_t njobs, const InjP& P, uint32_t n, hipStream_t st);
+void launch_combine_dc(double* out, const double* const d[4], const double v[4], uint32_t n3, hipStream_t st);
+void launch_plain(const IonState& S, const JobP* jobs, const BatchP& B, const double* rf, const double* dc,
+                  const double* ac, const uint8_t* mask, uint32_t n, uint32_t nsteps, uint32_t* live, hipStream_t st);
+void launch_summarize(const IonState& S, unsigned long long* acc, uint32_t n, hipStream_t st);
+void launch_fill_u64(unsigned long long* p, unsigned long long v, uint32_t n, hipStream_t st);
+void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t st);
+
+// Batched path.
+void launch_build_cells(float4* cells, const double* rf, const double* dc, const double* ac, const uint8_t* mask,
+                        int nx, int ny, int nz, hipStream_t st);
+void launch_iota(uint32_t* list, uint32_t n, hipStream_t st);
+
+// Counters block used by one worker launch: [0] head, [1] count_in, [2] count_out.
+struct WorkerArgs {
+    IonState S;
+    const JobP* jobs;
+    BatchP B;
+    const float4* cells;
+    const uint32_t* list_in;
+    uint32_t* list_out;
+    uint32_t* ctr_in;   // [0] head, [1] count
+    uint32_t* ctr_out;  // [1] count of ions carried to the next launch
+    uint32_t chunk;     // max steps per ion in this launch (multiple of 4)
+    uint32_t period;    // steps between refills (multiple of 4)
+};
+void launch_prepare(const WorkerArgs& a, hipStream_t st);
+void launch_batched(const WorkerArgs& a, uint32_t grid, uint32_t block, hipStream_t st);
+}  // namespace mq

@@ -1,0 +1,346 @@
+// kernels_batched.hip -- batched FP32 path: per-cell coefficient tables and the
+// persistent wavefront worker.
+//
+// Counterparts in the reference: tc_brick (channel_tiled.metal:54-76), the
+// persistent worker tc_tail (:386-482) with the register corner cache
+// (:97-124), and the live-list compaction tc_filter / tc_size_args (:341-374).
+//
+// Design for CDNA4:
+//  * Fields: one 304-byte record per grid cell holding the trilinear polynomial
+//    coefficients of the RF, combined-DC and AC bases (72 FP32) and the
+//    conductor bits of its 8 corners. A cell change costs 19 contiguous 16-byte
+//    loads; the record stays in VGPRs while the ion stays in the cell.
+//  * Precision: FP32 field values, interpolation, kick and collision sampling;
+//    FP64 position, drift, time (step counter) and RF/AC phase.
+//  * Scheduling: each wavefront keeps 64 ions in flight; every `period` steps
+//    the lanes whose ion finished (or used up the launch's chunk) are refilled
+//    from the live list with one atomic per wavefront (ballot + popcount
+//    prefix), and ions that used up the chunk are appended to the next live
+//    list the same way.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "mq_math.h"
+#include "mq_types.h"
+
+namespace mq {
+
+// ------------------------------------------------------------- cell table
+__global__ void k_build_cells(float4* __restrict__ cells, const double* __restrict__ rf,
+                              const double* __restrict__ dc, const double* __restrict__ ac,
+                              const uint8_t* __restrict__ mask, int nx, int ny, int nz)
+{
+    const uint32_t cx = (uint32_t)(nx - 1), cy = (uint32_t)(ny - 1), cz = (uint32_t)(nz - 1);
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cx * cy * cz) return;
+    const uint32_t i = c % cx, j = (c / cx) % cy, k = c / (cx * cy);
+    const uint32_t sy = (uint32_t)nx, sz = (uint32_t)nx * (uint32_t)ny;
+    const uint32_t b = i + sy * (j + (uint32_t)ny * k);
+    const uint32_t node[8] = {b, b + 1, b + sy, b + 1 + sy, b + sz, b + 1 + sz, b + sy + sz, b + 1 + sy + sz};
+    float out[76];
+    const double* F[3] = {rf, dc, ac};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            // corner f_abc, a = dx, b = dy, c = dz
+            const double f000 = F[q][3 * node[0] + d], f100 = F[q][3 * node[1] + d];
+            const double f010 = F[q][3 * node[2] + d], f110 = F[q][3 * node[3] + d];
+            const double f001 = F[q][3 * node[4] + d], f101 = F[q][3 * node[5] + d];
+            const double f011 = F[q][3 * node[6] + d], f111 = F[q][3 * node[7] + d];
+            float* o = out + 24 * q + d;
+            o[0] = (float)f000;
+            o[3] = (float)(f100 - f000);
+            o[6] = (float)(f010 - f000);
+            o[9] = (float)(f001 - f000);
+            o[12] = (float)((f110 - f100) - (f010 - f000));
+            o[15] = (float)((f011 - f010) - (f001 - f000));
+            o[18] = (float)((f101 - f100) - (f001 - f000));
+            o[21] = (float)(((f111 - f110) - (f101 - f100)) - ((f011 - f010) - (f001 - f000)));
+        }
+    }
+    uint32_t bits = 0;
+    if (mask) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) bits |= (mask[node[q]] ? 1u : 0u) << q;
+    }
+    out[72] = __uint_as_float(bits);
+    out[73] = out[74] = out[75] = 0.0f;
+    float4* dst = cells + (size_t)c * CELL_F4;
+#pragma unroll
+    for (int q = 0; q < CELL_F4; ++q) dst[q] = make_float4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+}
+
+__global__ void k_iota(uint32_t* list, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) list[i] = i;
+}
+
+// ctr_in[0] = head of the current launch, ctr_out[0..1] = next launch's head and count.
+__global__ void k_prepare(uint32_t* ctr_in, uint32_t* ctr_out)
+{
+    if (threadIdx.x == 0) {
+        ctr_in[0] = 0;
+        ctr_out[0] = 0;
+        ctr_out[1] = 0;
+    }
+}
+
+// ------------------------------------------------------------- the worker
+struct Lane {
+    double x, y, z, phi, w_rf, w_ac;
+    float vx, vy, vz;
+    float kq, vrf, vac, sdc, nsig, sgas, vb, mr, omr, floorv, mfpm;
+    uint32_t st, ncoll, cd, k0, k1, ion, id, visit;
+    float tx, ty, tz, fx, fy, fz, a_rf, a_ac;
+    uint32_t cw1, cw2, cw3;
+    int32_t cell;
+};
+
+__device__ __forceinline__ float sin_rev(float p) { return __builtin_amdgcn_sinf(p); }
+__device__ __forceinline__ float cos_rev(float p) { return __builtin_amdgcn_cosf(p); }
+__device__ __forceinline__ float log_f(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945f; }
+
+__device__ __forceinline__ int32_t locate(const BatchP& B, Lane& L)
+{
+    double gx = fmin(fmax(L.x * B.inv_h, 0.0), (double)(B.nx - 1));
+    double gy = fmin(fmax(L.y * B.inv_h, 0.0), (double)(B.ny - 1));
+    double gz = fmin(fmax(L.z * B.inv_h, 0.0), (double)(B.nz - 1));
+    int i = min((int)gx, B.nx - 2), j = min((int)gy, B.ny - 2), k = min((int)gz, B.nz - 2);
+    L.fx = (float)(gx - (double)i);
+    L.fy = (float)(gy - (double)j);
+    L.fz = (float)(gz - (double)k);
+    L.tx = B.nearest ? 0.0f : L.fx;
+    L.ty = B.nearest ? 0.0f : L.fy;
+    L.tz = B.nearest ? 0.0f : L.fz;
+    return i + (B.nx - 1) * (j + (B.ny - 1) * k);
+}
+
+__device__ __forceinline__ void load_cell(const float4* __restrict__ cells, int32_t c, float4 R[CELL_F4])
+{
+    const float4* src = cells + (size_t)c * CELL_F4;
+#pragma unroll
+    for (int q = 0; q < CELL_F4; ++q) R[q] = src[q];
+}
+
+__device__ __forceinline__ float rget(const float4 R[CELL_F4], int k)
+{
+    const float4 v = R[k >> 2];
+    return (k & 3) == 0 ? v.x : ((k & 3) == 1 ? v.y : ((k & 3) == 2 ? v.z : v.w));
+}
+
+__device__ __forceinline__ float poly(const float4 R[CELL_F4], int base, float tx, float ty, float tz, float pxy,
+                                      float pyz, float pxz, float pxyz)
+{
+    float f = rget(R, base);
+    f = fmaf(tx, rget(R, base + 3), f);
+    f = fmaf(ty, rget(R, base + 6), f);
+    f = fmaf(tz, rget(R, base + 9), f);
+    f = fmaf(pxy, rget(R, base + 12), f);
+    f = fmaf(pyz, rget(R, base + 15), f);
+    f = fmaf(pxz, rget(R, base + 18), f);
+    f = fmaf(pxyz, rget(R, base + 21), f);
+    return f;
+}
+
+__device__ __forceinline__ void phases(Lane& L)
+{
+    double pr = L.w_rf * (double)L.st;
+    pr = pr - floor(pr);
+    double pa = fma(L.w_ac, (double)L.st, L.phi);
+    pa = pa - floor(pa);
+    L.a_rf = L.vrf * sin_rev((float)pr);
+    L.a_ac = L.vac * sin_rev((float)pa);
+}
+
+__device__ __forceinline__ uint32_t countdown_f(float x)
+{
+    if (!(x < 2147483520.0f)) return 2147483647u;
+    if (x < 1.0f) return 1u;
+    return (uint32_t)x;
+}
+
+__global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
+{
+    const BatchP& B = A.B;
+    const IonState& S = A.S;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t count = A.ctr_in[1];
+    const double dt = B.dt;
+    const float dtf = (float)B.dt;
+    Lane L;
+    float4 R[CELL_F4];
+    bool have = false, drained = false;
+    L.cell = -1;
+    for (;;) {
+        // ---- refill idle lanes from the live list (one atomic per wavefront)
+        const uint64_t idle = __ballot(!have);
+        if (idle && !drained) {
+            const uint32_t nidle = (uint32_t)__popcll(idle);
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&A.ctr_in[0], nidle);
+            base = __shfl(base, (int)leader);
+            if (base + nidle >= count) drained = true;
+            if (!have) {
+                const uint32_t slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (slot < count) {
+                    const uint32_t id = A.list_in[slot];
+                    const JobP& J = A.jobs[S.job[id]];
+                    L.id = id;
+                    L.ion = (uint32_t)(id - J.off);
+                    L.x = S.x[id]; L.y = S.y[id]; L.z = S.z[id]; L.phi = S.phi[id];
+                    L.vx = S.fvx[id]; L.vy = S.fvy[id]; L.vz = S.fvz[id];
+                    L.st = S.steps[id]; L.ncoll = S.ncoll[id]; L.cd = S.cd[id];
+                    L.w_rf = J.w_rf; L.w_ac = J.w_ac;
+                    L.kq = (float)J.kq; L.vrf = (float)J.v_rf; L.vac = (float)J.v_ac; L.sdc = (float)J.s_dc;
+                    L.nsig = (float)J.nsig_dt; L.sgas = (float)J.s_gas; L.vb = (float)J.v_bound;
+                    L.mr = (float)J.mr; L.omr = (float)J.one_m_mr; L.floorv = (float)J.floor_v;
+                    L.mfpm = (float)J.mfp_mean;
+                    L.k0 = J.key0; L.k1 = J.key1;
+                    L.visit = 0;
+                    L.cw1 = L.cw2 = L.cw3 = 0;
+                    const int32_t c = locate(B, L);
+                    if (c != L.cell) { L.cell = c; load_cell(A.cells, c, R); }
+                    phases(L);
+                    have = true;
+                }
+            }
+        }
+        if (!__any(have)) break;
+        // ---- advance every held ion by up to `period` steps
+        bool stop = false;
+        int status = 0;
+        for (uint32_t r = 0; r < A.period; ++r) {
+            if (have && !stop) {
+                const float pxy = L.tx * L.ty, pyz = L.ty * L.tz, pxz = L.tx * L.tz, pxyz = pxy * L.tz;
+                float e[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const float er = poly(R, d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+                    const float ed = poly(R, 24 + d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+                    const float ea = poly(R, 48 + d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+                    e[d] = fmaf(L.a_ac, ea, fmaf(L.sdc, ed, L.a_rf * er));
+                }
+                L.vx = fmaf(L.kq, e[0], L.vx);
+                L.vy = fmaf(L.kq, e[1], L.vy);
+                L.vz = fmaf(L.kq, e[2], L.vz);
+                L.x = fma((double)L.vx, dt, L.x);
+                L.y = fma((double)L.vy, dt, L.y);
+                L.z = fma((double)L.vz, dt, L.z);
+                const uint32_t cur = L.st;
+                L.st += 1;
+                // next-step field cell (hoisted above the collision code: it depends on x only)
+                const int32_t c = locate(B, L);
+                if (c != L.cell) { L.cell = c; load_cell(A.cells, c, R); }
+                phases(L);
+                // ---- collisions
+                if (!B.legacy_coll) {
+                    if (L.nsig > 0.0f) {
+                        const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
+                        uint32_t cw;
+                        const uint32_t ph = cur & 3;
+                        if (ph == 0 || L.visit == 0) {
+                            const u32x4 w = philox(L.ion, cur >> 2, 0, D_CAND, L.k0, L.k1);
+                            cw = pick(w, ph);
+                            L.cw1 = w.b; L.cw2 = w.c; L.cw3 = w.d;
+                        } else {
+                            cw = ph == 1 ? L.cw1 : (ph == 2 ? L.cw2 : L.cw3);
+                        }
+                        if (u01f(cw) < L.nsig * bound) {
+                            const u32x4 w = philox(L.ion, cur, 0, D_COLL, L.k0, L.k1);
+                            const float r1 = sqrtf(-2.0f * log_f(u01f(w.a))), r2 = sqrtf(-2.0f * log_f(u01f(w.c)));
+                            const float a1 = u01f(w.b), a2 = u01f(w.d);
+                            const float gx = L.vx - L.sgas * (r1 * cos_rev(a1));
+                            const float gy = L.vy - L.sgas * (r1 * sin_rev(a1));
+                            const float gz = L.vz - L.sgas * (r2 * cos_rev(a2));
+                            const float gl = sqrtf(fmaf(gz, gz, fmaf(gy, gy, gx * gx)));
+                            const u32x4 w2 = philox(L.ion, cur, 1, D_COLL, L.k0, L.k1);
+                            if (u01f(w2.a) * bound < gl) {
+                                const float cth = 2.0f * u01f(w2.b) - 1.0f, a3 = u01f(w2.c);
+                                const float sn = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
+                                const float mg = L.mr * gl;
+                                L.vx = fmaf(mg, sn * cos_rev(a3), fmaf(-L.mr, gx, L.vx));
+                                L.vy = fmaf(mg, sn * sin_rev(a3), fmaf(-L.mr, gy, L.vy));
+                                L.vz = fmaf(mg, cth, fmaf(-L.mr, gz, L.vz));
+                                L.ncoll += 1;
+                            }
+                        }
+                    }
+                } else {
+                    L.cd -= 1;
+                    if (L.cd == 0) {
+                        const u32x4 w = philox(L.ion, cur, 0, D_COLL, L.k0, L.k1);
+                        const float vm = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx)));
+                        if (vm > L.floorv) {
+                            const float cth = 2.0f * u01f(w.a) - 1.0f, a3 = u01f(w.b);
+                            const float sn = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
+                            const float mv = L.mr * vm;
+                            L.vx = fmaf(mv, sn * cos_rev(a3), L.vx * L.omr);
+                            L.vy = fmaf(mv, sn * sin_rev(a3), L.vy * L.omr);
+                            L.vz = fmaf(mv, cth, L.vz * L.omr);
+                            L.ncoll += 1;
+                        }
+                        const float mfp = -log_f(fmaxf(u01f(w.c), 1e-7f)) * L.mfpm;
+                        const float vm2 = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx)));
+                        L.cd = countdown_f(floorf(mfp / fmaxf(vm2, 1e-3f) / dtf));
+                    }
+                }
+                L.visit += 1;
+                // ---- classification (channel.metal:114-120)
+                if (L.z < B.z_ret) status = 2;
+                else if (L.z > B.z_exit) status = 3;
+                else if (L.x < 0.0 || L.y < 0.0 || L.z < 0.0 || L.x > B.lim0 || L.y > B.lim1 || L.z > B.lim2) status = 4;
+                else if (L.z > B.cz0 && L.z < B.cz1) {
+                    const uint32_t bits = __float_as_uint(R[18].x);
+                    const uint32_t corner = (L.fx >= 0.5f ? 1u : 0u) | (L.fy >= 0.5f ? 2u : 0u) | (L.fz >= 0.5f ? 4u : 0u);
+                    if ((bits >> corner) & 1u) status = 1;
+                }
+                if (status == 0 && L.st >= B.max_steps) status = 5;
+                if (status != 0 || L.visit >= A.chunk) stop = true;
+            }
+        }
+        // ---- write back stopped ions; carry unfinished ones to the next launch
+        const bool carry = have && stop && status == 0;
+        const uint64_t cm = __ballot(carry);
+        if (cm) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)cm) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&A.ctr_out[1], (uint32_t)__popcll(cm));
+            base = __shfl(base, (int)leader);
+            if (carry) A.list_out[base + (uint32_t)__popcll(cm & ((1ull << lane) - 1ull))] = L.id;
+        }
+        if (have && stop) {
+            const uint32_t id = L.id;
+            S.x[id] = L.x; S.y[id] = L.y; S.z[id] = L.z;
+            S.fvx[id] = L.vx; S.fvy[id] = L.vy; S.fvz[id] = L.vz;
+            S.steps[id] = L.st; S.ncoll[id] = L.ncoll; S.cd[id] = L.cd;
+            S.status[id] = (uint8_t)status;
+            have = false;
+        }
+    }
+}
+
+static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+void launch_build_cells(float4* cells, const double* rf, const double* dc, const double* ac, const uint8_t* mask,
+                        int nx, int ny, int nz, hipStream_t st)
+{
+    const uint64_t nc = (uint64_t)(nx - 1) * (ny - 1) * (nz - 1);
+    k_build_cells<<<nblk(nc, 128), 128, 0, st>>>(cells, rf, dc, ac, mask, nx, ny, nz);
+}
+
+void launch_iota(uint32_t* list, uint32_t n, hipStream_t st)
+{
+    if (n) k_iota<<<nblk(n, 256), 256, 0, st>>>(list, n);
+}
+
+void launch_prepare(const WorkerArgs& a, hipStream_t st) { k_prepare<<<1, 64, 0, st>>>(a.ctr_in, a.ctr_out); }
+
+void launch_batched(const WorkerArgs& a, uint32_t grid, uint32_t block, hipStream_t st)
+{
+    k_batched<<<grid, block, 0, st>>>(a);
+}
+
+}  // namespace mq

@@ -1,0 +1,308 @@
+// kernels_plain.hip -- FP64 "plain" path and the kernels both paths share:
+// injection, DC combination, summarize.
+//
+// The plain kernel is the FP64 form of chsim's direct per-ion kernel
+// (simulation/chsim/src/channel.metal:70-124, launched by Engine::run_direct,
+// channel.rs:478-493): one thread per ion, node-major FP64 basis grids read from
+// global memory every step, relaunched for nsteps until every ion has finished.
+// Built with -ffp-contract=off; fused multiply-adds are explicit fma().
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "mq_math.h"
+#include "mq_types.h"
+
+namespace mq {
+
+constexpr double QE = 1.602176634e-19;
+constexpr double KB = 1.380649e-23;
+
+__device__ __forceinline__ uint32_t job_of(const JobP* jobs, uint32_t njobs, uint32_t i)
+{
+    uint32_t lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].off <= (uint64_t)i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ double norm3(double a, double b, double c) { return sqrt((a * a + b * b) + c * c); }
+
+// ---------------------------------------------------------------- inject
+// Engine::inject (channel.rs:342-374) with the Philox stream: three blocks
+// (ion, 0, k, D_INJ), k = 0..2, give up to 12 uniforms.
+__global__ void k_inject(IonState S, const JobP* __restrict__ jobs, uint32_t njobs, InjP P, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = job_of(jobs, njobs, i);
+    const JobP J = jobs[j];
+    const uint32_t ion = (uint32_t)(i - J.off);
+    u32x4 w0 = philox(ion, 0, 0, D_INJ, J.key0, J.key1);
+    u32x4 w1 = philox(ion, 0, 1, D_INJ, J.key0, J.key1);
+    u32x4 w2 = philox(ion, 0, 2, D_INJ, J.key0, J.key1);
+    const double u[12] = {u01(w0.a), u01(w0.b), u01(w0.c), u01(w0.d), u01(w1.a), u01(w1.b),
+                          u01(w1.c), u01(w1.d), u01(w2.a), u01(w2.b), u01(w2.c), u01(w2.d)};
+    const double m = J.m, ax = P.axis_x, ay = P.axis_y, z0 = P.z_inject;
+    double x, y, z, vx, vy, vz, phi, ufirst, sn, cs;
+    if (P.model == 2) {
+        double sig = P.inj_sigma_mm * 1e-3, s = sqrt(KB * P.temp_k / m);
+        double r1 = sqrt(-2.0 * log_d(u[0]));
+        sincos2pi(u[1], sn, cs);
+        x = ax + sig * (r1 * cs);
+        y = ay + sig * (r1 * sn);
+        z = z0;
+        double r2 = sqrt(-2.0 * log_d(u[2]));
+        sincos2pi(u[3], sn, cs);
+        vx = s * (r2 * cs);
+        vy = s * (r2 * sn);
+        double r3 = sqrt(-2.0 * log_d(u[4]));
+        sincos2pi(u[5], sn, cs);
+        vz = s * (r3 * cs);
+        phi = u[6];
+        ufirst = u[7];
+    } else if (P.model == 1) {
+        // channel.rs:350-355: KE = 0.1 z (0.75 + 0.75 u) eV in a +-10 degree cone on axis
+        double kin = 0.1 * (0.75 + 0.75 * u[0]) * J.zc;
+        double vin = sqrt(2.0 * kin * QE / m);
+        sincos2pi((-10.0 + 20.0 * u[1]) / 360.0, sn, cs);
+        double vxy = 0.707 * vin * sn;
+        x = ax; y = ay; z = z0;
+        vx = vxy; vy = vxy; vz = vin * cs;
+        phi = u[2];
+        ufirst = u[3];
+    } else {
+        double s = sqrt(KB * P.temp_k / m);
+        double r = P.r_inject_mm * 1e-3 * sqrt(u[0]);
+        sincos2pi(u[1], sn, cs);
+        vz = s * sqrt(-2.0 * log_d(u[2]));
+        x = ax + r * cs;
+        y = ay + r * sn;
+        z = z0;
+        double c3, s3;
+        sincos2pi(u[4], s3, c3);
+        vx = s * (sqrt(-2.0 * log_d(u[3])) * c3);
+        sincos2pi(u[6], s3, c3);
+        vy = s * (sqrt(-2.0 * log_d(u[5])) * c3);
+        phi = u[7];
+        ufirst = u[8];
+    }
+    double vmag = norm3(vx, vy, vz);
+    double first_mfp = -log_d(fmax(ufirst, 1e-7)) * J.mfp_mean / 10.0;
+    S.x[i] = x; S.y[i] = y; S.z[i] = z; S.phi[i] = phi;
+    S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
+    S.fvx[i] = (float)vx; S.fvy[i] = (float)vy; S.fvz[i] = (float)vz;
+    S.steps[i] = 0; S.ncoll[i] = 0;
+    S.cd[i] = countdown(floor(first_mfp / vmag / P.dt) + 1.0);
+    S.job[i] = j;
+    S.status[i] = 0;
+}
+
+
+// ------------------------------------------------------- DC combination
+// Fields::dc_field (channel.rs:220-226) in FP64: e = sum_b v_b E_b, basis order.
+__global__ void k_combine_dc(double* __restrict__ out, const double* d0, const double* d1, const double* d2,
+                             const double* d3, double v0, double v1, double v2, double v3, uint32_t n3)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    double e = 0.0;
+    if (d0 && v0 != 0.0) e = e + v0 * d0[i];
+    if (d1 && v1 != 0.0) e = e + v1 * d1[i];
+    if (d2 && v2 != 0.0) e = e + v2 * d2[i];
+    if (d3 && v3 != 0.0) e = e + v3 * d3[i];
+    out[i] = e;
+}
+
+// ------------------------------------------------------ FP64 sampling
+__device__ __forceinline__ void loc_d(double g, int n, int& i, double& t, int nearest)
+{
+    double gc = g;
+    if (!(gc >= 0.0)) gc = 0.0; /* also maps NaN to 0 */
+    if (gc > (double)(n - 1)) gc = (double)(n - 1);
+    int ii = (int)gc;
+    if (ii > n - 2) ii = n - 2;
+    i = ii;
+    t = nearest ? 0.0 : gc - (double)ii;
+}
+
+// Trilinear interpolation with the corner order and lerp chain of `interp`
+// (channel.metal:63-68): x first, then y, then z.
+__device__ __forceinline__ void sample_d(const double* __restrict__ F, uint32_t b, uint32_t sy, uint32_t sz,
+                                         double tx, double ty, double tz, double out[3])
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double c00 = lerp_d(F[3 * b + d], F[3 * (b + 1) + d], tx);
+        double c10 = lerp_d(F[3 * (b + sy) + d], F[3 * (b + 1 + sy) + d], tx);
+        double c01 = lerp_d(F[3 * (b + sz) + d], F[3 * (b + 1 + sz) + d], tx);
+        double c11 = lerp_d(F[3 * (b + sy + sz) + d], F[3 * (b + 1 + sy + sz) + d], tx);
+        out[d] = lerp_d(lerp_d(c00, c10, ty), lerp_d(c01, c11, ty), tz);
+    }
+}
+
+// ---------------------------------------------------- FP64 plain step
+// channel.metal:75-123 in FP64: kick with E at the current position, drift,
+// advance the step counter, collide, classify (return, exit, box, crash, timeout).
+__global__ __launch_bounds__(256) void k_plain(IonState S, const JobP* __restrict__ jobs, BatchP B,
+                                               const double* __restrict__ rf, const double* __restrict__ dc,
+                                               const double* __restrict__ ac, const uint8_t* __restrict__ mask,
+                                               uint32_t n, uint32_t nsteps, uint32_t* __restrict__ live)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool alive = false;
+    if (i < n && S.status[i] == 0) {
+        const JobP J = jobs[S.job[i]];
+        const uint32_t ion = (uint32_t)(i - J.off);
+        double x = S.x[i], y = S.y[i], z = S.z[i], vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
+        const double phi = S.phi[i];
+        uint32_t st = S.steps[i], ncoll = S.ncoll[i], cd = S.cd[i];
+        const uint32_t sy = (uint32_t)B.nx, sz = (uint32_t)B.nx * (uint32_t)B.ny;
+        const double dt = B.dt;
+        int status = 0;
+        u32x4 cw = {0, 0, 0, 0};
+        for (uint32_t s = 0; s < nsteps; ++s) {
+            int ii, jj, kk;
+            double tx, ty, tz;
+            loc_d(x * B.inv_h, B.nx, ii, tx, B.nearest);
+            loc_d(y * B.inv_h, B.ny, jj, ty, B.nearest);
+            loc_d(z * B.inv_h, B.nz, kk, tz, B.nearest);
+            const uint32_t b = (uint32_t)ii + sy * ((uint32_t)jj + (uint32_t)B.ny * (uint32_t)kk);
+            double er[3], ed[3], ea[3];
+            sample_d(rf, b, sy, sz, tx, ty, tz, er);
+            sample_d(dc, b, sy, sz, tx, ty, tz, ed);
+            sample_d(ac, b, sy, sz, tx, ty, tz, ea);
+            const double a_rf = J.v_rf * sin2pi(frac_d(J.w_rf * (double)st));
+            const double a_ac = J.v_ac * sin2pi(frac_d(fma(J.w_ac, (double)st, phi)));
+            double E;
+            E = (a_rf * er[0] + J.s_dc * ed[0]) + a_ac * ea[0]; vx = fma(J.kq, E, vx); x = fma(vx, dt, x);
+            E = (a_rf * er[1] + J.s_dc * ed[1]) + a_ac * ea[1]; vy = fma(J.kq, E, vy); y = fma(vy, dt, y);
+            E = (a_rf * er[2] + J.s_dc * ed[2]) + a_ac * ea[2]; vz = fma(J.kq, E, vz); z = fma(vz, dt, z);
+            const uint32_t cur = st;
+            st += 1;
+            if (!B.legacy_coll) {
+                if (J.nsig_dt > 0.0) {
+                    const double bound = norm3(vx, vy, vz) + J.v_bound;
+                    if ((cur & 3) == 0 || s == 0) cw = philox(ion, cur >> 2, 0, D_CAND, J.key0, J.key1);
+                    if (u01(pick(cw, cur & 3)) < J.nsig_dt * bound) {
+                        u32x4 w = philox(ion, cur, 0, D_COLL, J.key0, J.key1);
+                        double r1 = sqrt(-2.0 * log_d(u01(w.a))), r2 = sqrt(-2.0 * log_d(u01(w.c)));
+                        double s1, c1, s2, c2;
+                        sincos2pi(u01(w.b), s1, c1);
+                        sincos2pi(u01(w.d), s2, c2);
+                        const double gx = vx - J.s_gas * (r1 * c1), gy = vy - J.s_gas * (r1 * s1),
+                                     gz = vz - J.s_gas * (r2 * c2);
+                        const double gl = norm3(gx, gy, gz);
+                        u32x4 w2 = philox(ion, cur, 1, D_COLL, J.key0, J.key1);
+                        if (u01(w2.a) * bound < gl) {
+                            double c = 2.0 * u01(w2.b) - 1.0, sp, cp;
+                            sincos2pi(u01(w2.c), sp, cp);
+                            double sn = sqrt(fmax(0.0, 1.0 - c * c));
+                            double mg = J.mr * gl;
+                            vx = (vx - J.mr * gx) + mg * (sn * cp);
+                            vy = (vy - J.mr * gy) + mg * (sn * sp);
+                            vz = (vz - J.mr * gz) + mg * c;
+                            ncoll += 1;
+                        }
+                    }
+                }
+            } else {
+                cd -= 1;
+                if (cd == 0) {
+                    u32x4 w = philox(ion, cur, 0, D_COLL, J.key0, J.key1);
+                    double vm = norm3(vx, vy, vz);
+                    if (vm > J.floor_v) {
+                        double c = 2.0 * u01(w.a) - 1.0, sp, cp;
+                        sincos2pi(u01(w.b), sp, cp);
+                        double sn = sqrt(fmax(0.0, 1.0 - c * c));
+                        double mv = J.mr * vm;
+                        vx = vx * J.one_m_mr + mv * (sn * cp);
+                        vy = vy * J.one_m_mr + mv * (sn * sp);
+                        vz = vz * J.one_m_mr + mv * c;
+                        ncoll += 1;
+                    }
+                    double mfp = -log_d(fmax(u01(w.c), 1e-7)) * J.mfp_mean;
+                    cd = countdown(floor(mfp / fmax(norm3(vx, vy, vz), 1e-3) / dt));
+                }
+            }
+            if (z < B.z_ret) status = 2;
+            else if (z > B.z_exit) status = 3;
+            else if (x < 0.0 || y < 0.0 || z < 0.0 || x > B.lim0 || y > B.lim1 || z > B.lim2) status = 4;
+            else if (z > B.cz0 && z < B.cz1 &&
+                     mask[(uint32_t)rint(x * B.inv_h) +
+                          sy * ((uint32_t)rint(y * B.inv_h) + (uint32_t)B.ny * (uint32_t)rint(z * B.inv_h))])
+                status = 1;
+            else if (st >= B.max_steps) status = 5;
+            if (status) break;
+        }
+        S.x[i] = x; S.y[i] = y; S.z[i] = z; S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
+        S.steps[i] = st; S.ncoll[i] = ncoll; S.cd[i] = cd; S.status[i] = (uint8_t)status;
+        alive = status == 0;
+    }
+    const uint64_t m = __ballot(alive);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(live, (uint32_t)__popcll(m));
+}
+
+// ------------------------------------------------------------ summarize
+// summarize (channel.rs:496-506): per-job counts, sum of steps and collisions.
+__global__ void k_summarize(const uint8_t* __restrict__ status, const uint32_t* __restrict__ steps,
+                            const uint32_t* __restrict__ ncoll, const uint32_t* __restrict__ job,
+                            unsigned long long* __restrict__ acc, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long* a = acc + (size_t)job[i] * ACC;
+    uint32_t st = status[i];
+    atomicAdd(&a[0], 1ull);
+    atomicAdd(&a[st >= 1 && st <= 5 ? st : 5], 1ull);
+    atomicAdd(&a[6], (unsigned long long)steps[i]);
+    atomicAdd(&a[7], (unsigned long long)ncoll[i]);
+}
+
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+void launch_inject(const IonState& S, const JobP* jobs, uint32_t njobs, const InjP& P, uint32_t n, hipStream_t st)
+{
+    if (n) k_inject<<<nblk(n, 256), 256, 0, st>>>(S, jobs, njobs, P, n);
+}
+
+void launch_combine_dc(double* out, const double* const d[4], const double v[4], uint32_t n3, hipStream_t st)
+{
+    k_combine_dc<<<nblk(n3, 256), 256, 0, st>>>(out, d[0], d[1], d[2], d[3], v[0], v[1], v[2], v[3], n3);
+}
+
+void launch_plain(const IonState& S, const JobP* jobs, const BatchP& B, const double* rf, const double* dc,
+                  const double* ac, const uint8_t* mask, uint32_t n, uint32_t nsteps, uint32_t* live, hipStream_t st)
+{
+    k_plain<<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, rf, dc, ac, mask, n, nsteps, live);
+}
+
+void launch_summarize(const IonState& S, unsigned long long* acc, uint32_t n, hipStream_t st)
+{
+    if (n) k_summarize<<<nblk(n, 256), 256, 0, st>>>(S.status, S.steps, S.ncoll, S.job, acc, n);
+}
+
+void launch_fill_u64(unsigned long long* p, unsigned long long v, uint32_t n, hipStream_t st)
+{
+    if (n) k_fill_u64<<<nblk(n, 256), 256, 0, st>>>(p, v, n);
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t st)
+{
+    if (n) k_fill_u32<<<nblk(n, 256), 256, 0, st>>>(p, v, n);
+}
+
+}  // namespace mq

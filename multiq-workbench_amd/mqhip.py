@@ -1,0 +1,274 @@
+"""ctypes binding of libmqhip.so (the C ABI in include/mq_engine.h).
+
+Python mirror of chsim's batch API (simulation/chsim/src/channel.rs:233-506):
+`Engine(fields)` replaces `Engine::new`, `Engine.run_batch` replaces
+`Engine::run_batch`, `Engine.run_batch_ions` replaces `Engine::run_batch_ions`,
+`summarize` replaces `summarize`, and `Physics.legacy` / `Physics.correct`
+mirror the presets of channel.rs:260-272. Every compute call goes to the HIP
+library; there is no CPU fallback: a missing library or GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmqhip.so")
+
+ALIVE, CRASHED, RETURNED, TRANSMITTED, LEFT, TIMEOUT = range(6)
+PATH_BATCHED, PATH_PLAIN = 0, 1
+INJ_REFERENCE, INJ_GAUSSIAN = 0, 1
+
+AMU = 1.6605390666e-27
+QE = 1.602176634e-19
+KB = 1.380649e-23
+
+
+class Species(C.Structure):
+    _fields_ = [("mass", C.c_double), ("z", C.c_double), ("ccs_n2", C.c_double)]
+
+
+class RfAc(C.Structure):
+    _fields_ = [("v_rf", C.c_double), ("f_rf", C.c_double), ("v_ac", C.c_double), ("f_ac", C.c_double)]
+
+
+class Job(C.Structure):
+    _fields_ = [("sp", Species), ("drive", RfAc), ("n", C.c_uint64), ("seed", C.c_uint64), ("dc_scale", C.c_double),
+                ("pressure_pa", C.c_double)]
+
+
+class Physics(C.Structure):
+    _fields_ = [
+        ("legacy_force", C.c_int32), ("legacy_collisions", C.c_int32), ("legacy_fields", C.c_int32),
+        ("legacy_injection", C.c_int32), ("legacy_criteria", C.c_int32), ("injection", C.c_int32),
+        ("pressure_pa", C.c_double), ("temp_k", C.c_double), ("dt", C.c_double), ("t_max", C.c_double),
+        ("r_inject_mm", C.c_double), ("inj_sigma_mm", C.c_double), ("max_steps", C.c_uint64),
+    ]
+
+    @staticmethod
+    def legacy(pressure_pa: float) -> "Physics":
+        """Physics::legacy (channel.rs:261-264)."""
+        return Physics(1, 1, 1, 1, 1, INJ_REFERENCE, pressure_pa, 300.0, 1e-7, 1.0, 0.0, 0.0, 0)
+
+    @staticmethod
+    def correct(pressure_pa: float, dt_div: float = 50.0) -> "Physics":
+        """Physics::correct (channel.rs:268-272): dt = T_RF / dt_div at 500 kHz."""
+        return Physics(0, 0, 0, 0, 0, INJ_REFERENCE, pressure_pa, 300.0, 2e-6 / dt_div, 0.02, 2.0, 0.0, 0)
+
+    def copy(self, **kw) -> "Physics":
+        p = Physics()
+        C.pointer(p)[0] = self
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+class Outcome(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64), ("crashed", C.c_uint64), ("returned", C.c_uint64), ("transmitted", C.c_uint64),
+        ("left", C.c_uint64), ("timeout", C.c_uint64), ("mean_coll", C.c_double), ("mean_time", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+    def frac(self, k: int) -> float:
+        return k / self.n if self.n else 0.0
+
+
+class Fields(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("h", C.c_double),
+        ("axis_x", C.c_double), ("axis_y", C.c_double), ("z_inject", C.c_double),
+        ("z_return", C.c_double), ("z_exit", C.c_double),
+        ("z_return_legacy", C.c_double), ("z_exit_legacy", C.c_double),
+        ("crash_z0_legacy", C.c_double), ("crash_z1_legacy", C.c_double),
+        ("rf", C.POINTER(C.c_double)), ("ac", C.POINTER(C.c_double)), ("dc", C.POINTER(C.c_double) * 4),
+        ("mask", C.POINTER(C.c_uint8)),
+    ]
+
+
+class IonSoA(C.Structure):
+    _fields_ = [
+        ("x", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double)), ("z", C.POINTER(C.c_double)),
+        ("vx", C.POINTER(C.c_double)), ("vy", C.POINTER(C.c_double)), ("vz", C.POINTER(C.c_double)),
+        ("steps", C.POINTER(C.c_uint32)), ("ncoll", C.POINTER(C.c_uint32)), ("status", C.POINTER(C.c_uint8)),
+    ]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("ion_steps", C.c_uint64), ("n_ions", C.c_uint64), ("launches", C.c_uint32),
+                ("kernel_ms", C.c_double), ("wall_ms", C.c_double)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("block", C.c_int32), ("waves_per_cu", C.c_int32), ("refill_period", C.c_int32),
+                ("chunk", C.c_int32), ("chunk_cap", C.c_int32), ("resort", C.c_int32)]
+
+
+# Exported symbols of include/mq_engine.h with their ctypes signatures.
+SIGNATURES = {
+    "mq_last_error": (C.c_char_p, []),
+    "mq_engine_create": (C.c_int, [C.POINTER(Fields), C.POINTER(C.c_void_p)]),
+    "mq_engine_destroy": (None, [C.c_void_p]),
+    "mq_run_batch": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_size_t, C.POINTER(C.c_double), C.POINTER(Physics),
+                               C.c_int32, C.POINTER(Outcome)]),
+    "mq_run_batch_ions": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_size_t, C.POINTER(C.c_double),
+                                    C.POINTER(Physics), C.c_int32, C.POINTER(IonSoA), C.POINTER(C.c_uint32)]),
+    "mq_summarize": (C.c_int, [C.POINTER(C.c_uint8), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                               C.POINTER(C.c_uint32), C.c_size_t, C.c_size_t, C.c_double, C.POINTER(Outcome)]),
+    "mq_last_stats": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
+    "mq_set_tune": (C.c_int, [C.c_void_p, C.POINTER(Tune)]),
+    "mq_get_tune": (C.c_int, [C.c_void_p, C.POINTER(Tune)]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load libmqhip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built (run `make -C {HERE}`)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class MqError(RuntimeError):
+    pass
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = load().mq_last_error().decode(errors="replace")
+        raise MqError(f"mq error {rc}: {msg}")
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_fields(grid: dict) -> tuple[Fields, list]:
+    """Build an mq_fields struct from a grid dict (see synth.py); returns the
+    struct and the list of arrays it points into (keep it alive)."""
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a
+
+    f = Fields()
+    f.nx, f.ny, f.nz = grid["nx"], grid["ny"], grid["nz"]
+    f.h = grid["h"]
+    for k in ("axis_x", "axis_y", "z_inject", "z_return", "z_exit", "z_return_legacy", "z_exit_legacy",
+              "crash_z0_legacy", "crash_z1_legacy"):
+        setattr(f, k, grid[k])
+    f.rf = _dptr(arr(grid["rf"], np.float64))
+    f.ac = _dptr(arr(grid["ac"], np.float64))
+    for b in range(4):
+        d = grid["dc"][b]
+        f.dc[b] = _dptr(arr(d, np.float64)) if d is not None else C.POINTER(C.c_double)()
+    m = grid.get("mask")
+    f.mask = arr(m, np.uint8).ctypes.data_as(C.POINTER(C.c_uint8)) if m is not None else C.POINTER(C.c_uint8)()
+    return f, keep
+
+
+def jobs_array(jobs) -> C.Array:
+    arr = (Job * len(jobs))()
+    for i, j in enumerate(jobs):
+        arr[i] = j
+    return arr
+
+
+def job(mass, z, ccs, v_rf, f_rf, v_ac, f_ac, n, seed, dc_scale=1.0, pressure_pa=0.0) -> Job:
+    return Job(Species(mass, z, ccs), RfAc(v_rf, f_rf, v_ac, f_ac), n, seed, dc_scale, pressure_pa)
+
+
+class Engine:
+    """Engine::new (channel.rs:327): uploads the grids once."""
+
+    def __init__(self, grid: dict):
+        lib = load()
+        f, keep = make_fields(grid)
+        h = C.c_void_p()
+        _check(lib.mq_engine_create(C.byref(f), C.byref(h)))
+        del keep
+        self._h = h
+        self.grid_meta = {k: grid[k] for k in ("nx", "ny", "nz", "h")}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().mq_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run_batch(self, jobs, dc, ph: Physics, path: int = PATH_BATCHED) -> list[Outcome]:
+        ja = jobs_array(jobs)
+        out = (Outcome * len(jobs))()
+        dcv = (C.c_double * 4)(*dc)
+        _check(load().mq_run_batch(self._h, ja, len(jobs), dcv, C.byref(ph), path, out))
+        return list(out)
+
+    def run_batch_ions(self, jobs, dc, ph: Physics, path: int = PATH_BATCHED) -> dict:
+        ja = jobs_array(jobs)
+        n = int(sum(j.n for j in jobs))
+        o = {k: np.zeros(n, np.float64) for k in ("x", "y", "z", "vx", "vy", "vz")}
+        o["steps"] = np.zeros(n, np.uint32)
+        o["ncoll"] = np.zeros(n, np.uint32)
+        o["status"] = np.zeros(n, np.uint8)
+        o["job"] = np.zeros(n, np.uint32)
+        soa = IonSoA()
+        for k in ("x", "y", "z", "vx", "vy", "vz"):
+            setattr(soa, k, _dptr(o[k]))
+        soa.steps = o["steps"].ctypes.data_as(C.POINTER(C.c_uint32))
+        soa.ncoll = o["ncoll"].ctypes.data_as(C.POINTER(C.c_uint32))
+        soa.status = o["status"].ctypes.data_as(C.POINTER(C.c_uint8))
+        dcv = (C.c_double * 4)(*dc)
+        _check(load().mq_run_batch_ions(self._h, ja, len(jobs), dcv, C.byref(ph), path, C.byref(soa),
+                                        o["job"].ctypes.data_as(C.POINTER(C.c_uint32))))
+        return o
+
+    def stats(self) -> dict:
+        s = RunStats()
+        _check(load().mq_last_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def set_tune(self, **kw) -> None:
+        t = Tune()
+        _check(load().mq_get_tune(self._h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        _check(load().mq_set_tune(self._h, C.byref(t)))
+
+    def get_tune(self) -> dict:
+        t = Tune()
+        _check(load().mq_get_tune(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+
+def summarize(status, ncoll, steps, job, njobs: int, dt: float) -> list[Outcome]:
+    """summarize (channel.rs:496-506) through the C ABI."""
+    status = np.ascontiguousarray(status, np.uint8)
+    ncoll = np.ascontiguousarray(ncoll, np.uint32)
+    steps = np.ascontiguousarray(steps, np.uint32)
+    job = np.ascontiguousarray(job, np.uint32)
+    out = (Outcome * njobs)()
+    u32 = C.POINTER(C.c_uint32)
+    _check(load().mq_summarize(status.ctypes.data_as(C.POINTER(C.c_uint8)), ncoll.ctypes.data_as(u32),
+                               steps.ctypes.data_as(u32), job.ctypes.data_as(u32), len(status), njobs, dt, out))
+    return list(out)

@@ -1,0 +1,113 @@
+"""Synthetic basis-field grids and workload generators.
+
+The reference builds its grids with a multigrid Laplace solve of the channel
+geometry (channel.rs:143-217, simulation/fieldsolve). These synthetic grids
+stand in for them: an ideal quadrupole whose RF potential has the reference's
+orientation phi = -2 X Y / r0^2 (channel.rs:510-522, r0_profile), a uniform
+AC dipole across the left/right rod pairs (ac_sign, channel.rs:105), one DC
+basis (the plate slot, index 1) shaped as a Gaussian axial barrier with its
+Laplace-consistent radial term, and a conductor mask on the hyperbolic rod
+surfaces |X Y| >= r0^2 / 2 plus a cylindrical wall at 1.1 r0. All grids are
+node-major (i fastest), E per volt, FP64.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+R0_REFERENCE = 2.765e-3   # r0 of the reference channel (channel.rs:58, Geom::default)
+
+
+def ideal_channel(nx: int, ny: int, nz: int, h: float, r0: float = R0_REFERENCE, z_gate: float | None = None,
+                  w_gate: float = 1.0e-3, z_exit: float = 18.6e-3, wall: float = 1.1) -> dict:
+    ax = (nx // 2) * h
+    ay = (ny // 2) * h
+    x = np.arange(nx) * h - ax
+    y = np.arange(ny) * h - ay
+    z = np.arange(nz) * h
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")  # shape (nz, ny, nx): i fastest
+    N = nx * ny * nz
+    rf = np.empty((nz, ny, nx, 3))
+    rf[..., 0] = 2.0 * Y / r0**2
+    rf[..., 1] = 2.0 * X / r0**2
+    rf[..., 2] = 0.0
+    ac = np.zeros((nz, ny, nx, 3))
+    ac[..., 0] = -1.0 / r0
+    zg = z_exit if z_gate is None else z_gate
+    u = Z - zg
+    g = np.exp(-(u / w_gate) ** 2)
+    g1 = -2.0 * u / w_gate**2 * g
+    g2 = (4.0 * u**2 / w_gate**4 - 2.0 / w_gate**2) * g
+    g3 = (12.0 * u / w_gate**4 - 8.0 * u**3 / w_gate**6) * g
+    r2 = X**2 + Y**2
+    dc1 = np.empty((nz, ny, nx, 3))
+    dc1[..., 0] = g2 * X / 2.0
+    dc1[..., 1] = g2 * Y / 2.0
+    dc1[..., 2] = -g1 + g3 * r2 / 4.0
+    mask = ((np.abs(X * Y) >= r0**2 / 2.0) | (r2 >= (wall * r0) ** 2)).astype(np.uint8)
+    return {
+        "nx": nx, "ny": ny, "nz": nz, "h": h,
+        "axis_x": ax, "axis_y": ay, "z_inject": 1.0e-3,
+        "z_return": 0.5e-3, "z_exit": z_exit,
+        "z_return_legacy": 1.0e-3, "z_exit_legacy": 23.7e-3,
+        "crash_z0_legacy": 3.0e-3, "crash_z1_legacy": 15.0e-3,
+        "rf": rf.reshape(3 * N), "ac": ac.reshape(3 * N),
+        "dc": [None, dc1.reshape(3 * N), None, None],
+        "mask": mask.reshape(N),
+        "r0": r0,
+    }
+
+
+def c1_grid() -> dict:
+    """64 x 64 x 256 nodes at 0.1 mm (the Fig-4 single-channel configuration)."""
+    return ideal_channel(64, 64, 256, 1.0e-4)
+
+
+def c2_grid() -> dict:
+    """128 x 128 x 512 nodes at 0.05 mm (the batched mass scan)."""
+    return ideal_channel(128, 128, 512, 0.5e-4)
+
+
+def c3_grid() -> dict:
+    """192 x 192 x 768 nodes at 1/30 mm (the operating-setting sweep)."""
+    return ideal_channel(192, 192, 768, 1.0e-4 / 3.0)
+
+
+def c1_jobs(mod, n: int = 1000, mz: float = 499.0, z: float = 1.0, ccs: float = 230.0, seed: int | None = None):
+    """Fig-4 point: 100 V RF at 500 kHz, 0.5 V AC at 100 kHz, seed 1000 + m/z."""
+    s = int(1000 + mz) if seed is None else seed
+    return [mod.job(mz * z, z, ccs, 100.0, 500e3, 0.5, 100e3, n, s)]
+
+
+def c2_jobs(mod, n: int = 1000, z: float = 1.0, ccs: float = 230.0):
+    """Mass scan m/z 475..523 step 2 (25) x 64 RF frequencies 450..550 kHz x n ions."""
+    jobs = []
+    for mz in range(475, 524, 2):
+        for k in range(64):
+            f_rf = 450e3 + 100e3 * k / 63.0
+            jobs.append(mod.job(mz * z, z, ccs, 100.0, f_rf, 0.5, 100e3, n, 1000 + mz + 100000 * k))
+    return jobs
+
+
+def lhs(n: int, d: int, seed: int) -> np.ndarray:
+    """Latin-hypercube sample of n points in [0,1)^d (stratified, seeded)."""
+    rng = np.random.default_rng(seed)
+    out = np.empty((n, d))
+    for k in range(d):
+        out[:, k] = (rng.permutation(n) + rng.random(n)) / n
+    return out
+
+
+def c3_jobs(mod, n_settings: int = 4096, n: int = 256, seed: int = 3):
+    """Operating-setting sweep: V_rf 50-150 V, f_exc 80-120 kHz, V_exc 0.1-1 V,
+    V_dc 0-10 V (per-job DC scale of the plate basis at 1 V), N2 0.5-2 mTorr
+    (per-job pressure), n ions of m/z 499, z = 1 per setting."""
+    u = lhs(n_settings, 5, seed)
+    jobs = []
+    for i in range(n_settings):
+        v_rf = 50.0 + 100.0 * u[i, 0]
+        f_exc = 80e3 + 40e3 * u[i, 1]
+        v_exc = 0.1 + 0.9 * u[i, 2]
+        v_dc = 10.0 * u[i, 3]
+        p = (0.5 + 1.5 * u[i, 4]) * 0.133322
+        jobs.append(mod.job(499.0, 1.0, 230.0, v_rf, 500e3, v_exc, f_exc, n, 7000 + i, v_dc, p))
+    return jobs

@@ -1,0 +1,510 @@
+/*
+ * oracle.c -- CPU FP64 oracle of the batched ion-trajectory integrator.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h): the checker for the HIP path and
+ * the timed CPU baseline. Build with -O2 -ffp-contract=off (oracle/Makefile):
+ * every fused multiply-add below is an explicit fma() call, every other
+ * product and sum is rounded separately, so the GPU FP64 path, which follows
+ * the same operation order, is bit-identical.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define AMU 1.6605390666e-27
+#define QE 1.602176634e-19
+#define KB 1.380649e-23
+
+enum { D_CAND = 0, D_COLL = 1, D_INJ = 2 };
+
+/* ------------------------------------------------------------------ Philox */
+
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static inline void blk(const uint32_t key[2], uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t w[4])
+{
+    uint32_t ctr[4] = {a, b, c, d};
+    or_philox4x32_10(ctr, key, w);
+}
+
+/* Uniform in (0,1): (w + 1/2) 2^-32, exact in FP64. */
+static inline double u01(uint32_t w) { return ((double)w + 0.5) * 0x1p-32; }
+
+/* -------------------------------------------------- sin/cos/log (own code) */
+/* sin(pi r/2) = r * sum S_k r^2k, cos(pi r/2) = sum C_k r^2k for |r| <= 1/2:
+ * Taylor coefficients (-1)^k (pi/2)^(2k+1)/(2k+1)! and (-1)^k (pi/2)^(2k)/(2k)!
+ * rounded to nearest double (truncation error < 1e-19 on |r| <= 1/2). */
+static const double S_[10] = {
+    0x1.921fb54442d18p+0, -0x1.4abbce625be53p-1, 0x1.466bc6775aae2p-4, -0x1.32d2cce62bd86p-8,
+    0x1.50783487ee782p-13, -0x1.e3074fde8871fp-19, 0x1.e8f434d018d63p-25, -0x1.6fadb9f155744p-31,
+    0x1.aaec32af93359p-38, -0x1.8a404211f9547p-45};
+static const double C_[10] = {
+    0x1.0000000000000p+0, -0x1.3bd3cc9be45dep+0, 0x1.03c1f081b5ac4p-2, -0x1.55d3c7e3cbffap-6,
+    0x1.e1f506891babbp-11, -0x1.a6d1f2a204a8cp-16, 0x1.f9d38a3763cc3p-22, -0x1.b6e24f44b128fp-28,
+    0x1.20c62c2f2d7f5p-34, -0x1.2a0c591af8314p-41};
+/* 1/(2k+1), k = 1..12, for log(m) = 2 atanh(s) = 2 s (1 + s^2/3 + s^4/5 + ...). */
+static const double L_[12] = {
+    0x1.5555555555555p-2, 0x1.999999999999ap-3, 0x1.2492492492492p-3, 0x1.c71c71c71c71cp-4,
+    0x1.745d1745d1746p-4, 0x1.3b13b13b13b14p-4, 0x1.1111111111111p-4, 0x1.e1e1e1e1e1e1ep-5,
+    0x1.af286bca1af28p-5, 0x1.8618618618618p-5, 0x1.642c8590b2164p-5, 0x1.47ae147ae147bp-5};
+static const double LN2_HI = 0x1.62e4200000000p-1; /* ln 2 with the low 32 mantissa bits cleared */
+static const double LN2_LO = 0x1.fdf473de6af28p-22; /* ln 2 - LN2_HI */
+static const double SQRT_HALF = 0x1.6a09e667f3bcdp-1;
+
+void or_sincos2pi(double p, double* s, double* c)
+{
+    double q = p * 4.0;          /* quarter turns, exact */
+    double k = rint(q);
+    double r = q - k;            /* exact, |r| <= 1/2 */
+    double r2 = r * r;
+    double ps = S_[9], pc = C_[9];
+    for (int i = 8; i >= 0; --i) {
+        ps = fma(ps, r2, S_[i]);
+        pc = fma(pc, r2, C_[i]);
+    }
+    double sn = ps * r, cs = pc;
+    switch ((int)((int64_t)k & 3)) {
+    case 0: *s = sn; *c = cs; break;
+    case 1: *s = cs; *c = -sn; break;
+    case 2: *s = -sn; *c = -cs; break;
+    default: *s = -cs; *c = sn; break;
+    }
+}
+
+static inline double sin2pi(double p)
+{
+    double s, c;
+    or_sincos2pi(p, &s, &c);
+    return s;
+}
+
+double or_log(double x)
+{
+    uint64_t b;
+    memcpy(&b, &x, 8);
+    int e = (int)((b >> 52) & 0x7ff) - 1022;
+    b = (b & 0x000fffffffffffffULL) | 0x3fe0000000000000ULL;
+    double m;
+    memcpy(&m, &b, 8);            /* x = m 2^e, m in [1/2, 1) */
+    if (m < SQRT_HALF) { m = m * 2.0; e -= 1; }
+    double f = m - 1.0;
+    double s = f / (2.0 + f);
+    double z = s * s;
+    double P = L_[11];
+    for (int i = 10; i >= 0; --i) P = fma(P, z, L_[i]);
+    double t = s * (z * P);
+    double lm = 2.0 * (s + t);
+    double de = (double)e;
+    return de * LN2_HI + (de * LN2_LO + lm);
+}
+
+static inline double frac(double p) { return p - floor(p); }
+static inline double lerp(double a, double b, double t) { return fma(t, b - a, a); }
+static inline double norm3(const double v[3]) { return sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); }
+
+static inline uint32_t cd_from(double x)
+{
+    if (!(x < 2147483647.0)) return 2147483647u;
+    if (x < 1.0) return 1u;
+    return (uint32_t)x;
+}
+
+/* ---------------------------------------------------------- field sampling */
+
+static inline void loc(double g, int n, int* i, double* t, int nearest)
+{
+    double gc = g;
+    if (!(gc >= 0.0)) gc = 0.0; /* also maps NaN to 0 */
+    if (gc > (double)(n - 1)) gc = (double)(n - 1);
+    int ii = (int)gc;
+    if (ii > n - 2) ii = n - 2;
+    *i = ii;
+    *t = nearest ? 0.0 : gc - (double)ii;
+}
+
+void or_sample(const double* F, int nx, int ny, int nz, const double g[3], int nearest, double out[3])
+{
+    int i, j, k;
+    double tx, ty, tz;
+    loc(g[0], nx, &i, &tx, nearest);
+    loc(g[1], ny, &j, &ty, nearest);
+    loc(g[2], nz, &k, &tz, nearest);
+    size_t sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
+    size_t b = (size_t)i + sy * ((size_t)j + (size_t)ny * (size_t)k);
+    for (int d = 0; d < 3; ++d) {
+        double c00 = lerp(F[3 * b + d], F[3 * (b + 1) + d], tx);
+        double c10 = lerp(F[3 * (b + sy) + d], F[3 * (b + 1 + sy) + d], tx);
+        double c01 = lerp(F[3 * (b + sz) + d], F[3 * (b + 1 + sz) + d], tx);
+        double c11 = lerp(F[3 * (b + sy + sz) + d], F[3 * (b + 1 + sy + sz) + d], tx);
+        out[d] = lerp(lerp(c00, c10, ty), lerp(c01, c11, ty), tz);
+    }
+}
+
+/* ------------------------------------------------------ job/batch params */
+
+typedef struct {
+    double kq, v_rf, w_rf, v_ac, w_ac, s_dc;
+    double nsig_dt, s_gas, v_bound, mr, one_m_mr, floor_v, mfp_mean, m;
+    uint32_t key[2];
+} or_jp;
+
+typedef struct {
+    const mq_fields* f;
+    const double* dcc; /* combined DC, 3N */
+    double dt, inv_h, lim[3];
+    double z_ret, z_exit, cz0, cz1;
+    uint64_t max_steps;
+    int nearest, legacy_coll;
+} or_ctx;
+
+/* channel.rs:376-381 */
+static double legacy_mfp_mean(const mq_species* sp, double pressure_pa)
+{
+    double p_torr = pressure_pa / 133.322 + 1e-10;
+    double np = 2.68e5 * (p_torr / 760.0);
+    double mm = sp->mass;
+    double ccs_he = 2.81e-9 * mm * mm * mm - 3.55e-5 * mm * mm + 0.232 * mm + 41.91;
+    double ccs = ccs_he * 10.0 * sqrt(sp->z);
+    return 1.0 / (np * ccs);
+}
+
+/* channel.rs:412-426 */
+static void job_params(const mq_job* j, const mq_physics* ph, or_jp* J)
+{
+    double m = j->sp.mass * AMU, mg = 28.0 * AMU;
+    double qm = j->sp.z * QE / m;
+    double p = j->pressure_pa > 0.0 ? j->pressure_pa : ph->pressure_pa;
+    double n_gas = p / (KB * ph->temp_k);
+    double s_gas = sqrt(KB * ph->temp_k / mg);
+    J->m = m;
+    J->kq = (ph->legacy_force ? 0.5 : 1.0) * qm * ph->dt;
+    J->v_rf = j->drive.v_rf;
+    J->w_rf = j->drive.f_rf * ph->dt;
+    J->v_ac = j->drive.v_ac;
+    J->w_ac = j->drive.f_ac * ph->dt;
+    J->s_dc = j->dc_scale;
+    J->nsig_dt = n_gas * j->sp.ccs_n2 * 1e-20 * ph->dt;
+    J->s_gas = s_gas;
+    J->v_bound = 6.0 * s_gas;
+    J->mr = mg / (m + mg);
+    J->one_m_mr = 1.0 - J->mr;
+    J->floor_v = ph->legacy_collisions ? sqrt(2.0 * 0.035 * 1.6e-19 / (j->sp.mass * 1.66e-27)) : -1.0;
+    J->mfp_mean = legacy_mfp_mean(&j->sp, p);
+    J->key[0] = (uint32_t)j->seed;
+    J->key[1] = (uint32_t)(j->seed >> 32);
+}
+
+static uint64_t max_steps_of(const mq_physics* ph)
+{
+    if (ph->max_steps) return ph->max_steps;
+    return (uint64_t)llround(ph->t_max / ph->dt);
+}
+
+static void ctx_init(or_ctx* C, const mq_fields* f, const double* dcc, const mq_physics* ph)
+{
+    C->f = f;
+    C->dcc = dcc;
+    C->dt = ph->dt;
+    C->inv_h = 1.0 / f->h;
+    C->lim[0] = (double)(f->nx - 1) * f->h;
+    C->lim[1] = (double)(f->ny - 1) * f->h;
+    C->lim[2] = (double)(f->nz - 1) * f->h;
+    C->z_ret = ph->legacy_criteria ? f->z_return_legacy : f->z_return;
+    C->z_exit = ph->legacy_criteria ? f->z_exit_legacy : f->z_exit;
+    C->cz0 = ph->legacy_criteria ? f->crash_z0_legacy : -1e30;
+    C->cz1 = ph->legacy_criteria ? f->crash_z1_legacy : 1e30;
+    C->max_steps = max_steps_of(ph);
+    C->nearest = ph->legacy_fields != 0;
+    C->legacy_coll = ph->legacy_collisions != 0;
+}
+
+/* --------------------------------------------------------------- inject */
+
+void or_inject(const mq_fields* f, const mq_job* job, const mq_physics* ph, uint32_t ion, or_ion* o)
+{
+    or_jp J;
+    job_params(job, ph, &J);
+    uint32_t w[12];
+    blk(J.key, ion, 0, 0, D_INJ, w);
+    blk(J.key, ion, 0, 1, D_INJ, w + 4);
+    blk(J.key, ion, 0, 2, D_INJ, w + 8);
+    double u[12];
+    for (int i = 0; i < 12; ++i) u[i] = u01(w[i]);
+    double m = J.m, ax = f->axis_x, ay = f->axis_y, z0 = f->z_inject;
+    double ufirst, sn, cs;
+    if (ph->injection == MQ_INJ_GAUSSIAN) {
+        double sig = ph->inj_sigma_mm * 1e-3, s = sqrt(KB * ph->temp_k / m);
+        double r1 = sqrt(-2.0 * or_log(u[0]));
+        or_sincos2pi(u[1], &sn, &cs);
+        o->x[0] = ax + sig * (r1 * cs);
+        o->x[1] = ay + sig * (r1 * sn);
+        o->x[2] = z0;
+        double r2 = sqrt(-2.0 * or_log(u[2]));
+        or_sincos2pi(u[3], &sn, &cs);
+        o->v[0] = s * (r2 * cs);
+        o->v[1] = s * (r2 * sn);
+        double r3 = sqrt(-2.0 * or_log(u[4]));
+        or_sincos2pi(u[5], &sn, &cs);
+        o->v[2] = s * (r3 * cs);
+        o->phi_ac = u[6];
+        ufirst = u[7];
+    } else if (ph->legacy_injection) {
+        /* channel.rs:350-355 */
+        double kin = 0.1 * (0.75 + 0.75 * u[0]) * job->sp.z;
+        double vin = sqrt(2.0 * kin * QE / m);
+        or_sincos2pi((-10.0 + 20.0 * u[1]) / 360.0, &sn, &cs);
+        double vx = 0.707 * vin * sn;
+        o->x[0] = ax; o->x[1] = ay; o->x[2] = z0;
+        o->v[0] = vx; o->v[1] = vx; o->v[2] = vin * cs;
+        o->phi_ac = u[2];
+        ufirst = u[3];
+    } else {
+        /* channel.rs:357-361, draw order r, theta, vz, vx(2), vy(2) */
+        double s = sqrt(KB * ph->temp_k / m);
+        double r = ph->r_inject_mm * 1e-3 * sqrt(u[0]);
+        or_sincos2pi(u[1], &sn, &cs);
+        double vz = s * sqrt(-2.0 * or_log(u[2]));
+        o->x[0] = ax + r * cs;
+        o->x[1] = ay + r * sn;
+        o->x[2] = z0;
+        double c3, s3;
+        or_sincos2pi(u[4], &s3, &c3);
+        o->v[0] = s * (sqrt(-2.0 * or_log(u[3])) * c3);
+        or_sincos2pi(u[6], &s3, &c3);
+        o->v[1] = s * (sqrt(-2.0 * or_log(u[5])) * c3);
+        o->v[2] = vz;
+        o->phi_ac = u[7];
+        ufirst = u[8];
+    }
+    double vmag = norm3(o->v);
+    double first_mfp = -or_log(fmax(ufirst, 1e-7)) * J.mfp_mean / 10.0;
+    o->cd = cd_from(floor(first_mfp / vmag / ph->dt) + 1.0);
+    o->steps = 0;
+    o->ncoll = 0;
+    o->status = MQ_ALIVE;
+}
+
+/* ---------------------------------------------------------------- advance */
+
+static void advance(const or_ctx* C, const or_jp* J, or_ion* s, uint32_t ion)
+{
+    const mq_fields* f = C->f;
+    const double dt = C->dt;
+    double x[3] = {s->x[0], s->x[1], s->x[2]}, v[3] = {s->v[0], s->v[1], s->v[2]};
+    uint32_t n = s->steps, ncoll = s->ncoll, cd = s->cd;
+    int status = 0;
+    while (status == 0) {
+        double g[3] = {x[0] * C->inv_h, x[1] * C->inv_h, x[2] * C->inv_h};
+        double er[3], ed[3], ea[3];
+        or_sample(f->rf, f->nx, f->ny, f->nz, g, C->nearest, er);
+        or_sample(C->dcc, f->nx, f->ny, f->nz, g, C->nearest, ed);
+        or_sample(f->ac, f->nx, f->ny, f->nz, g, C->nearest, ea);
+        double a_rf = J->v_rf * sin2pi(frac(J->w_rf * (double)n));
+        double a_ac = J->v_ac * sin2pi(frac(fma(J->w_ac, (double)n, s->phi_ac)));
+        for (int d = 0; d < 3; ++d) {
+            double E = (a_rf * er[d] + J->s_dc * ed[d]) + a_ac * ea[d];
+            v[d] = fma(J->kq, E, v[d]);
+            x[d] = fma(v[d], dt, x[d]);
+        }
+        const uint32_t st = n;
+        n += 1;
+        /* collisions (channel.metal:93-113) */
+        if (!C->legacy_coll) {
+            if (J->nsig_dt > 0.0) {
+                double bound = norm3(v) + J->v_bound;
+                uint32_t w[4];
+                blk(J->key, ion, st >> 2, 0, D_CAND, w);
+                if (u01(w[st & 3]) < J->nsig_dt * bound) {
+                    blk(J->key, ion, st, 0, D_COLL, w);
+                    double r1 = sqrt(-2.0 * or_log(u01(w[0]))), r2 = sqrt(-2.0 * or_log(u01(w[2])));
+                    double s1, c1, s2, c2;
+                    or_sincos2pi(u01(w[1]), &s1, &c1);
+                    or_sincos2pi(u01(w[3]), &s2, &c2);
+                    double gv[3] = {v[0] - J->s_gas * (r1 * c1), v[1] - J->s_gas * (r1 * s1), v[2] - J->s_gas * (r2 * c2)};
+                    double gl = norm3(gv);
+                    blk(J->key, ion, st, 1, D_COLL, w);
+                    if (u01(w[0]) * bound < gl) {
+                        double c = 2.0 * u01(w[1]) - 1.0, sp, cp;
+                        or_sincos2pi(u01(w[2]), &sp, &cp);
+                        double sn = sqrt(fmax(0.0, 1.0 - c * c));
+                        double nn[3] = {sn * cp, sn * sp, c};
+                        double mg = J->mr * gl;
+                        for (int d = 0; d < 3; ++d) v[d] = (v[d] - J->mr * gv[d]) + mg * nn[d];
+                        ncoll += 1;
+                    }
+                }
+            }
+        } else {
+            cd -= 1;
+            if (cd == 0) {
+                uint32_t w[4];
+                blk(J->key, ion, st, 0, D_COLL, w);
+                double vm = norm3(v);
+                if (vm > J->floor_v) {
+                    double c = 2.0 * u01(w[0]) - 1.0, sp, cp;
+                    or_sincos2pi(u01(w[1]), &sp, &cp);
+                    double sn = sqrt(fmax(0.0, 1.0 - c * c));
+                    double nn[3] = {sn * cp, sn * sp, c};
+                    double mv = J->mr * vm;
+                    for (int d = 0; d < 3; ++d) v[d] = v[d] * J->one_m_mr + mv * nn[d];
+                    ncoll += 1;
+                }
+                double mfp = -or_log(fmax(u01(w[2]), 1e-7)) * J->mfp_mean;
+                cd = cd_from(floor(mfp / fmax(norm3(v), 1e-3) / dt));
+            }
+        }
+        /* classification (channel.metal:114-120) */
+        if (x[2] < C->z_ret) status = MQ_RETURNED;
+        else if (x[2] > C->z_exit) status = MQ_TRANSMITTED;
+        else if (x[0] < 0.0 || x[1] < 0.0 || x[2] < 0.0 || x[0] > C->lim[0] || x[1] > C->lim[1] || x[2] > C->lim[2])
+            status = MQ_LEFT;
+        else if (x[2] > C->cz0 && x[2] < C->cz1 && f->mask &&
+                 f->mask[(size_t)rint(x[0] * C->inv_h) +
+                         (size_t)f->nx * ((size_t)rint(x[1] * C->inv_h) + (size_t)f->ny * (size_t)rint(x[2] * C->inv_h))])
+            status = MQ_CRASHED;
+        else if ((uint64_t)n >= C->max_steps) status = MQ_TIMEOUT;
+    }
+    s->x[0] = x[0]; s->x[1] = x[1]; s->x[2] = x[2];
+    s->v[0] = v[0]; s->v[1] = v[1]; s->v[2] = v[2];
+    s->steps = n;
+    s->ncoll = ncoll;
+    s->cd = cd;
+    s->status = (uint8_t)status;
+}
+
+/* ------------------------------------------------------------- batch run */
+
+static double* combine_dc(const mq_fields* f, const double dc[4])
+{
+    size_t n3 = 3 * (size_t)f->nx * (size_t)f->ny * (size_t)f->nz;
+    double* e = (double*)calloc(n3, sizeof(double));
+    if (!e) return NULL;
+    for (int b = 0; b < 4; ++b) {
+        if (!f->dc[b] || dc[b] == 0.0) continue;
+        for (size_t i = 0; i < n3; ++i) e[i] = e[i] + dc[b] * f->dc[b][i];
+    }
+    return e;
+}
+
+typedef struct {
+    const or_ctx* C;
+    const mq_job* jobs;
+    const or_jp* jp;
+    const uint64_t* off; /* job offsets, njobs + 1 */
+    size_t njobs, total;
+    const mq_physics* ph;
+    const mq_fields* f;
+    or_ion* out;
+    uint32_t* out_job;
+    uint32_t i0; /* first ion index inside the job (range runs) */
+    atomic_size_t next;
+} or_work;
+
+static void* worker(void* arg)
+{
+    or_work* W = (or_work*)arg;
+    for (;;) {
+        size_t a = atomic_fetch_add(&W->next, 16);
+        if (a >= W->total) break;
+        size_t b = a + 16 < W->total ? a + 16 : W->total;
+        size_t j = 0;
+        while (j + 1 < W->njobs && W->off[j + 1] <= a) ++j;
+        for (size_t i = a; i < b; ++i) {
+            while (j + 1 < W->njobs && W->off[j + 1] <= i) ++j;
+            uint32_t ion = (uint32_t)(i - W->off[j]) + W->i0;
+            or_inject(W->f, &W->jobs[j], W->ph, ion, &W->out[i]);
+            advance(W->C, &W->jp[j], &W->out[i], ion);
+            if (W->out_job) W->out_job[i] = (uint32_t)j;
+        }
+    }
+    return NULL;
+}
+
+static int run_common(const mq_fields* f, const mq_job* jobs, size_t njobs, const double dc[4], const mq_physics* ph,
+                      int nthreads, or_ion* out, uint32_t* out_job, uint32_t i0, size_t total_override)
+{
+    double* dcc = combine_dc(f, dc);
+    if (!dcc) return -1;
+    or_ctx C;
+    ctx_init(&C, f, dcc, ph);
+    or_jp* jp = (or_jp*)malloc(njobs * sizeof(or_jp));
+    uint64_t* off = (uint64_t*)malloc((njobs + 1) * sizeof(uint64_t));
+    off[0] = 0;
+    for (size_t j = 0; j < njobs; ++j) {
+        job_params(&jobs[j], ph, &jp[j]);
+        off[j + 1] = off[j] + jobs[j].n;
+    }
+    or_work W;
+    W.C = &C; W.jobs = jobs; W.jp = jp; W.off = off; W.njobs = njobs;
+    W.total = total_override ? total_override : (size_t)off[njobs];
+    W.ph = ph; W.f = f; W.out = out; W.out_job = out_job; W.i0 = i0;
+    atomic_init(&W.next, 0);
+    if (nthreads < 1) nthreads = 1;
+    pthread_t* th = (pthread_t*)malloc((size_t)nthreads * sizeof(pthread_t));
+    for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, worker, &W);
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th);
+    free(jp);
+    free(off);
+    free(dcc);
+    return 0;
+}
+
+int or_run_batch(const mq_fields* f, const mq_job* jobs, size_t njobs, const double dc[4], const mq_physics* ph,
+                 int nthreads, or_ion* out, uint32_t* out_job)
+{
+    return run_common(f, jobs, njobs, dc, ph, nthreads, out, out_job, 0, 0);
+}
+
+int or_run_range(const mq_fields* f, const mq_job* job, const double dc[4], const mq_physics* ph, uint32_t i0,
+                 uint32_t i1, int nthreads, or_ion* out)
+{
+    if (i1 <= i0) return 0;
+    mq_job j = *job;
+    j.n = i1 - i0;
+    return run_common(f, &j, 1, dc, ph, nthreads, out, NULL, i0, 0);
+}
+
+void or_summarize(const or_ion* ions, const uint32_t* job, size_t n, size_t njobs, double dt, mq_outcome* out)
+{
+    memset(out, 0, njobs * sizeof(mq_outcome));
+    for (size_t i = 0; i < n; ++i) {
+        mq_outcome* o = &out[job[i]];
+        o->n += 1;
+        switch (ions[i].status) {
+        case MQ_CRASHED: o->crashed += 1; break;
+        case MQ_RETURNED: o->returned += 1; break;
+        case MQ_TRANSMITTED: o->transmitted += 1; break;
+        case MQ_LEFT: o->left += 1; break;
+        default: o->timeout += 1; break;
+        }
+        o->mean_coll += (double)ions[i].ncoll;
+        o->mean_time += (double)ions[i].steps * dt;
+    }
+    for (size_t j = 0; j < njobs; ++j)
+        if (out[j].n) {
+            out[j].mean_coll /= (double)out[j].n;
+            out[j].mean_time /= (double)out[j].n;
+        }
+}

@@ -1,0 +1,76 @@
+/*
+ * oracle.h -- CPU FP64 oracle of the batched ion-trajectory integrator.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * or the timed CPU baseline. The product (multiq-workbench_amd/) never links,
+ * loads or calls it.
+ *
+ * What it restates (reference file:line):
+ *   step arithmetic and order ........ simulation/chsim/src/channel.metal:82-121
+ *   trilinear/nearest sampling ....... channel.metal:55-68
+ *   thermal null-collision bath ...... channel.metal:93-99 (form of world.rs:473-488)
+ *   legacy stationary-gas collisions . channel.metal:100-113
+ *   classification order ............. channel.metal:114-120
+ *   job and batch parameters ......... simulation/chsim/src/channel.rs:397-426
+ *   injection models ................. channel.rs:342-374 (legacy cone, thermal disc)
+ *   summarize ........................ channel.rs:496-506
+ *   DC combination ................... channel.rs:220-226 (in FP64 here)
+ *
+ * Deliberate, specified differences from the reference (the new design):
+ *   - Random numbers: Philox4x32-10 keyed by the job seed with counter
+ *     (ion index in job, step, sub-block, domain) instead of xorshift64 +
+ *     xoshiro128** (channel.rs:343-373, channel.metal:38-44).
+ *   - FP64 position and velocity (the reference stores 62.5 pm fixed-point
+ *     positions and FP32 velocities, channel.metal:31-36), RF/AC phases
+ *     computed from the step counter in FP64 (the reference accumulates them
+ *     in FP32, channel.metal:91-92), step cap in steps.
+ *   - sin/cos/log are the self-written routines below, identical on the GPU,
+ *     so the FP64 GPU path can be compared bit for bit.
+ */
+#ifndef MQ_ORACLE_H
+#define MQ_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+#include "../include/mq_engine.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Philox4x32-10 block (Salmon et al., SC'11). */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* sin(2 pi p), cos(2 pi p); log(x) for positive normal x. */
+void or_sincos2pi(double p, double* s, double* c);
+double or_log(double x);
+
+/* Initial state of ion `ion` of job `job` (FP64). */
+typedef struct {
+    double x[3], v[3];
+    double phi_ac;
+    uint32_t steps, ncoll, cd;
+    uint8_t status;
+} or_ion;
+
+void or_inject(const mq_fields* f, const mq_job* job, const mq_physics* ph, uint32_t ion, or_ion* out);
+
+/* Runs every ion of every job to completion with `nthreads` CPU threads.
+ * out_* are arrays of sum(n) entries, job order. Returns 0 on success. */
+int or_run_batch(const mq_fields* f, const mq_job* jobs, size_t njobs, const double dc[4],
+                 const mq_physics* ph, int nthreads, or_ion* out, uint32_t* out_job);
+
+/* Runs jobs[0] ions [i0, i1) only (bounded CPU-baseline sample). */
+int or_run_range(const mq_fields* f, const mq_job* job, const double dc[4], const mq_physics* ph,
+                 uint32_t i0, uint32_t i1, int nthreads, or_ion* out);
+
+void or_summarize(const or_ion* ions, const uint32_t* job, size_t n, size_t njobs, double dt, mq_outcome* out);
+
+/* Trilinear sample of one node-major FP64 basis at grid coordinates g. */
+void or_sample(const double* F, int nx, int ny, int nz, const double g[3], int nearest, double out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

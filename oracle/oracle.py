@@ -1,0 +1,142 @@
+"""ctypes loader of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or the timed CPU baseline. The
+product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "multiq-workbench_amd"))
+import mqhip  # noqa: E402  (C struct definitions of include/mq_engine.h)
+
+
+class OrIon(C.Structure):
+    _fields_ = [("x", C.c_double * 3), ("v", C.c_double * 3), ("phi_ac", C.c_double),
+                ("steps", C.c_uint32), ("ncoll", C.c_uint32), ("cd", C.c_uint32), ("status", C.c_uint8)]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    u32p = C.POINTER(C.c_uint32)
+    lib.or_philox4x32_10.argtypes = [u32p, u32p, u32p]
+    lib.or_sincos2pi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.or_log.argtypes = [C.c_double]
+    lib.or_log.restype = C.c_double
+    lib.or_inject.argtypes = [C.POINTER(mqhip.Fields), C.POINTER(mqhip.Job), C.POINTER(mqhip.Physics), C.c_uint32,
+                              C.POINTER(OrIon)]
+    lib.or_run_batch.argtypes = [C.POINTER(mqhip.Fields), C.POINTER(mqhip.Job), C.c_size_t, C.POINTER(C.c_double),
+                                 C.POINTER(mqhip.Physics), C.c_int, C.POINTER(OrIon), u32p]
+    lib.or_run_range.argtypes = [C.POINTER(mqhip.Fields), C.POINTER(mqhip.Job), C.POINTER(C.c_double),
+                                 C.POINTER(mqhip.Physics), C.c_uint32, C.c_uint32, C.c_int, C.POINTER(OrIon)]
+    lib.or_summarize.argtypes = [C.POINTER(OrIon), u32p, C.c_size_t, C.c_size_t, C.c_double,
+                                 C.POINTER(mqhip.Outcome)]
+    lib.or_sample.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int,
+                              C.POINTER(C.c_double)]
+    _lib = lib
+    return lib
+
+
+def philox(ctr, key) -> list[int]:
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    load().or_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def sincos2pi(p: float) -> tuple[float, float]:
+    s, c = C.c_double(), C.c_double()
+    load().or_sincos2pi(p, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def log(x: float) -> float:
+    return load().or_log(x)
+
+
+def _ions_to_dict(ions, job=None) -> dict:
+    a = np.ctypeslib.as_array(ions)
+    out = {
+        "x": a["x"][:, 0].copy(), "y": a["x"][:, 1].copy(), "z": a["x"][:, 2].copy(),
+        "vx": a["v"][:, 0].copy(), "vy": a["v"][:, 1].copy(), "vz": a["v"][:, 2].copy(),
+        "phi_ac": a["phi_ac"].copy(), "steps": a["steps"].copy(), "ncoll": a["ncoll"].copy(),
+        "cd": a["cd"].copy(), "status": a["status"].copy(),
+    }
+    if job is not None:
+        out["job"] = np.ctypeslib.as_array(job).copy()
+    return out
+
+
+def run_batch(grid: dict, jobs, dc, ph, nthreads: int = 8) -> dict:
+    f, keep = mqhip.make_fields(grid)
+    ja = mqhip.jobs_array(jobs)
+    n = int(sum(j.n for j in jobs))
+    ions = (OrIon * max(n, 1))()
+    job = (C.c_uint32 * max(n, 1))()
+    rc = load().or_run_batch(C.byref(f), ja, len(jobs), (C.c_double * 4)(*dc), C.byref(ph), nthreads, ions, job)
+    if rc != 0:
+        raise RuntimeError("oracle run failed")
+    del keep
+    d = _ions_to_dict(ions, job)
+    return {k: v[:n] for k, v in d.items()}
+
+
+def run_range(grid: dict, job, dc, ph, i0: int, i1: int, nthreads: int = 1) -> dict:
+    f, keep = mqhip.make_fields(grid)
+    ions = (OrIon * max(i1 - i0, 1))()
+    rc = load().or_run_range(C.byref(f), C.byref(job), (C.c_double * 4)(*dc), C.byref(ph), i0, i1, nthreads, ions)
+    if rc != 0:
+        raise RuntimeError("oracle run failed")
+    del keep
+    d = _ions_to_dict(ions)
+    return {k: v[: i1 - i0] for k, v in d.items()}
+
+
+def inject(grid: dict, job, ph, ion: int) -> dict:
+    f, keep = mqhip.make_fields(grid)
+    o = OrIon()
+    load().or_inject(C.byref(f), C.byref(job), C.byref(ph), ion, C.byref(o))
+    del keep
+    return {"x": list(o.x), "v": list(o.v), "phi_ac": o.phi_ac, "cd": o.cd, "status": o.status}
+
+
+def summarize(res: dict, njobs: int, dt: float) -> list:
+    n = len(res["status"])
+    a = (OrIon * max(n, 1))()
+    arr = np.ctypeslib.as_array(a)
+    arr["status"][:n] = res["status"]
+    arr["steps"][:n] = res["steps"]
+    arr["ncoll"][:n] = res["ncoll"]
+    job = np.ascontiguousarray(res["job"], np.uint32)
+    out = (mqhip.Outcome * njobs)()
+    load().or_summarize(a, job.ctypes.data_as(C.POINTER(C.c_uint32)), n, njobs, dt, out)
+    return list(out)
+
+
+def sample(F: np.ndarray, nx: int, ny: int, nz: int, g, nearest: int = 0) -> np.ndarray:
+    F = np.ascontiguousarray(F, np.float64)
+    gg = (C.c_double * 3)(*g)
+    out = (C.c_double * 3)()
+    load().or_sample(F.ctypes.data_as(C.POINTER(C.c_double)), nx, ny, nz, gg, nearest, out)
+    return np.array(list(out))

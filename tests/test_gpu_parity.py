@@ -1,0 +1,206 @@
+"""GPU parity: the HIP paths through the C ABI against the CPU FP64 oracle.
+
+* FP64 plain path: per-ion outcomes, step counts and collision counts
+  identical, final positions and velocities within 1e-10 relative (the code
+  follows the oracle's operation order, so they are expected bit-identical).
+* FP32 batched path: outcome counts per class within a 3-sigma binomial bound
+  of the oracle on the same seeded inputs; per-ion results independent of the
+  batch composition and of the launch tuning; deterministic.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import mqhip
+import synth
+from gridutil import binom_pull
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("x", "y", "z", "vx", "vy", "vz")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def c1():
+    g = synth.c1_grid()
+    say("[gpu] creating engine on the 64x64x256 grid")
+    e = mqhip.Engine(g)
+    yield g, e
+    e.close()
+
+
+def physics(case):
+    base = mqhip.Physics.correct(0.133322).copy(dt=1e-7, max_steps=20000, injection=mqhip.INJ_GAUSSIAN,
+                                                inj_sigma_mm=0.1)
+    if case == "full_thermal":
+        return base
+    if case == "half_thermal":
+        return base.copy(legacy_force=1)
+    if case == "half_thermal_nearest":
+        return base.copy(legacy_force=1, legacy_fields=1)
+    if case == "thermal_disc":
+        return mqhip.Physics.correct(0.133322).copy(max_steps=20000, legacy_force=1)
+    if case == "legacy":
+        return mqhip.Physics.legacy(1e-3 * 133.322).copy(max_steps=20000)
+    if case == "legacy_trilinear":
+        return mqhip.Physics.legacy(1e-3 * 133.322).copy(max_steps=20000, legacy_fields=0)
+    raise KeyError(case)
+
+
+def jobs_for(case, n):
+    if case.startswith("legacy"):
+        return [mqhip.job(5 * 499.0, 5.0, 0.0, 100.0, 500e3, 0.5, 100e3, n, 1499),
+                mqhip.job(5 * 503.0, 5.0, 0.0, 100.0, 500e3, 0.5, 100e3, n // 2, 1503)]
+    return [mqhip.job(499.0, 1.0, 230.0, 100.0, 500e3, 0.5, 100e3, n, 1499),
+            mqhip.job(505.0, 1.0, 235.0, 100.0, 480e3, 0.8, 95e3, n // 2, 77, 0.5)]
+
+
+CASES = ["full_thermal", "half_thermal", "half_thermal_nearest", "thermal_disc", "legacy", "legacy_trilinear"]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_plain_fp64_matches_oracle(c1, oracle_mod, case):
+    g, e = c1
+    ph = physics(case)
+    jobs = jobs_for(case, 200)
+    say(f"[gpu] plain FP64 vs oracle: {case}, {sum(j.n for j in jobs)} ions")
+    got = e.run_batch_ions(jobs, [0, 5, 0, 0], ph, mqhip.PATH_PLAIN)
+    want = oracle_mod.run_batch(g, jobs, [0, 5, 0, 0], ph, 8)
+    assert np.array_equal(got["job"], want["job"])
+    assert np.array_equal(got["status"], want["status"])
+    assert np.array_equal(got["steps"], want["steps"])
+    assert np.array_equal(got["ncoll"], want["ncoll"])
+    worst = 0.0
+    for k in KEYS:
+        d = np.abs(got[k] - want[k]) / np.maximum(np.abs(want[k]), 1e-300)
+        worst = max(worst, float(d.max()))
+    say(f"[gpu]   max relative difference {worst:.3e}; bit-identical: "
+        f"{all(np.array_equal(got[k], want[k]) for k in KEYS)}")
+    assert worst <= 1e-10
+    assert all(np.array_equal(got[k], want[k]) for k in KEYS)
+
+
+@pytest.mark.parametrize("case", ["full_thermal", "half_thermal", "half_thermal_nearest", "legacy"])
+def test_batched_fp32_counts_within_3sigma(c1, oracle_mod, case):
+    g, e = c1
+    ph = physics(case)
+    jobs = jobs_for(case, 3000)
+    say(f"[gpu] batched FP32 vs oracle: {case}, {sum(j.n for j in jobs)} ions")
+    got = e.run_batch_ions(jobs, [0, 5, 0, 0], ph, mqhip.PATH_BATCHED)
+    want = oracle_mod.run_batch(g, jobs, [0, 5, 0, 0], ph, 8)
+    same = float(np.mean(got["status"] == want["status"]))
+    for j in range(len(jobs)):
+        sg, sw = got["status"][got["job"] == j], want["status"][want["job"] == j]
+        n = len(sw)
+        assert len(sg) == n
+        for k in range(1, 6):
+            pull = binom_pull(int((sg == k).sum()), n, int((sw == k).sum()), n)
+            say(f"[gpu]   job {j} status {k}: gpu {(sg == k).sum()} oracle {(sw == k).sum()} pull {pull:+.2f}")
+            assert abs(pull) <= 3.0
+    say(f"[gpu]   per-ion outcome identical to the FP64 oracle for {same:.4f} of ions")
+
+
+def test_batched_independent_of_batch_and_tuning(c1):
+    """Each ion's FP32 computation depends only on its own state and counters, so it
+    is bit-identical alone or in a batch, and under any launch shape (check.rs:22-60)."""
+    g, e = c1
+    ph = physics("half_thermal")
+    jobs = jobs_for("half_thermal", 500)
+    say("[gpu] batched path: batch-composition and tuning independence")
+    full = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+    alone = e.run_batch_ions(jobs[1:], [0, 5, 0, 0], ph)
+    sel = full["job"] == 1
+    for k in KEYS + ("steps", "ncoll", "status"):
+        assert np.array_equal(full[k][sel], alone[k]), k
+    t0 = e.get_tune()
+    try:
+        e.set_tune(block=64, waves_per_cu=3, refill_period=4, chunk=100, chunk_cap=400)
+        other = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+        assert e.stats()["launches"] > 1
+    finally:
+        e.set_tune(**t0)
+    for k in KEYS + ("steps", "ncoll", "status"):
+        assert np.array_equal(full[k], other[k]), k
+    again = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+    for k in KEYS + ("steps", "ncoll", "status"):
+        assert np.array_equal(full[k], again[k]), k
+
+
+def test_plain_independent_of_batch(c1):
+    g, e = c1
+    ph = physics("legacy")
+    jobs = jobs_for("legacy", 200)
+    say("[gpu] plain path: batch-composition independence")
+    full = e.run_batch_ions(jobs, [0, 5, 0, 0], ph, mqhip.PATH_PLAIN)
+    alone = e.run_batch_ions(jobs[1:], [0, 5, 0, 0], ph, mqhip.PATH_PLAIN)
+    sel = full["job"] == 1
+    for k in KEYS + ("steps", "ncoll", "status"):
+        assert np.array_equal(full[k][sel], alone[k]), k
+
+
+@pytest.mark.parametrize("path", [mqhip.PATH_BATCHED, mqhip.PATH_PLAIN])
+def test_outcomes_and_stats_consistent(c1, path):
+    g, e = c1
+    ph = physics("half_thermal")
+    jobs = jobs_for("half_thermal", 400)
+    say(f"[gpu] run_batch vs summarize(run_batch_ions), path {path}")
+    outs = e.run_batch(jobs, [0, 5, 0, 0], ph, path)
+    st = e.stats()
+    ions = e.run_batch_ions(jobs, [0, 5, 0, 0], ph, path)
+    ref = mqhip.summarize(ions["status"], ions["ncoll"], ions["steps"], ions["job"], len(jobs), ph.dt)
+    for a, b in zip(outs, ref):
+        assert a.as_dict() == b.as_dict()
+    assert st["ion_steps"] == int(ions["steps"].sum())
+    assert st["n_ions"] == len(ions["status"])
+    assert st["kernel_ms"] > 0 and st["launches"] >= 1
+
+
+@pytest.mark.parametrize("path", [mqhip.PATH_BATCHED, mqhip.PATH_PLAIN])
+def test_edge_cases(c1, oracle_mod, path):
+    g, e = c1
+    ph = physics("half_thermal")
+    say(f"[gpu] edge cases, path {path}")
+    # an empty job between two others, and a batch with no ions at all
+    jobs = [mqhip.job(499.0, 1.0, 230.0, 100.0, 500e3, 0.5, 100e3, 3, 1), mqhip.job(499.0, 1.0, 230.0, 100.0,
+            500e3, 0.5, 100e3, 0, 2), mqhip.job(499.0, 1.0, 230.0, 100.0, 500e3, 0.5, 100e3, 1, 3)]
+    outs = e.run_batch(jobs, [0, 5, 0, 0], ph, path)
+    assert [o.n for o in outs] == [3, 0, 1]
+    assert outs[1].mean_time == 0.0
+    outs = e.run_batch(jobs[1:2], [0, 5, 0, 0], ph, path)
+    assert outs[0].n == 0 and e.stats()["ion_steps"] == 0
+    # a one-step cap: every ion ends after exactly one step
+    r = e.run_batch_ions(jobs, [0, 5, 0, 0], ph.copy(max_steps=1), path)
+    assert np.all(r["steps"] == 1) and np.all(r["status"] != 0)
+    # ions injected outside the grid leave the box on their first step
+    g2 = dict(g)
+    g2["axis_x"] = -1e-3
+    e2 = mqhip.Engine(g2)
+    try:
+        r = e2.run_batch_ions(jobs, [0, 5, 0, 0], ph, path)
+        assert np.all(r["status"] == mqhip.LEFT) and np.all(r["steps"] == 1)
+        w = oracle_mod.run_batch(g2, jobs, [0, 5, 0, 0], ph, 1)
+        assert np.array_equal(w["status"], r["status"])
+    finally:
+        e2.close()
+
+
+def test_rejects_bad_arguments(c1):
+    g, e = c1
+    ph = physics("half_thermal")
+    with pytest.raises(mqhip.MqError, match="no jobs"):
+        e.run_batch([], [0, 0, 0, 0], ph)
+    with pytest.raises(mqhip.MqError, match="path"):
+        e.run_batch(jobs_for("half_thermal", 2), [0, 0, 0, 0], ph, 7)
+    with pytest.raises(mqhip.MqError, match="dt"):
+        e.run_batch(jobs_for("half_thermal", 2), [0, 0, 0, 0], ph.copy(dt=0.0))
+    with pytest.raises(mqhip.MqError, match="block"):
+        e.set_tune(block=100)
+    with pytest.raises(mqhip.MqError, match="refill"):
+        e.set_tune(refill_period=2)
