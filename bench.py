@@ -133,11 +133,22 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one torch elementwise copy of 1 GiB (read 1 GiB, write 1 GiB) first, as the "
+                         "known-size reference for rocprofv3 FETCH_SIZE / WRITE_SIZE")
     a = ap.parse_args()
     if a.gpus != 1:
         sys.exit("bench.py runs on one GPU only: the integrator has no multi-GPU path in this build "
                  "(ions are independent; see the sharding design). Use --gpus 1.")
 
+    if a.copy_calibration:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy calibration: 1 GiB read + 1 GiB written by one elementwise kernel")
     log(f"building the {a.config} workload")
     grid, jobs, dc, ph, desc = workload(a.config)
     n_traj = int(sum(j.n for j in jobs))
