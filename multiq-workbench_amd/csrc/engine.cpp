@@ -131,7 +131,7 @@ struct mq_engine {
     bool cells_valid = false;
     DevBuf cells;  // FP32 cell records
     // per-call pools
-    DevBuf jobs, acc, list[2], ctr;
+    DevBuf jobs, acc, list[3], ctr, bin_of, rank_of, hist, bsum;
     DevBuf sx, sy, sz, sphi, svx, svy, svz, sfx, sfy, sfz, ssteps, sncoll, scd, sjob, sstatus;
     mq_tune tune{256, 8, 8, 20000, 200000, 0};
     mq_run_stats stats{};
@@ -180,7 +180,7 @@ struct mq_engine {
     void destroy()
     {
         for (DevBuf* b : {&rf, &ac, &dcb[0], &dcb[1], &dcb[2], &dcb[3], &mask, &dcc, &cells, &jobs, &acc, &list[0],
-                          &list[1], &ctr, &sx, &sy, &sz, &sphi, &svx, &svy, &svz, &sfx, &sfy, &sfz, &ssteps,
+                          &list[1], &list[2], &ctr, &bin_of, &rank_of, &hist, &bsum, &sx, &sy, &sz, &sphi, &svx, &svy, &svz, &sfx, &sfy, &sfz, &ssteps,
                           &sncoll, &scd, &sjob, &sstatus})
             b->release();
         if (ev0) (void)hipEventDestroy(ev0);
@@ -331,15 +331,39 @@ struct mq_engine {
         }
     }
 
+    SortGeom sort_geom() const
+    {
+        SortGeom G{};
+        G.sx = G.sy = G.sz = 2;  // 4 x 4 x 4 cells per bin
+        G.nbx = (uint32_t)((nx - 2) >> G.sx) + 1;
+        G.nby = (uint32_t)((ny - 2) >> G.sy) + 1;
+        G.nbz = (uint32_t)((nz - 2) >> G.sz) + 1;
+        G.nbins = G.nbx * G.nby * G.nbz;
+        return G;
+    }
+
+    // Persistent-worker launches (run_tail, tiled.rs:159-218): each launch advances
+    // every listed ion by up to `chunk` steps and appends the unfinished ones to the
+    // next list; the chunk doubles up to its cap. With re-sort on, the list is
+    // sorted by cell before every launch and consumed as one segment per XCD.
     void run_batched(const IonState& S, const BatchP& B, uint32_t n, float& kms)
     {
-        list[0].ensure((size_t)n * 4);
-        list[1].ensure((size_t)n * 4);
-        ctr.ensure(256);
+        const bool resort = tune.resort != 0;
+        for (int k = 0; k < (resort ? 3 : 2); ++k) list[k].ensure((size_t)n * 4);
+        ctr.ensure(2 * CTR_BLOCK * 4);
         uint32_t* c0 = ctr.as<uint32_t>();
-        uint32_t* c1 = c0 + 16;
+        uint32_t* c1 = c0 + CTR_BLOCK;
+        SortGeom G{};
+        if (resort) {
+            G = sort_geom();
+            if (G.nbins > 1024u * 1024u) throw Err(-1, "grid too large for the re-sort bins");
+            bin_of.ensure((size_t)n * 4);
+            rank_of.ensure((size_t)n * 4);
+            hist.ensure((size_t)G.nbins * 4);
+            bsum.ensure(1024 * 4);
+        }
         launch_iota(list[0].as<uint32_t>(), n, stream);
-        launch_fill_u32(c0, n, 2, stream);  // [1] = count
+        launch_fill_u32(c0 + CTR_COUNT, n, 1, stream);
         const uint32_t block = (uint32_t)tune.block;
         const uint32_t grid = std::max<uint32_t>(1, (uint32_t)((uint64_t)cus * tune.waves_per_cu * 64 / block));
         uint32_t chunk = ((uint32_t)tune.chunk + 3u) & ~3u;
@@ -348,17 +372,27 @@ struct mq_engine {
         kms = 0.0f;
         int cur = 0;
         for (;;) {
+            uint32_t* cin = cur ? c1 : c0;
+            uint32_t* cout = cur ? c0 : c1;
+            const uint32_t* lin = list[cur].as<uint32_t>();
+            if (resort) {
+                launch_sort(S, B, lin, cin, list[2].as<uint32_t>(), bin_of.as<uint32_t>(), rank_of.as<uint32_t>(),
+                            hist.as<uint32_t>(), bsum.as<uint32_t>(), G, n, stream);
+                HIPCHK(hipGetLastError());
+                lin = list[2].as<uint32_t>();
+            }
             WorkerArgs a;
             a.S = S;
             a.jobs = jobs.as<JobP>();
             a.B = B;
             a.cells = cells.as<float4>();
-            a.list_in = list[cur].as<uint32_t>();
+            a.list_in = lin;
             a.list_out = list[cur ^ 1].as<uint32_t>();
-            a.ctr_in = cur ? c1 : c0;
-            a.ctr_out = cur ? c0 : c1;
+            a.ctr_in = cin;
+            a.ctr_out = cout;
             a.chunk = chunk;
             a.period = period;
+            a.nqueues = resort ? MAX_QUEUES : 1u;
             launch_prepare(a, stream);
             HIPCHK(hipEventRecord(ev0, stream));
             launch_batched(a, grid, block, stream);
@@ -366,7 +400,7 @@ struct mq_engine {
             HIPCHK(hipEventRecord(ev1, stream));
             stats.launches += 1;
             uint32_t alive = 0;
-            HIPCHK(hipMemcpyAsync(&alive, a.ctr_out + 1, 4, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipMemcpyAsync(&alive, cout + CTR_COUNT, 4, hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
             float ms = 0.0f;
             HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));

@@ -20,7 +20,13 @@ void launch_build_cells(float4* cells, const double* rf, const double* dc, const
                         int nx, int ny, int nz, hipStream_t st);
 void launch_iota(uint32_t* list, uint32_t n, hipStream_t st);
 
-// Counters block used by one worker launch: [0] head, [1] count_in, [2] count_out.
+// Counter block of one live list: queue heads at [q * CTR_Q], q < MAX_QUEUES,
+// and the list length at [CTR_COUNT]; CTR_BLOCK u32 per block.
+constexpr uint32_t MAX_QUEUES = 8;
+constexpr uint32_t CTR_Q = 32;
+constexpr uint32_t CTR_COUNT = MAX_QUEUES * CTR_Q;
+constexpr uint32_t CTR_BLOCK = CTR_COUNT + 32;
+
 struct WorkerArgs {
     IonState S;
     const JobP* jobs;
@@ -28,11 +34,20 @@ struct WorkerArgs {
     const float4* cells;
     const uint32_t* list_in;
     uint32_t* list_out;
-    uint32_t* ctr_in;   // [0] head, [1] count
-    uint32_t* ctr_out;  // [1] count of ions carried to the next launch
+    uint32_t* ctr_in;   // heads + count of list_in
+    uint32_t* ctr_out;  // count of list_out (ions carried to the next launch)
     uint32_t chunk;     // max steps per ion in this launch (multiple of 4)
     uint32_t period;    // steps between refills (multiple of 4)
+    uint32_t nqueues;   // 1, or MAX_QUEUES contiguous segments (one per XCD)
 };
+
+// Coarse cell bins of the re-sort: bin = ((k >> sz) nby + (j >> sy)) nbx + (i >> sx).
+struct SortGeom {
+    uint32_t sx, sy, sz, nbx, nby, nbz, nbins;
+};
+void launch_sort(const IonState& S, const BatchP& B, const uint32_t* list, const uint32_t* ctr, uint32_t* out,
+                 uint32_t* bin_of, uint32_t* rank_of, uint32_t* hist, uint32_t* bsum, const SortGeom& G,
+                 uint32_t nmax, hipStream_t st);
 void launch_prepare(const WorkerArgs& a, hipStream_t st);
 void launch_batched(const WorkerArgs& a, uint32_t grid, uint32_t block, hipStream_t st);
 }  // namespace mq

@@ -71,20 +71,29 @@ __global__ void k_build_cells(float4* __restrict__ cells, const double* __restri
     for (int q = 0; q < CELL_F4; ++q) dst[q] = make_float4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
 }
 
+__global__ void k_fill_hist(uint32_t* hist, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) hist[i] = 0u;
+}
+
 __global__ void k_iota(uint32_t* list, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) list[i] = i;
 }
 
-// ctr_in[0] = head of the current launch, ctr_out[0..1] = next launch's head and count.
+// Counter block: queue heads at [q * CTR_Q] (one 128-byte line each), the list
+// length at [CTR_COUNT]. Zero the heads of the launch about to run and the
+// whole block of the next one.
 __global__ void k_prepare(uint32_t* ctr_in, uint32_t* ctr_out)
 {
-    if (threadIdx.x == 0) {
-        ctr_in[0] = 0;
-        ctr_out[0] = 0;
-        ctr_out[1] = 0;
+    const uint32_t t = threadIdx.x;
+    if (t < MAX_QUEUES) {
+        ctr_in[t * CTR_Q] = 0;
+        ctr_out[t * CTR_Q] = 0;
     }
+    if (t == 0) ctr_out[CTR_COUNT] = 0;
 }
 
 // ------------------------------------------------------------- the worker
@@ -166,7 +175,9 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
     const BatchP& B = A.B;
     const IonState& S = A.S;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t count = A.ctr_in[1];
+    const uint32_t count = A.ctr_in[CTR_COUNT];
+    const uint32_t nq = A.nqueues;
+    uint32_t q = nq > 1 ? blockIdx.x % nq : 0, tried = 0;
     const double dt = B.dt;
     const float dtf = (float)B.dt;
     Lane L;
@@ -174,19 +185,25 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
     bool have = false, drained = false;
     L.cell = -1;
     for (;;) {
-        // ---- refill idle lanes from the live list (one atomic per wavefront)
+        // ---- refill idle lanes from the live list (one atomic per wavefront). With
+        // several queues the list is cut into contiguous (spatially sorted) segments
+        // and the wavefronts of one XCD start on their own segment, then steal.
         const uint64_t idle = __ballot(!have);
         if (idle && !drained) {
             const uint32_t nidle = (uint32_t)__popcll(idle);
             const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1;
+            const uint32_t s0 = (uint32_t)((uint64_t)count * q / nq), s1 = (uint32_t)((uint64_t)count * (q + 1) / nq);
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&A.ctr_in[0], nidle);
+            if (lane == leader) base = atomicAdd(&A.ctr_in[q * CTR_Q], nidle);
             base = __shfl(base, (int)leader);
-            if (base + nidle >= count) drained = true;
-            if (!have) {
+            const uint32_t avail = s1 - s0;
+            if (base >= avail) {
+                if (++tried >= nq) drained = true;
+                else q = (q + 1 == nq) ? 0 : q + 1;
+            } else if (!have) {
                 const uint32_t slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (slot < count) {
-                    const uint32_t id = A.list_in[slot];
+                if (slot < avail) {
+                    const uint32_t id = A.list_in[s0 + slot];
                     const JobP& J = A.jobs[S.job[id]];
                     L.id = id;
                     L.ion = (uint32_t)(id - J.off);
@@ -307,7 +324,7 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
         if (cm) {
             const uint32_t leader = (uint32_t)__ffsll((unsigned long long)cm) - 1;
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&A.ctr_out[1], (uint32_t)__popcll(cm));
+            if (lane == leader) base = atomicAdd(&A.ctr_out[CTR_COUNT], (uint32_t)__popcll(cm));
             base = __shfl(base, (int)leader);
             if (carry) A.list_out[base + (uint32_t)__popcll(cm & ((1ull << lane) - 1ull))] = L.id;
         }
@@ -322,7 +339,103 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
     }
 }
 
+// ------------------------------------------------- re-sort by grid cell
+// Counting sort of the live list by coarse cell bin (z-major, x fastest):
+// k_bin counts, the three scan kernels turn counts into offsets, k_scatter
+// places each ion. Order inside a bin follows the atomics; per-ion results do
+// not depend on list order.
+__global__ void k_bin(IonState S, BatchP B, const uint32_t* __restrict__ list, const uint32_t* __restrict__ ctr,
+                      uint32_t* __restrict__ bin_of, uint32_t* __restrict__ rank_of, uint32_t* __restrict__ hist,
+                      SortGeom G)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ctr[CTR_COUNT]) return;
+    const uint32_t id = list[i];
+    const int ci = min((int)fmin(fmax(S.x[id] * B.inv_h, 0.0), (double)(B.nx - 1)), B.nx - 2);
+    const int cj = min((int)fmin(fmax(S.y[id] * B.inv_h, 0.0), (double)(B.ny - 1)), B.ny - 2);
+    const int ck = min((int)fmin(fmax(S.z[id] * B.inv_h, 0.0), (double)(B.nz - 1)), B.nz - 2);
+    const uint32_t bin = (((uint32_t)ck >> G.sz) * G.nby + ((uint32_t)cj >> G.sy)) * G.nbx + ((uint32_t)ci >> G.sx);
+    bin_of[i] = bin;
+    rank_of[i] = atomicAdd(&hist[bin], 1u);
+}
+
+// Exclusive scan of hist in place: blocks of 1024 entries (256 threads x 4).
+__global__ __launch_bounds__(256) void k_scan_blocks(uint32_t* __restrict__ hist, uint32_t* __restrict__ bsum, uint32_t n)
+{
+    __shared__ uint32_t wsum[4];
+    const uint32_t t = threadIdx.x, base = blockIdx.x * 1024 + t * 4;
+    uint32_t v[4], s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { v[k] = base + k < n ? hist[base + k] : 0u; s += v[k]; }
+    const uint32_t lane = t & 63, w = t >> 6;
+    uint32_t incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o);
+        if (lane >= (uint32_t)o) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (uint32_t k = 0; k < w; ++k) woff += wsum[k];
+    uint32_t run = woff + incl - s;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) hist[base + k] = run;
+        run += v[k];
+    }
+    if (t == 255) bsum[blockIdx.x] = woff + incl;
+}
+
+// Exclusive scan of the block sums (at most 1024) by one block.
+__global__ __launch_bounds__(1024) void k_scan_sums(uint32_t* __restrict__ bsum, uint32_t nb)
+{
+    __shared__ uint32_t wsum[16];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t s = t < nb ? bsum[t] : 0u;
+    uint32_t incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o);
+        if (lane >= (uint32_t)o) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (uint32_t k = 0; k < w; ++k) woff += wsum[k];
+    if (t < nb) bsum[t] = woff + incl - s;
+}
+
+__global__ void k_scan_add(uint32_t* __restrict__ hist, const uint32_t* __restrict__ bsum, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) hist[i] += bsum[i >> 10];
+}
+
+__global__ void k_scatter(const uint32_t* __restrict__ list, const uint32_t* __restrict__ ctr,
+                          const uint32_t* __restrict__ bin_of, const uint32_t* __restrict__ rank_of,
+                          const uint32_t* __restrict__ offs, uint32_t* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ctr[CTR_COUNT]) return;
+    out[offs[bin_of[i]] + rank_of[i]] = list[i];
+}
+
 static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+void launch_sort(const IonState& S, const BatchP& B, const uint32_t* list, const uint32_t* ctr, uint32_t* out,
+                 uint32_t* bin_of, uint32_t* rank_of, uint32_t* hist, uint32_t* bsum, const SortGeom& G,
+                 uint32_t nmax, hipStream_t st)
+{
+    const uint32_t nb = G.nbins;
+    k_fill_hist<<<nblk(nb, 256), 256, 0, st>>>(hist, nb);
+    if (nmax) k_bin<<<nblk(nmax, 256), 256, 0, st>>>(S, B, list, ctr, bin_of, rank_of, hist, G);
+    const uint32_t nblocks = nblk(nb, 1024);
+    k_scan_blocks<<<nblocks, 256, 0, st>>>(hist, bsum, nb);
+    k_scan_sums<<<1, 1024, 0, st>>>(bsum, nblocks);
+    k_scan_add<<<nblk(nb, 256), 256, 0, st>>>(hist, bsum, nb);
+    if (nmax) k_scatter<<<nblk(nmax, 256), 256, 0, st>>>(list, ctr, bin_of, rank_of, hist, out);
+}
 
 void launch_build_cells(float4* cells, const double* rf, const double* dc, const double* ac, const uint8_t* mask,
                         int nx, int ny, int nz, hipStream_t st)

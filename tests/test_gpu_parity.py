@@ -124,10 +124,14 @@ def test_batched_independent_of_batch_and_tuning(c1):
         e.set_tune(block=64, waves_per_cu=3, refill_period=4, chunk=100, chunk_cap=400)
         other = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
         assert e.stats()["launches"] > 1
+        e.set_tune(block=128, waves_per_cu=8, refill_period=8, chunk=64, chunk_cap=64, resort=1)
+        sorted_run = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+        assert e.stats()["launches"] > 4
     finally:
         e.set_tune(**t0)
     for k in KEYS + ("steps", "ncoll", "status"):
         assert np.array_equal(full[k], other[k]), k
+        assert np.array_equal(full[k], sorted_run[k]), k
     again = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
     for k in KEYS + ("steps", "ncoll", "status"):
         assert np.array_equal(full[k], again[k]), k
