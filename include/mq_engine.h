@@ -166,6 +166,46 @@ int mq_last_stats(const mq_engine* eng, mq_run_stats* out);
 int mq_set_tune(mq_engine* eng, const mq_tune* t);
 int mq_get_tune(const mq_engine* eng, mq_tune* out);
 
+/* ------------------------------------------------------------ scan API */
+
+/* One point of an adaptive scan: AC frequency [kHz], transmitted and crashed
+ * fractions, ions simulated (scan.rs:51, the (f, T, C, n) tuple). */
+typedef struct { double f_khz, transmitted, crashed; uint64_t n; } mq_scan_point;
+
+/* Replaces scan::cached_batch (scan.rs:31-47): outcomes of jobs whose key is
+ * already in the TSV file `cache_path` are read from it, the rest run as one
+ * batch and are appended. geom_key is the caller's geometry key (the Rust
+ * Debug string of Geom in the reference). *n_computed (optional) receives the
+ * number of jobs run. */
+int mq_cached_batch(mq_engine* eng, const char* geom_key, const mq_job* jobs, size_t njobs, const double dc[4],
+                    const mq_physics* ph, int32_t path, const char* cache_path, mq_outcome* out,
+                    size_t* n_computed);
+
+/* Replaces scan::adaptive_scan (scan.rs:52-78): coarse AC-frequency scan
+ * f_lo..f_hi [kHz] step f_step with n_lo ions (seeds seed0 + i), then a fine
+ * scan at f_step/4 with n_hi ions (seeds seed0 + 100000 + i) around points with
+ * T >= 0.35 T_max and T - base > 4 sigma; both through mq_cached_batch.
+ * Writes up to max_out points sorted by f; *n_out = number of points. Returns
+ * 0, < 0 on error, or the required size if max_out is too small. */
+int mq_adaptive_scan(mq_engine* eng, const char* geom_key, const mq_species* sp, const mq_rfac* rf,
+                     const double dc[4], const mq_physics* ph, double f_lo, double f_hi, double f_step,
+                     uint64_t n_lo, uint64_t n_hi, double base, uint64_t seed0, int32_t path,
+                     const char* cache_path, mq_scan_point* out, size_t max_out, size_t* n_out);
+
+/* The refinement rule of adaptive_scan (scan.rs:59-67) on given coarse
+ * points; same return convention as mq_adaptive_scan. */
+int mq_adaptive_fine_points(const double* f_khz, const double* t, size_t nf, uint64_t n_lo, double base,
+                            double f_step, double* out, size_t max_out, size_t* n_out);
+
+/* The cache key of one job (scan.rs:9-11) into buf; returns 0, < 0 on error,
+ * or the required length if len is too small. */
+int mq_scan_key(const char* geom_key, const mq_job* job, const double dc[4], const mq_physics* ph, char* buf,
+                size_t len);
+
+/* An f64 in Rust Debug ({:?}, display = 0) or Display ({}, display = 1)
+ * notation, as used in the cache file; same return convention as mq_scan_key. */
+int mq_format_f64(double x, int32_t display, char* buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -414,6 +414,8 @@ struct mq_engine {
 
 // ------------------------------------------------------------------ C ABI
 
+void mq_set_last_error(const std::string& msg) { g_err = msg; }
+
 #define MQ_GUARD(...)                                       \
     try {                                                   \
         g_err.clear();                                      \

@@ -108,6 +108,10 @@ class Tune(C.Structure):
                 ("chunk", C.c_int32), ("chunk_cap", C.c_int32), ("resort", C.c_int32)]
 
 
+class ScanPoint(C.Structure):
+    _fields_ = [("f_khz", C.c_double), ("transmitted", C.c_double), ("crashed", C.c_double), ("n", C.c_uint64)]
+
+
 # Exported symbols of include/mq_engine.h with their ctypes signatures.
 SIGNATURES = {
     "mq_last_error": (C.c_char_p, []),
@@ -122,6 +126,19 @@ SIGNATURES = {
     "mq_last_stats": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
     "mq_set_tune": (C.c_int, [C.c_void_p, C.POINTER(Tune)]),
     "mq_get_tune": (C.c_int, [C.c_void_p, C.POINTER(Tune)]),
+    "mq_cached_batch": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(Job), C.c_size_t, C.POINTER(C.c_double),
+                                  C.POINTER(Physics), C.c_int32, C.c_char_p, C.POINTER(Outcome),
+                                  C.POINTER(C.c_size_t)]),
+    "mq_adaptive_scan": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(Species), C.POINTER(RfAc), C.POINTER(C.c_double),
+                                   C.POINTER(Physics), C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint64,
+                                   C.c_double, C.c_uint64, C.c_int32, C.c_char_p, C.POINTER(ScanPoint), C.c_size_t,
+                                   C.POINTER(C.c_size_t)]),
+    "mq_adaptive_fine_points": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_size_t, C.c_uint64,
+                                          C.c_double, C.c_double, C.POINTER(C.c_double), C.c_size_t,
+                                          C.POINTER(C.c_size_t)]),
+    "mq_scan_key": (C.c_int, [C.c_char_p, C.POINTER(Job), C.POINTER(C.c_double), C.POINTER(Physics), C.c_char_p,
+                              C.c_size_t]),
+    "mq_format_f64": (C.c_int, [C.c_double, C.c_int32, C.c_char_p, C.c_size_t]),
 }
 
 _lib = None
@@ -272,3 +289,53 @@ def summarize(status, ncoll, steps, job, njobs: int, dt: float) -> list[Outcome]
     _check(load().mq_summarize(status.ctypes.data_as(C.POINTER(C.c_uint8)), ncoll.ctypes.data_as(u32),
                                steps.ctypes.data_as(u32), job.ctypes.data_as(u32), len(status), njobs, dt, out))
     return list(out)
+
+
+# ------------------------------------------------------------------ scan API
+
+def format_f64(x: float, display: bool = False) -> str:
+    """Rust {:?} (display=False) or {} (display=True) notation of an f64."""
+    buf = C.create_string_buffer(64)
+    _check(load().mq_format_f64(x, int(display), buf, len(buf)))
+    return buf.value.decode()
+
+
+def scan_key(geom_key: str, j: Job, dc, ph: Physics) -> str:
+    """Cache key of one job (scan.rs:9-11)."""
+    buf = C.create_string_buffer(4096)
+    _check(load().mq_scan_key(geom_key.encode(), C.byref(j), (C.c_double * 4)(*dc), C.byref(ph), buf, len(buf)))
+    return buf.value.decode()
+
+
+def adaptive_fine_points(f_khz, t, n_lo: int, base: float, f_step: float) -> list[float]:
+    f = np.ascontiguousarray(f_khz, np.float64)
+    tt = np.ascontiguousarray(t, np.float64)
+    n = C.c_size_t()
+    out = np.zeros(4 * len(f) * 16 + 8)
+    _check(load().mq_adaptive_fine_points(_dptr(f), _dptr(tt), len(f), n_lo, base, f_step, _dptr(out), len(out),
+                                          C.byref(n)))
+    return list(out[: n.value])
+
+
+def cached_batch(eng: "Engine", geom_key: str, jobs, dc, ph: Physics, cache_path: str,
+                 path: int = PATH_BATCHED) -> tuple[list[Outcome], int]:
+    """scan::cached_batch (scan.rs:31-47); returns (outcomes, jobs computed)."""
+    ja = jobs_array(jobs)
+    out = (Outcome * len(jobs))()
+    nc = C.c_size_t()
+    _check(load().mq_cached_batch(eng._h, geom_key.encode(), ja, len(jobs), (C.c_double * 4)(*dc), C.byref(ph), path,
+                                  cache_path.encode(), out, C.byref(nc)))
+    return list(out), nc.value
+
+
+def adaptive_scan(eng: "Engine", geom_key: str, sp: Species, rf: RfAc, dc, ph: Physics, f_lo: float, f_hi: float,
+                  f_step: float, n_lo: int, n_hi: int, base: float, seed0: int, cache_path: str,
+                  path: int = PATH_BATCHED) -> list[tuple]:
+    """scan::adaptive_scan (scan.rs:52-78): sorted (f_kHz, T, C, n) points."""
+    cap = 8 * int((f_hi - f_lo) / f_step + 2) + 16
+    out = (ScanPoint * cap)()
+    n = C.c_size_t()
+    _check(load().mq_adaptive_scan(eng._h, geom_key.encode(), C.byref(sp), C.byref(rf), (C.c_double * 4)(*dc),
+                                   C.byref(ph), f_lo, f_hi, f_step, n_lo, n_hi, base, seed0, path, cache_path.encode(),
+                                   out, cap, C.byref(n)))
+    return [(p.f_khz, p.transmitted, p.crashed, p.n) for p in out[: n.value]]
