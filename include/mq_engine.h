@@ -122,7 +122,7 @@ typedef struct {
 /* Tuning of the batched path (TileCfg, tiled.rs:12-29). Zero fields keep
  * the current value. */
 typedef struct {
-    int32_t block;           /* threads per workgroup (64..1024, multiple of 64) */
+    int32_t block;           /* threads per workgroup (64..256, multiple of 64) */
     int32_t waves_per_cu;    /* persistent waves launched per CU */
     int32_t refill_period;   /* steps between idle-lane refills (compaction period) */
     int32_t chunk;           /* max steps per ion per launch (doubles per launch) */

@@ -553,7 +553,8 @@ int mq_set_tune(mq_engine* eng, const mq_tune* t)
         if (!eng || !t) throw Err(-1, "NULL argument");
         mq_tune n = eng->tune;
         if (t->block) {
-            if (t->block < 64 || t->block > 1024 || t->block % 64) throw Err(-1, "block must be a multiple of 64 in [64, 1024]");
+            // k_batched is compiled for at most 256 threads per workgroup (__launch_bounds__)
+            if (t->block < 64 || t->block > 256 || t->block % 64) throw Err(-1, "block must be a multiple of 64 in [64, 256]");
             n.block = t->block;
         }
         if (t->waves_per_cu) {

@@ -206,5 +206,7 @@ def test_rejects_bad_arguments(c1):
         e.run_batch(jobs_for("half_thermal", 2), [0, 0, 0, 0], ph.copy(dt=0.0))
     with pytest.raises(mqhip.MqError, match="block"):
         e.set_tune(block=100)
+    with pytest.raises(mqhip.MqError, match="block"):
+        e.set_tune(block=512)
     with pytest.raises(mqhip.MqError, match="refill"):
         e.set_tune(refill_period=2)
