@@ -119,15 +119,21 @@ typedef struct {
     double wall_ms;          /* host wall time of the call */
 } mq_run_stats;
 
-/* Tuning of the batched path (TileCfg, tiled.rs:12-29). Zero fields keep
- * the current value. */
+/* Tuning of the batched path (TileCfg, tiled.rs:12-29). mq_set_tune: zero
+ * in block, waves_per_cu, refill_period, chunk, chunk_cap, sync_every keeps
+ * the current value; resort and graph are always taken. Results per ion do
+ * not depend on any of these. */
 typedef struct {
     int32_t block;           /* threads per workgroup (64..256, multiple of 64) */
     int32_t waves_per_cu;    /* persistent waves launched per CU */
     int32_t refill_period;   /* steps between idle-lane refills (compaction period) */
-    int32_t chunk;           /* max steps per ion per launch (doubles per launch) */
-    int32_t chunk_cap;       /* chunk growth cap */
-    int32_t resort;          /* 1: sort the live list by grid cell between launches */
+    int32_t chunk;           /* max steps per ion per launch (the re-sort interval) */
+    int32_t chunk_cap;       /* without re-sort the chunk doubles per launch up to this */
+    int32_t resort;          /* 1: sort the live list by grid cell before every launch,
+                                one queue per XCD over the sorted list */
+    int32_t sync_every;      /* launches enqueued per host check of the live count */
+    int32_t graph;           /* 1: capture those launches as one hipGraph and replay it
+                                (sync_every must be even) */
 } mq_tune;
 
 typedef struct mq_engine mq_engine;

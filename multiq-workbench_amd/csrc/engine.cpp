@@ -133,7 +133,8 @@ struct mq_engine {
     // per-call pools
     DevBuf jobs, acc, list[3], ctr, bin_of, rank_of, hist, bsum;
     DevBuf sx, sy, sz, sphi, svx, svy, svz, sfx, sfy, sfz, ssteps, sncoll, scd, sjob, sstatus;
-    mq_tune tune{256, 8, 8, 20000, 200000, 0};
+    mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0};
+    std::vector<hipEvent_t> evs;
     mq_run_stats stats{};
 
     size_t nodes() const { return (size_t)nx * ny * nz; }
@@ -183,6 +184,8 @@ struct mq_engine {
                           &list[1], &list[2], &ctr, &bin_of, &rank_of, &hist, &bsum, &sx, &sy, &sz, &sphi, &svx, &svy, &svz, &sfx, &sfy, &sfz, &ssteps,
                           &sncoll, &scd, &sjob, &sstatus})
             b->release();
+        for (auto e : evs) (void)hipEventDestroy(e);
+        evs.clear();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -344,15 +347,56 @@ struct mq_engine {
 
     // Persistent-worker launches (run_tail, tiled.rs:159-218): each launch advances
     // every listed ion by up to `chunk` steps and appends the unfinished ones to the
-    // next list; the chunk doubles up to its cap. With re-sort on, the list is
-    // sorted by cell before every launch and consumed as one segment per XCD.
+    // next list. With re-sort on, the list is sorted by cell before every launch
+    // and consumed as one segment per XCD, and the chunk stays fixed (it is the
+    // re-sort interval); otherwise the chunk doubles up to its cap. `sync_every`
+    // launches are enqueued back to back (every kernel reads its list length from
+    // device memory, so launches past the end of the work are empty) and the host
+    // checks the live count once per block; with `graph` on, the block is
+    // captured once as a hipGraph and replayed.
+    void enqueue_block(const IonState& S, const BatchP& B, uint32_t n, const SortGeom& G, int& cur, uint32_t& chunk,
+                       uint32_t cap, uint32_t period, uint32_t grid, uint32_t block, int iters, bool timed,
+                       std::vector<hipEvent_t>* evs)
+    {
+        const bool resort = tune.resort != 0;
+        uint32_t* c0 = ctr.as<uint32_t>();
+        uint32_t* c1 = c0 + CTR_BLOCK;
+        for (int it = 0; it < iters; ++it) {
+            uint32_t* cin = cur ? c1 : c0;
+            uint32_t* cout = cur ? c0 : c1;
+            const uint32_t* lin = list[cur].as<uint32_t>();
+            if (resort) {
+                launch_sort(S, B, lin, cin, list[2].as<uint32_t>(), bin_of.as<uint32_t>(), rank_of.as<uint32_t>(),
+                            hist.as<uint32_t>(), bsum.as<uint32_t>(), G, n, stream);
+                lin = list[2].as<uint32_t>();
+            }
+            WorkerArgs a;
+            a.S = S;
+            a.jobs = jobs.as<JobP>();
+            a.B = B;
+            a.cells = cells.as<float4>();
+            a.list_in = lin;
+            a.list_out = list[cur ^ 1].as<uint32_t>();
+            a.ctr_in = cin;
+            a.ctr_out = cout;
+            a.chunk = chunk;
+            a.period = period;
+            a.nqueues = resort ? MAX_QUEUES : 1u;
+            launch_prepare(a, stream);
+            if (timed) HIPCHK(hipEventRecord((*evs)[2 * it], stream));
+            launch_batched(a, grid, block, stream);
+            if (timed) HIPCHK(hipEventRecord((*evs)[2 * it + 1], stream));
+            cur ^= 1;
+            if (!resort) chunk = std::min(cap, chunk * 2u);
+        }
+    }
+
     void run_batched(const IonState& S, const BatchP& B, uint32_t n, float& kms)
     {
         const bool resort = tune.resort != 0;
         for (int k = 0; k < (resort ? 3 : 2); ++k) list[k].ensure((size_t)n * 4);
         ctr.ensure(2 * CTR_BLOCK * 4);
         uint32_t* c0 = ctr.as<uint32_t>();
-        uint32_t* c1 = c0 + CTR_BLOCK;
         SortGeom G{};
         if (resort) {
             G = sort_geom();
@@ -369,45 +413,69 @@ struct mq_engine {
         uint32_t chunk = ((uint32_t)tune.chunk + 3u) & ~3u;
         const uint32_t cap = std::max(chunk, ((uint32_t)tune.chunk_cap + 3u) & ~3u);
         const uint32_t period = std::max(4u, ((uint32_t)tune.refill_period + 3u) & ~3u);
+        const int iters = std::max(1, tune.sync_every);
         kms = 0.0f;
         int cur = 0;
-        for (;;) {
-            uint32_t* cin = cur ? c1 : c0;
-            uint32_t* cout = cur ? c0 : c1;
-            const uint32_t* lin = list[cur].as<uint32_t>();
-            if (resort) {
-                launch_sort(S, B, lin, cin, list[2].as<uint32_t>(), bin_of.as<uint32_t>(), rank_of.as<uint32_t>(),
-                            hist.as<uint32_t>(), bsum.as<uint32_t>(), G, n, stream);
-                HIPCHK(hipGetLastError());
-                lin = list[2].as<uint32_t>();
+        if (tune.graph) {
+            // capture one block of `iters` launches; replay it until the list is empty
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            int cur_cap = cur;
+            uint32_t chunk_cap = chunk;
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_block(S, B, n, G, cur_cap, chunk_cap, cap, period, grid, block, iters, false, nullptr);
+            } catch (...) {
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
             }
-            WorkerArgs a;
-            a.S = S;
-            a.jobs = jobs.as<JobP>();
-            a.B = B;
-            a.cells = cells.as<float4>();
-            a.list_in = lin;
-            a.list_out = list[cur ^ 1].as<uint32_t>();
-            a.ctr_in = cin;
-            a.ctr_out = cout;
-            a.chunk = chunk;
-            a.period = period;
-            a.nqueues = resort ? MAX_QUEUES : 1u;
-            launch_prepare(a, stream);
-            HIPCHK(hipEventRecord(ev0, stream));
-            launch_batched(a, grid, block, stream);
+            HIPCHK(hipStreamEndCapture(stream, &g));
+            HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            // the block leaves its output in list[cur_cap]; with an even number of
+            // launches the ping-pong ends where it started and the block can repeat
+            if (cur_cap != cur) {
+                (void)hipGraphExecDestroy(ge);
+                (void)hipGraphDestroy(g);
+                throw Err(-1, "graph mode needs an even sync_every");
+            }
+            uint32_t* cfinal = cur ? c0 + CTR_BLOCK : c0;
+            for (;;) {
+                HIPCHK(hipEventRecord(ev0, stream));
+                HIPCHK(hipGraphLaunch(ge, stream));
+                HIPCHK(hipEventRecord(ev1, stream));
+                stats.launches += (uint32_t)iters;
+                uint32_t alive = 0;
+                HIPCHK(hipMemcpyAsync(&alive, cfinal + CTR_COUNT, 4, hipMemcpyDeviceToHost, stream));
+                HIPCHK(hipStreamSynchronize(stream));
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+                kms += ms;
+                if (alive == 0) break;
+            }
+            (void)hipGraphExecDestroy(ge);
+            (void)hipGraphDestroy(g);
+            return;
+        }
+        if ((int)evs.size() < 2 * iters) {
+            for (auto e : evs) (void)hipEventDestroy(e);
+            evs.assign(2 * iters, nullptr);
+            for (auto& e : evs) HIPCHK(hipEventCreate(&e));
+        }
+        for (;;) {
+            enqueue_block(S, B, n, G, cur, chunk, cap, period, grid, block, iters, true, &evs);
             HIPCHK(hipGetLastError());
-            HIPCHK(hipEventRecord(ev1, stream));
-            stats.launches += 1;
+            stats.launches += (uint32_t)iters;
             uint32_t alive = 0;
-            HIPCHK(hipMemcpyAsync(&alive, cout + CTR_COUNT, 4, hipMemcpyDeviceToHost, stream));
+            uint32_t* cfinal = cur ? c0 + CTR_BLOCK : c0;
+            HIPCHK(hipMemcpyAsync(&alive, cfinal + CTR_COUNT, 4, hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
-            float ms = 0.0f;
-            HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-            kms += ms;
+            for (int it = 0; it < iters; ++it) {
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, evs[2 * it], evs[2 * it + 1]));
+                kms += ms;
+            }
             if (alive == 0) break;
-            cur ^= 1;
-            chunk = std::min(cap, chunk * 2u);
         }
     }
 };
@@ -570,7 +638,15 @@ int mq_set_tune(mq_engine* eng, const mq_tune* t)
             n.chunk = t->chunk;
         }
         if (t->chunk_cap) n.chunk_cap = t->chunk_cap;
+        if (t->resort < 0 || t->resort > 1) throw Err(-1, "resort must be 0 or 1");
         n.resort = t->resort;
+        if (t->sync_every) {
+            if (t->sync_every < 1 || t->sync_every > 1024) throw Err(-1, "sync_every must be in [1, 1024]");
+            n.sync_every = t->sync_every;
+        }
+        if (t->graph < 0 || t->graph > 1) throw Err(-1, "graph must be 0 or 1");
+        n.graph = t->graph;
+        if (n.graph && (n.sync_every % 2)) throw Err(-1, "graph mode needs an even sync_every");
         eng->tune = n;
     })
 }
