@@ -172,6 +172,18 @@ int mq_last_stats(const mq_engine* eng, mq_run_stats* out);
 int mq_set_tune(mq_engine* eng, const mq_tune* t);
 int mq_get_tune(const mq_engine* eng, mq_tune* out);
 
+/* Autotuner of the batched path (the reference tunes by environment
+ * variables, tiled.rs:24-28): times candidate tunings (re-sort interval,
+ * waves per CU, re-sort on/off) on an evenly spaced subset of the jobs of
+ * about sample_ions ions (0: 262144), sets the fastest on the engine and
+ * appends it to the TSV file cache_path (NULL: no file) keyed by grid and
+ * physics switches; a later call with the same key reads it back instead of
+ * measuring unless force is 1. *out and *ion_steps_per_s (0 when read from
+ * the file) are optional. Per-ion results do not depend on the tuning. */
+int mq_autotune(mq_engine* eng, const mq_job* jobs, size_t njobs, const double dc[4], const mq_physics* ph,
+                uint64_t sample_ions, const char* cache_path, int32_t force, mq_tune* out,
+                double* ion_steps_per_s);
+
 /* ------------------------------------------------------------ scan API */
 
 /* One point of an adaptive scan: AC frequency [kHz], transmitted and crashed

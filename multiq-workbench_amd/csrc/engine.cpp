@@ -480,6 +480,101 @@ struct mq_engine {
     }
 };
 
+// ------------------------------------------------------------------ autotuner
+
+namespace {
+
+std::string tune_key(const mq_engine& e, const mq_physics& ph)
+{
+    char b[256];
+    std::snprintf(b, sizeof b, "grid=%dx%dx%d h=%.9g legacy=%d%d%d%d%d dt=%.9g", e.nx, e.ny, e.nz, e.h, ph.legacy_force,
+                  ph.legacy_collisions, ph.legacy_fields, ph.legacy_injection, ph.legacy_criteria, ph.dt);
+    return b;
+}
+
+bool read_tune_cache(const char* path, const std::string& key, mq_tune& t)
+{
+    if (!path) return false;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return false;
+    char line[1024];
+    bool found = false;
+    while (std::fgets(line, sizeof line, f)) {
+        std::string l(line);
+        const size_t tab = l.find('\t');
+        if (tab == std::string::npos || l.substr(0, tab) != key) continue;
+        mq_tune x{};
+        if (std::sscanf(l.c_str() + tab + 1, "%d,%d,%d,%d,%d,%d,%d,%d", &x.block, &x.waves_per_cu, &x.refill_period,
+                        &x.chunk, &x.chunk_cap, &x.resort, &x.sync_every, &x.graph) == 8) {
+            t = x;  // the last entry for a key wins
+            found = true;
+        }
+    }
+    std::fclose(f);
+    return found;
+}
+
+}  // namespace
+
+// Picks the fastest launch configuration for this grid and physics by timing
+// candidate tunings on an evenly spaced subset of the jobs (about `sample_ions`
+// ions), and appends the winner to the TSV cache `cache_path` (key: grid and
+// physics switches), where later calls find it without measuring. Results per
+// ion do not depend on the tuning.
+static void autotune(mq_engine& e, const mq_job* jobs, size_t njobs, const double dc[4], const mq_physics& ph,
+                     uint64_t sample_ions, const char* cache_path, int force, mq_tune& best, double& best_rate)
+{
+    const std::string key = tune_key(e, ph);
+    best_rate = 0.0;
+    if (!force && read_tune_cache(cache_path, key, best)) {
+        e.tune = best;
+        return;
+    }
+    uint64_t total = 0;
+    for (size_t k = 0; k < njobs; ++k) total += jobs[k].n;
+    if (total == 0) throw Err(-1, "autotune needs ions");
+    const size_t take = std::max<size_t>(1, std::min<size_t>(njobs, (size_t)std::ceil((double)njobs *
+                                                                                      (double)sample_ions / (double)total)));
+    std::vector<mq_job> sub;
+    for (size_t k = 0; k < take; ++k) sub.push_back(jobs[(size_t)((double)k * (double)njobs / (double)take)]);
+    std::vector<mq_tune> cands;
+    for (int resort : {1, 0})
+        for (int chunk : {128, 256, 512})
+            for (int waves : {4, 8}) {
+                if (!resort && chunk != 256) continue;
+                mq_tune t = e.tune;
+                t.resort = resort;
+                t.chunk = resort ? chunk : 20000;
+                t.chunk_cap = resort ? chunk : 200000;
+                t.waves_per_cu = waves;
+                t.block = 256;
+                t.refill_period = 8;
+                t.sync_every = 16;
+                t.graph = 0;
+                cands.push_back(t);
+            }
+    const mq_tune saved = e.tune;
+    std::vector<unsigned long long> acc;
+    size_t n = 0;
+    best = saved;
+    for (const mq_tune& t : cands) {
+        e.tune = t;
+        e.run(sub.data(), sub.size(), dc, ph, MQ_PATH_BATCHED, acc, n);
+        const double rate = (double)e.stats.ion_steps / (e.stats.wall_ms * 1e-3);
+        if (rate > best_rate) { best_rate = rate; best = t; }
+    }
+    e.tune = best;
+    if (cache_path) {
+        FILE* f = std::fopen(cache_path, "a");
+        if (f) {
+            std::fprintf(f, "%s\t%d,%d,%d,%d,%d,%d,%d,%d\t%.6g\n", key.c_str(), best.block, best.waves_per_cu,
+                         best.refill_period, best.chunk, best.chunk_cap, best.resort, best.sync_every, best.graph,
+                         best_rate);
+            std::fclose(f);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ C ABI
 
 void mq_set_last_error(const std::string& msg) { g_err = msg; }
@@ -648,6 +743,19 @@ int mq_set_tune(mq_engine* eng, const mq_tune* t)
         n.graph = t->graph;
         if (n.graph && (n.sync_every % 2)) throw Err(-1, "graph mode needs an even sync_every");
         eng->tune = n;
+    })
+}
+
+int mq_autotune(mq_engine* eng, const mq_job* jobs, size_t njobs, const double dc[4], const mq_physics* ph,
+                uint64_t sample_ions, const char* cache_path, int32_t force, mq_tune* out, double* ion_steps_per_s)
+{
+    MQ_GUARD({
+        if (!eng || !jobs || !dc || !ph) throw Err(-1, "NULL argument");
+        mq_tune best{};
+        double rate = 0.0;
+        autotune(*eng, jobs, njobs, dc, *ph, sample_ions ? sample_ions : 262144, cache_path, force, best, rate);
+        if (out) *out = best;
+        if (ion_steps_per_s) *ion_steps_per_s = rate;
     })
 }
 

@@ -127,6 +127,8 @@ SIGNATURES = {
     "mq_last_stats": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
     "mq_set_tune": (C.c_int, [C.c_void_p, C.POINTER(Tune)]),
     "mq_get_tune": (C.c_int, [C.c_void_p, C.POINTER(Tune)]),
+    "mq_autotune": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_size_t, C.POINTER(C.c_double), C.POINTER(Physics),
+                              C.c_uint64, C.c_char_p, C.c_int32, C.POINTER(Tune), C.POINTER(C.c_double)]),
     "mq_cached_batch": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(Job), C.c_size_t, C.POINTER(C.c_double),
                                   C.POINTER(Physics), C.c_int32, C.c_char_p, C.POINTER(Outcome),
                                   C.POINTER(C.c_size_t)]),
@@ -272,6 +274,15 @@ class Engine:
         for k, v in kw.items():
             setattr(t, k, v)
         _check(load().mq_set_tune(self._h, C.byref(t)))
+
+    def autotune(self, jobs, dc, ph: Physics, sample_ions: int = 0, cache_path: str | None = None,
+                 force: bool = False) -> tuple[dict, float]:
+        """Time candidate tunings on a subset of `jobs`; keep and return the fastest."""
+        t = Tune()
+        r = C.c_double()
+        _check(load().mq_autotune(self._h, jobs_array(jobs), len(jobs), (C.c_double * 4)(*dc), C.byref(ph), sample_ions,
+                                  cache_path.encode() if cache_path else None, int(force), C.byref(t), C.byref(r)))
+        return {k: getattr(t, k) for k, _ in t._fields_}, r.value
 
     def get_tune(self) -> dict:
         t = Tune()

@@ -217,3 +217,24 @@ def test_rejects_bad_arguments(c1):
         e.set_tune(block=512)
     with pytest.raises(mqhip.MqError, match="refill"):
         e.set_tune(refill_period=2)
+
+
+def test_autotune_persists_and_keeps_results(c1, tmp_path):
+    g, e = c1
+    ph = physics("half_thermal")
+    jobs = jobs_for("half_thermal", 2000)
+    say("[gpu] autotune")
+    ref = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+    t0 = e.get_tune()
+    cache = str(tmp_path / "tune.tsv")
+    try:
+        best, rate = e.autotune(jobs, [0, 5, 0, 0], ph, sample_ions=1500, cache_path=cache)
+        assert rate > 0 and e.get_tune() == best
+        assert len(open(cache).read().splitlines()) == 1
+        again, rate2 = e.autotune(jobs, [0, 5, 0, 0], ph, cache_path=cache)
+        assert again == best and rate2 == 0.0
+        tuned = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+    finally:
+        e.set_tune(**t0)
+    for k in KEYS + ("steps", "ncoll", "status"):
+        assert np.array_equal(ref[k], tuned[k]), k
