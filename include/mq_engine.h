@@ -184,6 +184,30 @@ int mq_autotune(mq_engine* eng, const mq_job* jobs, size_t njobs, const double d
                 uint64_t sample_ions, const char* cache_path, int32_t force, mq_tune* out,
                 double* ion_steps_per_s);
 
+/* ------------------------------------------------- differentiable form */
+
+/* Objective of the differentiable form: every ion of one job is integrated
+ * for exactly nsteps collision-free FP64 steps (no termination), then
+ *   J = mean_i [ sigmoid((z_i - z_exit) / scale)
+ *               - w_leak * sigmoid((r_i - r_leak) / scale) ],
+ * r_i = distance from the injection axis. ckpt_every = 0 picks ceil(sqrt(nsteps)). */
+typedef struct {
+    uint32_t nsteps, ckpt_every;
+    double z_exit, scale, r_leak, w_leak;
+} mq_objective;
+
+/* Forward op: injects job->n ions (as mq_run_batch would) and returns the
+ * objective in *J, keeping the checkpoints on the engine for mq_backward.
+ * Parameters theta = (drive.v_rf, drive.f_rf, drive.v_ac, drive.f_ac,
+ * dc_scale). The reference has no gradients (SURVEY: Optimize is a sweep). */
+int mq_forward(mq_engine* eng, const mq_job* job, const double dc[4], const mq_physics* ph,
+               const mq_objective* obj, double* J);
+
+/* Backward op of the last mq_forward: grad[5] = dJ_up * dJ/dtheta, and, if
+ * grad_state is not NULL, dJ_up * dJ/d(initial x, y, z, vx, vy, vz) of every
+ * ion as 6 arrays of n (x for all ions, then y, ...). */
+int mq_backward(mq_engine* eng, double dJ_up, double grad[5], double* grad_state);
+
 /* ------------------------------------------------------------ scan API */
 
 /* One point of an adaptive scan: AC frequency [kHz], transmitted and crashed

@@ -134,6 +134,13 @@ struct mq_engine {
     DevBuf jobs, acc, list[3], ctr, bin_of, rank_of, hist, bsum;
     DevBuf sx, sy, sz, sphi, svx, svy, svz, sfx, sfy, sfz, ssteps, sncoll, scd, sjob, sstatus;
     mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0};
+    // differentiable form: context of the last forward
+    bool adj_valid = false;
+    uint32_t adj_n = 0;
+    AdjObj adj_obj{};
+    BatchP adj_B{};
+    DevBuf adj_mem, adj_cols, adj_part, adj_out;
+    AdjBuf adj_buf{};
     std::vector<hipEvent_t> evs;
     mq_run_stats stats{};
 
@@ -182,7 +189,7 @@ struct mq_engine {
     {
         for (DevBuf* b : {&rf, &ac, &dcb[0], &dcb[1], &dcb[2], &dcb[3], &mask, &dcc, &cells, &jobs, &acc, &list[0],
                           &list[1], &list[2], &ctr, &bin_of, &rank_of, &hist, &bsum, &sx, &sy, &sz, &sphi, &svx, &svy, &svz, &sfx, &sfy, &sfz, &ssteps,
-                          &sncoll, &scd, &sjob, &sstatus})
+                          &sncoll, &scd, &sjob, &sstatus, &adj_mem, &adj_cols, &adj_part, &adj_out})
             b->release();
         for (auto e : evs) (void)hipEventDestroy(e);
         evs.clear();
@@ -263,6 +270,7 @@ struct mq_engine {
                  std::vector<unsigned long long>& acc_h, size_t& n_total)
     {
         auto t_wall = std::chrono::steady_clock::now();
+        adj_valid = false;
         if (!jobs_h || njobs == 0) throw Err(-1, "batch has no jobs");
         if (!(ph.dt > 0.0)) throw Err(-1, "dt must be positive");
         if (path != MQ_PATH_BATCHED && path != MQ_PATH_PLAIN) throw Err(-1, "unknown path");
@@ -332,6 +340,109 @@ struct mq_engine {
             kms += ms;
             if (alive == 0) break;
         }
+    }
+
+    // ---- differentiable form
+    double colsum(double* const* cols, uint32_t ncols, uint32_t n, double* host)
+    {
+        adj_cols.ensure(16 * sizeof(double*));
+        adj_part.ensure((size_t)ncols * std::max<uint32_t>(1, (n + 1023) / 1024) * sizeof(double));
+        adj_out.ensure(16 * sizeof(double));
+        HIPCHK(hipMemcpyAsync(adj_cols.p, cols, ncols * sizeof(double*), hipMemcpyHostToDevice, stream));
+        launch_colsum(adj_cols.as<const double*>(), ncols, n, adj_part.as<double>(), adj_out.as<double>(), stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(host, adj_out.p, ncols * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return host[0];
+    }
+
+    double forward(const mq_job& job, const double dc[4], const mq_physics& ph, const mq_objective& o)
+    {
+        auto t_wall = std::chrono::steady_clock::now();
+        adj_valid = false;
+        if (!(ph.dt > 0.0)) throw Err(-1, "dt must be positive");
+        if (job.n == 0 || job.n >= (1ull << 31)) throw Err(-1, "the job needs between 1 and 2^31 ions");
+        if (o.nsteps == 0) throw Err(-1, "objective needs nsteps > 0");
+        if (!(o.scale > 0.0)) throw Err(-1, "objective scale must be positive");
+        if (ph.pressure_pa > 0.0 || job.pressure_pa > 0.0)
+            throw Err(-1, "the differentiable form is collision-free: set pressure_pa to 0");
+        const uint32_t n = (uint32_t)job.n;
+        const uint32_t K = o.ckpt_every ? o.ckpt_every : (uint32_t)std::ceil(std::sqrt((double)o.nsteps));
+        const uint32_t slots = (o.nsteps + K - 1) / K;
+        JobP jp = job_params(job, ph, 0);
+        jobs.ensure(sizeof(JobP));
+        HIPCHK(hipMemcpyAsync(jobs.p, &jp, sizeof(JobP), hipMemcpyHostToDevice, stream));
+        prepare_dc(dc, false);
+        IonState S = state_buffers(n);
+        InjP ip{};
+        ip.axis_x = meta.axis_x; ip.axis_y = meta.axis_y; ip.z_inject = meta.z_inject;
+        ip.temp_k = ph.temp_k; ip.r_inject_mm = ph.r_inject_mm; ip.inj_sigma_mm = ph.inj_sigma_mm; ip.dt = ph.dt;
+        ip.model = ph.injection == MQ_INJ_GAUSSIAN ? 2 : (ph.legacy_injection ? 1 : 0);
+        launch_inject(S, jobs.as<JobP>(), 1, ip, n, stream);
+        HIPCHK(hipGetLastError());
+        // buffers: 6 slots of checkpoints, 3 x K segment, 3 final, J, 5 + 6 gradients
+        const size_t per_ion = (size_t)6 * slots + 3 * K + 3 + 1 + 5 + 6;
+        adj_mem.ensure(per_ion * n * sizeof(double));
+        double* p = adj_mem.as<double>();
+        AdjBuf A{};
+        for (int k = 0; k < 6; ++k) { A.ck[k] = p; p += (size_t)slots * n; }
+        for (int k = 0; k < 3; ++k) { A.seg[k] = p; p += (size_t)K * n; }
+        for (int k = 0; k < 3; ++k) { A.fin[k] = p; p += n; }
+        A.Ji = p; p += n;
+        for (int k = 0; k < 5; ++k) { A.gp[k] = p; p += n; }
+        for (int k = 0; k < 6; ++k) { A.gs[k] = p; p += n; }
+        AdjObj O{};
+        O.z_exit = o.z_exit; O.scale = o.scale; O.r_leak = o.r_leak; O.w_leak = o.w_leak;
+        O.axis_x = meta.axis_x; O.axis_y = meta.axis_y; O.nsteps = o.nsteps; O.ckpt = K;
+        BatchP B = batch_params(ph, 1);
+        AdjGrids G{rf.as<double>(), dcc.as<double>(), ac.as<double>()};
+        stats = mq_run_stats{};
+        stats.n_ions = n;
+        HIPCHK(hipEventRecord(ev0, stream));
+        launch_adj_forward(S, jobs.as<JobP>(), B, G, O, A, n, stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(ev1, stream));
+        double sum[1];
+        double* cols[1] = {A.Ji};
+        colsum(cols, 1, n, sum);
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        stats.kernel_ms = ms;
+        stats.launches = 1;
+        stats.ion_steps = (uint64_t)n * o.nsteps;
+        stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
+        adj_buf = A; adj_obj = O; adj_B = B; adj_n = n; adj_valid = true;
+        return sum[0] / (double)n;
+    }
+
+    void backward(double dJ, double grad[5], double* grad_state)
+    {
+        auto t_wall = std::chrono::steady_clock::now();
+        if (!adj_valid) throw Err(-1, "mq_backward needs a preceding mq_forward on this engine");
+        const uint32_t n = adj_n;
+        IonState S = state_buffers(n);
+        AdjGrids G{rf.as<double>(), dcc.as<double>(), ac.as<double>()};
+        HIPCHK(hipEventRecord(ev0, stream));
+        launch_adj_backward(S, jobs.as<JobP>(), adj_B, G, adj_obj, adj_buf, n, dJ / (double)n, stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(ev1, stream));
+        double g[5];
+        double* cols[5] = {adj_buf.gp[0], adj_buf.gp[1], adj_buf.gp[2], adj_buf.gp[3], adj_buf.gp[4]};
+        colsum(cols, 5, n, g);
+        for (int k = 0; k < 5; ++k) grad[k] = g[k];
+        if (grad_state)
+            for (int k = 0; k < 6; ++k)
+                HIPCHK(hipMemcpyAsync(grad_state + (size_t)k * n, adj_buf.gs[k], (size_t)n * sizeof(double),
+                                      hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        stats = mq_run_stats{};
+        stats.n_ions = n;
+        stats.kernel_ms = ms;
+        stats.launches = 1;
+        stats.ion_steps = 2ull * n * adj_obj.nsteps;  // recompute + backward
+        stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
     }
 
     SortGeom sort_geom() const
@@ -756,6 +867,23 @@ int mq_autotune(mq_engine* eng, const mq_job* jobs, size_t njobs, const double d
         autotune(*eng, jobs, njobs, dc, *ph, sample_ions ? sample_ions : 262144, cache_path, force, best, rate);
         if (out) *out = best;
         if (ion_steps_per_s) *ion_steps_per_s = rate;
+    })
+}
+
+int mq_forward(mq_engine* eng, const mq_job* job, const double dc[4], const mq_physics* ph, const mq_objective* obj,
+               double* J)
+{
+    MQ_GUARD({
+        if (!eng || !job || !dc || !ph || !obj || !J) throw Err(-1, "NULL argument");
+        *J = eng->forward(*job, dc, *ph, *obj);
+    })
+}
+
+int mq_backward(mq_engine* eng, double dJ_up, double grad[5], double* grad_state)
+{
+    MQ_GUARD({
+        if (!eng || !grad) throw Err(-1, "NULL argument");
+        eng->backward(dJ_up, grad, grad_state);
     })
 }
 

@@ -50,4 +50,26 @@ void launch_sort(const IonState& S, const BatchP& B, const uint32_t* list, const
                  uint32_t nmax, hipStream_t st);
 void launch_prepare(const WorkerArgs& a, hipStream_t st);
 void launch_batched(const WorkerArgs& a, uint32_t grid, uint32_t block, hipStream_t st);
+// Differentiable form (kernels_adjoint.hip).
+struct AdjGrids {
+    const double *rf, *dc, *ac;
+};
+struct AdjObj {
+    double z_exit, scale, r_leak, w_leak, axis_x, axis_y;
+    uint32_t nsteps, ckpt;
+};
+struct AdjBuf {
+    double* ck[6];   // checkpoints (x, y, z, vx, vy, vz) [slot * n + ion]
+    double* seg[3];  // recomputed positions of one segment [k * n + ion]
+    double* fin[3];  // final position
+    double* Ji;      // objective per ion
+    double* gp[5];   // per-ion dJ/d(V_rf, f_rf, V_ac, f_ac, s_dc)
+    double* gs[6];   // per-ion dJ/d(initial x, y, z, vx, vy, vz)
+};
+void launch_adj_forward(const IonState& S, const JobP* jobs, const BatchP& B, const AdjGrids& G, const AdjObj& O,
+                        const AdjBuf& A, uint32_t n, hipStream_t st);
+void launch_adj_backward(const IonState& S, const JobP* jobs, const BatchP& B, const AdjGrids& G, const AdjObj& O,
+                         const AdjBuf& A, uint32_t n, double scale, hipStream_t st);
+void launch_colsum(const double* const* cols_dev, uint32_t ncols, uint32_t n, double* part, double* out, hipStream_t st);
+
 }  // namespace mq

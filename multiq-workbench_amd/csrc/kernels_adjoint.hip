@@ -1,0 +1,274 @@
+// kernels_adjoint.hip -- the differentiable form: gradients of a smoothed
+// transmission / leakage objective through the FP64 collision-free step.
+//
+// The reference has no gradients (its Optimize search is a grid of sweeps with
+// Lorentzian fits, src-tauri/src/research.rs:133-163); this is the hand-written
+// discrete adjoint of the step the plain kernel integrates (channel.metal:82-92
+// without collisions or termination):
+//     v' = v + kq E(x, n; theta),   x' = x + v' dt,
+//     E  = a_rf(n) E_rf(x) + s_dc E_dc(x) + a_ac(n) E_ac(x),
+//     a_rf = V_rf sin(2 pi frac(f_rf dt n)),  a_ac = V_ac sin(2 pi frac(f_ac dt n + phi)).
+// Backward step (lambda' = dJ/d(x', v')):
+//     mu = lambda_v' + dt lambda_x',  G = kq mu,
+//     lambda_x = lambda_x' + (dE/dx)^T G,  lambda_v = mu,  dJ/dtheta += (dE/dtheta)^T G.
+// sqrt(N) checkpointing: the forward stores (x, v) every K steps; the backward
+// recomputes each K-step segment from its checkpoint into a per-ion buffer and
+// walks it backwards, so memory is O((N/K + K) ions) and work is 2 forwards
+// plus one backward.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.h"
+#include "mq_math.h"
+#include "mq_types.h"
+
+namespace mq {
+
+struct Cell3 {
+    uint32_t b;
+    double tx, ty, tz;
+    double dx, dy, dz;  // d(t)/d(x): inv_h inside the grid, 0 where the coordinate is clamped
+};
+
+__device__ __forceinline__ void locate_d(double x, int n, double inv_h, int nearest, int& i, double& t, double& dtdx)
+{
+    const double g = x * inv_h;
+    double gc = g;
+    bool clamped = false;
+    if (!(gc >= 0.0)) { gc = 0.0; clamped = true; }
+    if (gc > (double)(n - 1)) { gc = (double)(n - 1); clamped = true; }
+    int ii = (int)gc;
+    if (ii > n - 2) ii = n - 2;
+    i = ii;
+    t = nearest ? 0.0 : gc - (double)ii;
+    dtdx = (nearest || clamped) ? 0.0 : inv_h;
+}
+
+__device__ __forceinline__ Cell3 cell_of(const BatchP& B, double x, double y, double z)
+{
+    Cell3 c;
+    int i, j, k;
+    locate_d(x, B.nx, B.inv_h, B.nearest, i, c.tx, c.dx);
+    locate_d(y, B.ny, B.inv_h, B.nearest, j, c.ty, c.dy);
+    locate_d(z, B.nz, B.inv_h, B.nearest, k, c.tz, c.dz);
+    c.b = (uint32_t)i + (uint32_t)B.nx * ((uint32_t)j + (uint32_t)B.ny * (uint32_t)k);
+    return c;
+}
+
+// Value (same lerp chain as the forward step) and the Jacobian d out[d] / d x_e.
+__device__ __forceinline__ void sample_jac(const double* __restrict__ F, const Cell3& c, uint32_t sy, uint32_t sz,
+                                           double out[3], double jac[3][3])
+{
+    const uint32_t b = c.b;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double c000 = F[3 * b + d], c100 = F[3 * (b + 1) + d];
+        const double c010 = F[3 * (b + sy) + d], c110 = F[3 * (b + 1 + sy) + d];
+        const double c001 = F[3 * (b + sz) + d], c101 = F[3 * (b + 1 + sz) + d];
+        const double c011 = F[3 * (b + sy + sz) + d], c111 = F[3 * (b + 1 + sy + sz) + d];
+        const double c00 = lerp_d(c000, c100, c.tx), c10 = lerp_d(c010, c110, c.tx);
+        const double c01 = lerp_d(c001, c101, c.tx), c11 = lerp_d(c011, c111, c.tx);
+        const double lo = lerp_d(c00, c10, c.ty), hi = lerp_d(c01, c11, c.ty);
+        out[d] = lerp_d(lo, hi, c.tz);
+        const double dtx = lerp_d(lerp_d(c100 - c000, c110 - c010, c.ty), lerp_d(c101 - c001, c111 - c011, c.ty), c.tz);
+        const double dty = lerp_d(c10 - c00, c11 - c01, c.tz);
+        const double dtz = hi - lo;
+        jac[d][0] = dtx * c.dx;
+        jac[d][1] = dty * c.dy;
+        jac[d][2] = dtz * c.dz;
+    }
+}
+
+__device__ __forceinline__ void sample_val(const double* __restrict__ F, const Cell3& c, uint32_t sy, uint32_t sz,
+                                           double out[3])
+{
+    const uint32_t b = c.b;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double c00 = lerp_d(F[3 * b + d], F[3 * (b + 1) + d], c.tx);
+        const double c10 = lerp_d(F[3 * (b + sy) + d], F[3 * (b + 1 + sy) + d], c.tx);
+        const double c01 = lerp_d(F[3 * (b + sz) + d], F[3 * (b + 1 + sz) + d], c.tx);
+        const double c11 = lerp_d(F[3 * (b + sy + sz) + d], F[3 * (b + 1 + sy + sz) + d], c.tx);
+        out[d] = lerp_d(lerp_d(c00, c10, c.ty), lerp_d(c01, c11, c.ty), c.tz);
+    }
+}
+
+// One forward step (identical arithmetic to k_plain without collisions).
+__device__ __forceinline__ void step_fwd(const BatchP& B, const JobP& J, const AdjGrids& G, double phi, uint32_t n,
+                                         double& x, double& y, double& z, double& vx, double& vy, double& vz)
+{
+    const uint32_t sy = (uint32_t)B.nx, sz = (uint32_t)B.nx * (uint32_t)B.ny;
+    const Cell3 c = cell_of(B, x, y, z);
+    double er[3], ed[3], ea[3];
+    sample_val(G.rf, c, sy, sz, er);
+    sample_val(G.dc, c, sy, sz, ed);
+    sample_val(G.ac, c, sy, sz, ea);
+    const double a_rf = J.v_rf * sin2pi(frac_d(J.w_rf * (double)n));
+    const double a_ac = J.v_ac * sin2pi(frac_d(fma(J.w_ac, (double)n, phi)));
+    double E;
+    E = (a_rf * er[0] + J.s_dc * ed[0]) + a_ac * ea[0]; vx = fma(J.kq, E, vx); x = fma(vx, B.dt, x);
+    E = (a_rf * er[1] + J.s_dc * ed[1]) + a_ac * ea[1]; vy = fma(J.kq, E, vy); y = fma(vy, B.dt, y);
+    E = (a_rf * er[2] + J.s_dc * ed[2]) + a_ac * ea[2]; vz = fma(J.kq, E, vz); z = fma(vz, B.dt, z);
+}
+
+__device__ __forceinline__ double sigm(double u) { return 1.0 / (1.0 + exp(-u)); }
+
+// Objective of one ion at its final state and dJ_i/d(x, y, z).
+__device__ __forceinline__ double objective(const AdjObj& O, double x, double y, double z, double g[3])
+{
+    const double u = (z - O.z_exit) / O.scale;
+    const double st = sigm(u);
+    double Ji = st;
+    g[0] = 0.0; g[1] = 0.0;
+    g[2] = st * (1.0 - st) / O.scale;
+    if (O.w_leak != 0.0) {
+        const double rx = x - O.axis_x, ry = y - O.axis_y;
+        const double r = sqrt(rx * rx + ry * ry);
+        const double ur = (r - O.r_leak) / O.scale;
+        const double sl = sigm(ur);
+        Ji -= O.w_leak * sl;
+        if (r > 0.0) {
+            const double k = -O.w_leak * sl * (1.0 - sl) / O.scale / r;
+            g[0] = k * rx;
+            g[1] = k * ry;
+        }
+    }
+    return Ji;
+}
+
+// Forward: N steps per ion, (x, v) stored every K steps (slot s holds step s*K).
+__global__ __launch_bounds__(256) void k_adj_forward(IonState S, const JobP* __restrict__ jobs, BatchP B, AdjGrids G,
+                                                    AdjObj O, AdjBuf A, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const JobP J = jobs[S.job[i]];
+    double x = S.x[i], y = S.y[i], z = S.z[i], vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
+    const double phi = S.phi[i];
+    for (uint32_t s = 0; s < O.nsteps; ++s) {
+        if (s % O.ckpt == 0) {
+            const size_t o = (size_t)(s / O.ckpt) * n + i;
+            A.ck[0][o] = x; A.ck[1][o] = y; A.ck[2][o] = z; A.ck[3][o] = vx; A.ck[4][o] = vy; A.ck[5][o] = vz;
+        }
+        step_fwd(B, J, G, phi, s, x, y, z, vx, vy, vz);
+    }
+    double g[3];
+    A.Ji[i] = objective(O, x, y, z, g);
+    A.fin[0][i] = x; A.fin[1][i] = y; A.fin[2][i] = z;
+}
+
+// Backward: per ion, segments last to first; per-ion partial gradients and the
+// gradient with respect to the initial state.
+__global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __restrict__ jobs, BatchP B, AdjGrids G,
+                                                     AdjObj O, AdjBuf A, uint32_t n, double scale)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const JobP J = jobs[S.job[i]];
+    const double phi = S.phi[i];
+    const uint32_t sy = (uint32_t)B.nx, sz = (uint32_t)B.nx * (uint32_t)B.ny;
+    double g[3];
+    (void)objective(O, A.fin[0][i], A.fin[1][i], A.fin[2][i], g);
+    double lx = scale * g[0], ly = scale * g[1], lz = scale * g[2], lvx = 0.0, lvy = 0.0, lvz = 0.0;
+    double gvrf = 0.0, gfrf = 0.0, gvac = 0.0, gfac = 0.0, gdc = 0.0;
+    const double two_pi_dt = 6.283185307179586 * B.dt;
+    const uint32_t nseg = (O.nsteps + O.ckpt - 1) / O.ckpt;
+    for (int seg = (int)nseg - 1; seg >= 0; --seg) {
+        const uint32_t s0 = (uint32_t)seg * O.ckpt;
+        const uint32_t s1 = min(O.nsteps, s0 + O.ckpt);
+        const size_t oc = (size_t)seg * n + i;
+        double x = A.ck[0][oc], y = A.ck[1][oc], z = A.ck[2][oc], vx = A.ck[3][oc], vy = A.ck[4][oc], vz = A.ck[5][oc];
+        // recompute the segment's positions x_s, s0 <= s < s1
+        for (uint32_t s = s0; s < s1; ++s) {
+            const size_t o = (size_t)(s - s0) * n + i;
+            A.seg[0][o] = x; A.seg[1][o] = y; A.seg[2][o] = z;
+            step_fwd(B, J, G, phi, s, x, y, z, vx, vy, vz);
+        }
+        for (int s = (int)s1 - 1; s >= (int)s0; --s) {
+            const size_t o = (size_t)((uint32_t)s - s0) * n + i;
+            const Cell3 c = cell_of(B, A.seg[0][o], A.seg[1][o], A.seg[2][o]);
+            double er[3], ed[3], ea[3], jr[3][3], jd[3][3], ja[3][3];
+            sample_jac(G.rf, c, sy, sz, er, jr);
+            sample_jac(G.dc, c, sy, sz, ed, jd);
+            sample_jac(G.ac, c, sy, sz, ea, ja);
+            double srf, crf, sac, cac;
+            sincos2pi(frac_d(J.w_rf * (double)s), srf, crf);
+            sincos2pi(frac_d(fma(J.w_ac, (double)s, phi)), sac, cac);
+            const double a_rf = J.v_rf * srf, a_ac = J.v_ac * sac;
+            const double mx = lvx + B.dt * lx, my = lvy + B.dt * ly, mz = lvz + B.dt * lz;
+            const double Gx = J.kq * mx, Gy = J.kq * my, Gz = J.kq * mz;
+            // lambda_x += (dE/dx)^T G
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const double t = Gx * ((a_rf * jr[0][e] + J.s_dc * jd[0][e]) + a_ac * ja[0][e]) +
+                                 Gy * ((a_rf * jr[1][e] + J.s_dc * jd[1][e]) + a_ac * ja[1][e]) +
+                                 Gz * ((a_rf * jr[2][e] + J.s_dc * jd[2][e]) + a_ac * ja[2][e]);
+                if (e == 0) lx += t; else if (e == 1) ly += t; else lz += t;
+            }
+            lvx = mx; lvy = my; lvz = mz;
+            const double gr = Gx * er[0] + Gy * er[1] + Gz * er[2];
+            const double ga = Gx * ea[0] + Gy * ea[1] + Gz * ea[2];
+            gvrf += srf * gr;
+            gfrf += J.v_rf * crf * two_pi_dt * (double)s * gr;
+            gvac += sac * ga;
+            gfac += J.v_ac * cac * two_pi_dt * (double)s * ga;
+            gdc += Gx * ed[0] + Gy * ed[1] + Gz * ed[2];
+        }
+    }
+    A.gp[0][i] = gvrf; A.gp[1][i] = gfrf; A.gp[2][i] = gvac; A.gp[3][i] = gfac; A.gp[4][i] = gdc;
+    A.gs[0][i] = lx; A.gs[1][i] = ly; A.gs[2][i] = lz; A.gs[3][i] = lvx; A.gs[4][i] = lvy; A.gs[5][i] = lvz;
+}
+
+// Deterministic sum of `cols` columns of length n (fixed order: 1024-element
+// blocks summed by a fixed tree, then the block sums in order on one thread).
+__global__ __launch_bounds__(256) void k_colsum_blocks(const double* const* __restrict__ cols, uint32_t n,
+                                                      double* __restrict__ part)
+{
+    __shared__ double sh[256];
+    const uint32_t c = blockIdx.y, t = threadIdx.x, base = blockIdx.x * 1024;
+    double s = 0.0;
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t i = base + t * 4 + k;
+        if (i < n) s += cols[c][i];
+    }
+    sh[t] = s;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    if (t == 0) part[(size_t)c * gridDim.x + blockIdx.x] = sh[0];
+}
+
+__global__ void k_colsum_final(const double* __restrict__ part, uint32_t nblocks, uint32_t ncols, double* __restrict__ out)
+{
+    const uint32_t c = threadIdx.x;
+    if (c >= ncols) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < nblocks; ++b) s += part[(size_t)c * nblocks + b];
+    out[c] = s;
+}
+
+static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+void launch_adj_forward(const IonState& S, const JobP* jobs, const BatchP& B, const AdjGrids& G, const AdjObj& O,
+                        const AdjBuf& A, uint32_t n, hipStream_t st)
+{
+    if (n) k_adj_forward<<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, G, O, A, n);
+}
+
+void launch_adj_backward(const IonState& S, const JobP* jobs, const BatchP& B, const AdjGrids& G, const AdjObj& O,
+                         const AdjBuf& A, uint32_t n, double scale, hipStream_t st)
+{
+    if (n) k_adj_backward<<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, G, O, A, n, scale);
+}
+
+void launch_colsum(const double* const* cols_dev, uint32_t ncols, uint32_t n, double* part, double* out, hipStream_t st)
+{
+    const uint32_t nb = std::max(1u, nblk(n, 1024));
+    k_colsum_blocks<<<dim3(nb, ncols), 256, 0, st>>>(cols_dev, n, part);
+    k_colsum_final<<<1, 64, 0, st>>>(part, nb, ncols, out);
+}
+
+}  // namespace mq

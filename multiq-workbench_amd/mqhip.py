@@ -109,6 +109,11 @@ class Tune(C.Structure):
                 ("graph", C.c_int32)]
 
 
+class Objective(C.Structure):
+    _fields_ = [("nsteps", C.c_uint32), ("ckpt_every", C.c_uint32), ("z_exit", C.c_double), ("scale", C.c_double),
+                ("r_leak", C.c_double), ("w_leak", C.c_double)]
+
+
 class ScanPoint(C.Structure):
     _fields_ = [("f_khz", C.c_double), ("transmitted", C.c_double), ("crashed", C.c_double), ("n", C.c_uint64)]
 
@@ -129,6 +134,9 @@ SIGNATURES = {
     "mq_get_tune": (C.c_int, [C.c_void_p, C.POINTER(Tune)]),
     "mq_autotune": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_size_t, C.POINTER(C.c_double), C.POINTER(Physics),
                               C.c_uint64, C.c_char_p, C.c_int32, C.POINTER(Tune), C.POINTER(C.c_double)]),
+    "mq_forward": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_double), C.POINTER(Physics),
+                             C.POINTER(Objective), C.POINTER(C.c_double)]),
+    "mq_backward": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mq_cached_batch": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(Job), C.c_size_t, C.POINTER(C.c_double),
                                   C.POINTER(Physics), C.c_int32, C.c_char_p, C.POINTER(Outcome),
                                   C.POINTER(C.c_size_t)]),
@@ -274,6 +282,20 @@ class Engine:
         for k, v in kw.items():
             setattr(t, k, v)
         _check(load().mq_set_tune(self._h, C.byref(t)))
+
+    def forward(self, j: Job, dc, ph: Physics, obj: Objective) -> float:
+        """Forward op of the differentiable form: mean smoothed objective J."""
+        J = C.c_double()
+        _check(load().mq_forward(self._h, C.byref(j), (C.c_double * 4)(*dc), C.byref(ph), C.byref(obj), C.byref(J)))
+        return J.value
+
+    def backward(self, dJ: float = 1.0, state: bool = False):
+        """Backward op: dJ * dJ/d(V_rf, f_rf, V_ac, f_ac, dc_scale) [and dJ/d(initial state), shape (6, n)]."""
+        g = (C.c_double * 5)()
+        n = self.stats()["n_ions"]
+        gs = np.zeros(6 * n) if state else None
+        _check(load().mq_backward(self._h, dJ, g, _dptr(gs) if state else None))
+        return (np.array(list(g)), gs.reshape(6, n)) if state else np.array(list(g))
 
     def autotune(self, jobs, dc, ph: Physics, sample_ions: int = 0, cache_path: str | None = None,
                  force: bool = False) -> tuple[dict, float]:

@@ -508,3 +508,181 @@ void or_summarize(const or_ion* ions, const uint32_t* job, size_t n, size_t njob
             out[j].mean_time /= (double)out[j].n;
         }
 }
+
+/* ------------------------------------------------ differentiable form (dual numbers) */
+
+/* Value and Jacobian d out / d x of the trilinear sample at position x (clamped
+ * coordinates have zero derivative). */
+static void sample_jac_x(const double* F, const or_ctx* C, const double x[3], double out[3], double jac[3][3])
+{
+    const mq_fields* f = C->f;
+    const int n[3] = {f->nx, f->ny, f->nz};
+    int ijk[3];
+    double t[3], dt[3];
+    for (int a = 0; a < 3; ++a) {
+        double g = x[a] * C->inv_h, gc = g;
+        int cl = 0;
+        if (!(gc >= 0.0)) { gc = 0.0; cl = 1; }
+        if (gc > (double)(n[a] - 1)) { gc = (double)(n[a] - 1); cl = 1; }
+        int ii = (int)gc;
+        if (ii > n[a] - 2) ii = n[a] - 2;
+        ijk[a] = ii;
+        t[a] = C->nearest ? 0.0 : gc - (double)ii;
+        dt[a] = (C->nearest || cl) ? 0.0 : C->inv_h;
+    }
+    size_t sy = (size_t)f->nx, sz = (size_t)f->nx * (size_t)f->ny;
+    size_t b = (size_t)ijk[0] + sy * ((size_t)ijk[1] + (size_t)f->ny * (size_t)ijk[2]);
+    for (int d = 0; d < 3; ++d) {
+        double c000 = F[3 * b + d], c100 = F[3 * (b + 1) + d], c010 = F[3 * (b + sy) + d], c110 = F[3 * (b + 1 + sy) + d];
+        double c001 = F[3 * (b + sz) + d], c101 = F[3 * (b + 1 + sz) + d];
+        double c011 = F[3 * (b + sy + sz) + d], c111 = F[3 * (b + 1 + sy + sz) + d];
+        double c00 = lerp(c000, c100, t[0]), c10 = lerp(c010, c110, t[0]);
+        double c01 = lerp(c001, c101, t[0]), c11 = lerp(c011, c111, t[0]);
+        double lo = lerp(c00, c10, t[1]), hi = lerp(c01, c11, t[1]);
+        out[d] = lerp(lo, hi, t[2]);
+        /* partial derivatives of the trilinear form in the cell coordinates */
+        double w00 = (1.0 - t[1]) * (1.0 - t[2]), w10 = t[1] * (1.0 - t[2]), w01 = (1.0 - t[1]) * t[2], w11 = t[1] * t[2];
+        double dtx = w00 * (c100 - c000) + w10 * (c110 - c010) + w01 * (c101 - c001) + w11 * (c111 - c011);
+        double dty = (1.0 - t[2]) * (c10 - c00) + t[2] * (c11 - c01);
+        double dtz = hi - lo;
+        jac[d][0] = dtx * dt[0];
+        jac[d][1] = dty * dt[1];
+        jac[d][2] = dtz * dt[2];
+    }
+}
+
+typedef struct {
+    const or_ctx* C;
+    const or_jp* J;
+    const mq_job* job;
+    const mq_physics* ph;
+    const mq_objective* obj;
+    uint32_t n;
+    double* Ji;      /* n */
+    double* gi;      /* 5 * n */
+    double* gsi;     /* 6 * n */
+    atomic_size_t next;
+} or_gwork;
+
+static double sigm(double u) { return 1.0 / (1.0 + exp(-u)); }
+
+static void grad_one(const or_gwork* W, uint32_t ion)
+{
+    const or_ctx* C = W->C;
+    const or_jp* J = W->J;
+    const mq_fields* f = C->f;
+    enum { NT = 11 };
+    or_ion s0;
+    or_inject(f, W->job, W->ph, ion, &s0);
+    double x[3] = {s0.x[0], s0.x[1], s0.x[2]}, v[3] = {s0.v[0], s0.v[1], s0.v[2]};
+    double dx[NT][3], dv[NT][3];
+    memset(dx, 0, sizeof dx);
+    memset(dv, 0, sizeof dv);
+    for (int a = 0; a < 3; ++a) { dx[5 + a][a] = 1.0; dv[8 + a][a] = 1.0; }
+    const double dt = C->dt, two_pi = 6.283185307179586;
+    for (uint32_t n = 0; n < W->obj->nsteps; ++n) {
+        double er[3], ed[3], ea[3], jr[3][3], jd[3][3], ja[3][3];
+        sample_jac_x(f->rf, C, x, er, jr);
+        sample_jac_x(C->dcc, C, x, ed, jd);
+        sample_jac_x(f->ac, C, x, ea, ja);
+        double srf, crf, sac, cac;
+        or_sincos2pi(frac(J->w_rf * (double)n), &srf, &crf);
+        or_sincos2pi(frac(fma(J->w_ac, (double)n, s0.phi_ac)), &sac, &cac);
+        double a_rf = J->v_rf * srf, a_ac = J->v_ac * sac;
+        /* d a_rf / d f_rf and d a_ac / d f_ac */
+        double da_rf = J->v_rf * crf * two_pi * dt * (double)n, da_ac = J->v_ac * cac * two_pi * dt * (double)n;
+        double E[3], JE[3][3];
+        for (int d = 0; d < 3; ++d) {
+            E[d] = (a_rf * er[d] + J->s_dc * ed[d]) + a_ac * ea[d];
+            for (int e = 0; e < 3; ++e) JE[d][e] = (a_rf * jr[d][e] + J->s_dc * jd[d][e]) + a_ac * ja[d][e];
+        }
+        for (int k = 0; k < NT; ++k) {
+            double dE[3];
+            for (int d = 0; d < 3; ++d) {   /* all components from the tangent before this step */
+                dE[d] = JE[d][0] * dx[k][0] + JE[d][1] * dx[k][1] + JE[d][2] * dx[k][2];
+                if (k == 0) dE[d] += srf * er[d];
+                else if (k == 1) dE[d] += da_rf * er[d];
+                else if (k == 2) dE[d] += sac * ea[d];
+                else if (k == 3) dE[d] += da_ac * ea[d];
+                else if (k == 4) dE[d] += ed[d];
+            }
+            for (int d = 0; d < 3; ++d) {
+                dv[k][d] += J->kq * dE[d];
+                dx[k][d] += dt * dv[k][d];
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            v[d] = fma(J->kq, E[d], v[d]);
+            x[d] = fma(v[d], dt, x[d]);
+        }
+    }
+    /* objective and its gradient at the final position */
+    const mq_objective* O = W->obj;
+    double u = (x[2] - O->z_exit) / O->scale, st = sigm(u);
+    double Ji = st, gJ[3] = {0.0, 0.0, st * (1.0 - st) / O->scale};
+    if (O->w_leak != 0.0) {
+        double rx = x[0] - f->axis_x, ry = x[1] - f->axis_y, r = sqrt(rx * rx + ry * ry);
+        double sl = sigm((r - O->r_leak) / O->scale);
+        Ji -= O->w_leak * sl;
+        if (r > 0.0) {
+            double k = -O->w_leak * sl * (1.0 - sl) / O->scale / r;
+            gJ[0] = k * rx;
+            gJ[1] = k * ry;
+        }
+    }
+    W->Ji[ion] = Ji;
+    for (int k = 0; k < NT; ++k) {
+        double g = gJ[0] * dx[k][0] + gJ[1] * dx[k][1] + gJ[2] * dx[k][2];
+        if (k < 5) W->gi[(size_t)k * W->n + ion] = g;
+        else W->gsi[(size_t)(k - 5) * W->n + ion] = g;
+    }
+}
+
+static void* grad_worker(void* arg)
+{
+    or_gwork* W = (or_gwork*)arg;
+    for (;;) {
+        size_t a = atomic_fetch_add(&W->next, 1);
+        if (a >= W->n) break;
+        grad_one(W, (uint32_t)a);
+    }
+    return NULL;
+}
+
+int or_grad_reference(const mq_fields* f, const mq_job* job, const double dc[4], const mq_physics* ph,
+                      const mq_objective* obj, int nthreads, double* Jout, double grad[5], double* grad_state)
+{
+    double* dcc = combine_dc(f, dc);
+    if (!dcc) return -1;
+    or_ctx C;
+    ctx_init(&C, f, dcc, ph);
+    or_jp J;
+    job_params(job, ph, &J);
+    uint32_t n = (uint32_t)job->n;
+    or_gwork W;
+    W.C = &C; W.J = &J; W.job = job; W.ph = ph; W.obj = obj; W.n = n;
+    W.Ji = (double*)calloc(n, sizeof(double));
+    W.gi = (double*)calloc((size_t)5 * n, sizeof(double));
+    W.gsi = (double*)calloc((size_t)6 * n, sizeof(double));
+    atomic_init(&W.next, 0);
+    if (nthreads < 1) nthreads = 1;
+    pthread_t* th = (pthread_t*)malloc((size_t)nthreads * sizeof(pthread_t));
+    for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, grad_worker, &W);
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th);
+    double sJ = 0.0;
+    for (uint32_t i = 0; i < n; ++i) sJ += W.Ji[i];
+    *Jout = sJ / (double)n;
+    for (int k = 0; k < 5; ++k) {
+        double s = 0.0;
+        for (uint32_t i = 0; i < n; ++i) s += W.gi[(size_t)k * n + i];
+        grad[k] = s / (double)n;
+    }
+    if (grad_state)
+        for (size_t i = 0; i < (size_t)6 * n; ++i) grad_state[i] = W.gsi[i] / (double)n;
+    free(W.Ji);
+    free(W.gi);
+    free(W.gsi);
+    free(dcc);
+    return 0;
+}

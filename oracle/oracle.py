@@ -51,6 +51,9 @@ def load() -> C.CDLL:
                                  C.POINTER(mqhip.Physics), C.c_uint32, C.c_uint32, C.c_int, C.POINTER(OrIon)]
     lib.or_summarize.argtypes = [C.POINTER(OrIon), u32p, C.c_size_t, C.c_size_t, C.c_double,
                                  C.POINTER(mqhip.Outcome)]
+    lib.or_grad_reference.argtypes = [C.POINTER(mqhip.Fields), C.POINTER(mqhip.Job), C.POINTER(C.c_double),
+                                      C.POINTER(mqhip.Physics), C.POINTER(mqhip.Objective), C.c_int,
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.or_sample.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int,
                               C.POINTER(C.c_double)]
     _lib = lib
@@ -140,3 +143,18 @@ def sample(F: np.ndarray, nx: int, ny: int, nz: int, g, nearest: int = 0) -> np.
     out = (C.c_double * 3)()
     load().or_sample(F.ctypes.data_as(C.POINTER(C.c_double)), nx, ny, nz, gg, nearest, out)
     return np.array(list(out))
+
+
+def grad_reference(grid: dict, job, dc, ph, obj, nthreads: int = 8):
+    """Dual-number (forward-mode) FP64 gradient: (J, grad[5], grad_state (6, n))."""
+    f, keep = mqhip.make_fields(grid)
+    J = C.c_double()
+    g = (C.c_double * 5)()
+    n = int(job.n)
+    gs = np.zeros(6 * n)
+    rc = load().or_grad_reference(C.byref(f), C.byref(job), (C.c_double * 4)(*dc), C.byref(ph), C.byref(obj),
+                                  nthreads, C.byref(J), g, gs.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise RuntimeError("oracle gradient failed")
+    del keep
+    return J.value, np.array(list(g)), gs.reshape(6, n)

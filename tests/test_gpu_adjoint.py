@@ -1,0 +1,83 @@
+"""GPU: the hand-written adjoint of the differentiable form (mq_forward / mq_backward)
+against the oracle's FP64 dual-number forward-mode gradient and against central
+finite differences of the GPU forward (h = 1e-6 relative)."""
+import numpy as np
+import pytest
+
+import mqhip
+import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1():
+    g = synth.c1_grid()
+    e = mqhip.Engine(g)
+    yield g, e
+    e.close()
+
+
+def setting(n=256, nsteps=2000):
+    ph = mqhip.Physics.correct(0.0).copy(dt=1e-7, injection=mqhip.INJ_GAUSSIAN, inj_sigma_mm=0.1, legacy_force=1)
+    job = mqhip.job(499.0, 1.0, 0.0, 100.0, 500e3, 0.5, 100e3, n, 1499)
+    obj = mqhip.Objective(nsteps, 0, 2.0e-3, 2e-5, 1.0e-3, 0.3)
+    return job, ph, obj
+
+
+def test_adjoint_matches_dual_numbers(c1, oracle_mod):
+    g, e = c1
+    job, ph, obj = setting()
+    print("[gpu] adjoint vs dual numbers", flush=True)
+    J = e.forward(job, [0, 5, 0, 0], ph, obj)
+    grad, gstate = e.backward(1.0, state=True)
+    Jr, gr, gsr = oracle_mod.grad_reference(g, job, [0, 5, 0, 0], ph, obj, 8)
+    print(f"[gpu]   J {J:.15g} vs {Jr:.15g}; grad {grad} vs {gr}", flush=True)
+    assert J == pytest.approx(Jr, rel=1e-12)
+    scale = np.abs(gr).max()
+    assert np.allclose(grad, gr, rtol=1e-9, atol=1e-12 * scale)
+    sscale = np.abs(gsr).max()
+    assert np.allclose(gstate, gsr, rtol=1e-8, atol=1e-10 * sscale)
+
+
+def test_adjoint_matches_central_differences(c1):
+    g, e = c1
+    job, ph, obj = setting(n=512, nsteps=1500)
+    print("[gpu] adjoint vs central differences", flush=True)
+    e.forward(job, [0, 5, 0, 0], ph, obj)
+    grad = e.backward(1.0)
+    names = [("drive", "v_rf"), ("drive", "f_rf"), ("drive", "v_ac"), ("drive", "f_ac"), (None, "dc_scale")]
+    for k, (grp, name) in enumerate(names):
+        base = getattr(job.drive if grp else job, name)
+        h = 1e-6 * abs(base)
+        vals = []
+        for sgn in (+1, -1):
+            j2 = mqhip.Job()
+            j2.sp, j2.drive, j2.n, j2.seed, j2.dc_scale = job.sp, job.drive, job.n, job.seed, job.dc_scale
+            setattr(j2.drive if grp else j2, name, base + sgn * h)
+            vals.append(e.forward(j2, [0, 5, 0, 0], ph, obj))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        print(f"[gpu]   {name}: adjoint {grad[k]:.8g} central difference {fd:.8g}", flush=True)
+        assert fd == pytest.approx(grad[k], rel=1e-4, abs=1e-9 * np.abs(grad).max())
+
+
+def test_checkpoint_interval_does_not_change_gradient(c1):
+    g, e = c1
+    job, ph, obj = setting(n=128, nsteps=700)
+    out = []
+    for K in (1, 7, 0, 700):
+        o = mqhip.Objective(obj.nsteps, K, obj.z_exit, obj.scale, obj.r_leak, obj.w_leak)
+        J = e.forward(job, [0, 5, 0, 0], ph, o)
+        out.append((J, e.backward(2.0)))
+    for J, gr in out[1:]:
+        assert J == out[0][0] and np.array_equal(gr, out[0][1])
+
+
+def test_adjoint_errors(c1):
+    g, e = c1
+    job, ph, obj = setting(n=8, nsteps=10)
+    e.run_batch([job], [0, 5, 0, 0], ph)
+    with pytest.raises(mqhip.MqError, match="preceding mq_forward"):
+        e.backward()
+    with pytest.raises(mqhip.MqError, match="collision-free"):
+        e.forward(job, [0, 5, 0, 0], ph.copy(pressure_pa=0.1), obj)
