@@ -113,6 +113,53 @@ def cpu_baseline(grid, jobs, dc, ph, seconds: float):
                       f"{ncpu} CPUs visible"}
 
 
+def grad_bench(fd: bool):
+    """Differentiable form: 65,536 collision-free ions, m/z 499, 20,000 steps, 0.05 mm grid with
+    FP32-rounded basis values; one gradient evaluation = forward + backward (adjoint)."""
+    grid = synth.c2_grid()
+    for k in ("rf", "ac"):
+        grid[k] = grid[k].astype(np.float32).astype(np.float64)
+    grid["dc"] = [d.astype(np.float32).astype(np.float64) if d is not None else None for d in grid["dc"]]
+    eng = mqhip.Engine(grid)
+    ph = mqhip.Physics.correct(0.0).copy(dt=1e-7, injection=mqhip.INJ_GAUSSIAN, inj_sigma_mm=0.1, legacy_force=1)
+    job = mqhip.job(499.0, 1.0, 0.0, 100.0, 500e3, 0.5, 100e3, 65536, 4499)
+    obj = mqhip.Objective(20000, 0, 18.6e-3, 2e-5, 1.0e-3, 0.3)
+    dc = [0.0, 5.0, 0.0, 0.0]
+    log("gradient: forward + backward of 65536 ions x 20000 steps")
+    t0 = time.perf_counter()
+    J = eng.forward(job, dc, ph, obj)
+    fwd = eng.stats()
+    grad = eng.backward(1.0)
+    bwd = eng.stats()
+    dt = time.perf_counter() - t0
+    out = {"evals_per_s": 1.0 / dt, "seconds_per_eval": dt, "J": J,
+           "grad": dict(zip(["V_rf", "f_rf", "V_exc", "f_exc", "V_dc"], grad.tolist())),
+           "forward_ms": fwd["kernel_ms"], "backward_ms": bwd["kernel_ms"],
+           "ion_steps_per_s": 3 * 65536 * 20000 / dt,
+           "config": "65536 ions, m/z 499 z=1, 20000 collision-free steps, 128x128x512 @ 0.05 mm (FP32-rounded), "
+                     "objective mean sigmoid((z-18.6 mm)/0.02 mm) - 0.3 sigmoid((r-1 mm)/0.02 mm), sqrt(N) checkpoints"}
+    if fd:
+        rel = {}
+        names = [("drive", "v_rf", "V_rf"), ("drive", "f_rf", "f_rf"), ("drive", "v_ac", "V_exc"),
+                 ("drive", "f_ac", "f_exc"), (None, "dc_scale", "V_dc")]
+        for k, (grp, name, label) in enumerate(names):
+            base = getattr(job.drive if grp else job, name)
+            h = 1e-6 * abs(base)
+            vals = []
+            for sgn in (+1, -1):
+                j2 = mqhip.Job()
+                j2.sp, j2.drive, j2.n, j2.seed, j2.dc_scale = job.sp, job.drive, job.n, job.seed, job.dc_scale
+                setattr(j2.drive if grp else j2, name, base + sgn * h)
+                vals.append(eng.forward(j2, dc, ph, obj))
+            fdv = (vals[0] - vals[1]) / (2 * h)
+            rel[label] = {"adjoint": float(grad[k]), "central_difference": fdv,
+                          "rel_err": abs(fdv - grad[k]) / max(abs(grad[k]), 1e-300)}
+            log(f"gradient check {label}: adjoint {grad[k]:.6g} FD {fdv:.6g}")
+        out["fd_check_h_rel_1e-6"] = rel
+    eng.close()
+    return out
+
+
 def read_traffic(name: str):
     """HBM bytes per worker launch from the committed rocprofv3 PMC summary, if any."""
     path = os.path.join(ROOT, "profiles", f"k_batched_{name}_traffic.md")
@@ -133,6 +180,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-grad", action="store_true")
+    ap.add_argument("--grad-fd", type=int, default=1, help="check the adjoint against central differences")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one torch elementwise copy of 1 GiB (read 1 GiB, write 1 GiB) first, as the "
                          "known-size reference for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -213,9 +262,11 @@ def main():
         }
         result["speedup_vs_plain_fp64"] = value / pv
 
+    eng.close()
+    if not a.skip_grad:
+        result["grad"] = grad_bench(bool(a.grad_fd))
     if not a.skip_cpu:
         result["cpu_baseline"] = cpu_baseline(grid, jobs, dc, ph, a.cpu_seconds)
-    eng.close()
     print(json.dumps(result), flush=True)
 
 
