@@ -6,7 +6,9 @@ stand in for them: an ideal quadrupole whose RF potential has the reference's
 orientation phi = -2 X Y / r0^2 (channel.rs:510-522, r0_profile), a uniform
 AC dipole across the left/right rod pairs (ac_sign, channel.rs:105), one DC
 basis (the plate slot, index 1) shaped as a Gaussian axial barrier with its
-Laplace-consistent radial term, and a conductor mask on the hyperbolic rod
+Laplace-consistent radial term, centred where the reference's gate plate sits
+(16.0-17.6 mm, before the exit plane at 18.6 mm: Geom::layers, channel.rs:50-54),
+and a conductor mask on the hyperbolic rod
 surfaces |X Y| >= r0^2 / 2 plus a cylindrical wall at 1.1 r0. All grids are
 node-major (i fastest), E per volt, FP64.
 """
@@ -17,7 +19,7 @@ import numpy as np
 R0_REFERENCE = 2.765e-3   # r0 of the reference channel (channel.rs:58, Geom::default)
 
 
-def ideal_channel(nx: int, ny: int, nz: int, h: float, r0: float = R0_REFERENCE, z_gate: float | None = None,
+def ideal_channel(nx: int, ny: int, nz: int, h: float, r0: float = R0_REFERENCE, z_gate: float = 16.8e-3,
                   w_gate: float = 1.0e-3, z_exit: float = 18.6e-3, wall: float = 1.1) -> dict:
     ax = (nx // 2) * h
     ay = (ny // 2) * h
@@ -32,8 +34,7 @@ def ideal_channel(nx: int, ny: int, nz: int, h: float, r0: float = R0_REFERENCE,
     rf[..., 2] = 0.0
     ac = np.zeros((nz, ny, nx, 3))
     ac[..., 0] = -1.0 / r0
-    zg = z_exit if z_gate is None else z_gate
-    u = Z - zg
+    u = Z - z_gate
     g = np.exp(-(u / w_gate) ** 2)
     g1 = -2.0 * u / w_gate**2 * g
     g2 = (4.0 * u**2 / w_gate**4 - 2.0 / w_gate**2) * g

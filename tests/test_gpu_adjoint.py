@@ -18,9 +18,9 @@ def c1():
     e.close()
 
 
-def setting(n=256, nsteps=2000):
+def setting(n=256, nsteps=2000, v_dc=5.0):
     ph = mqhip.Physics.correct(0.0).copy(dt=1e-7, injection=mqhip.INJ_GAUSSIAN, inj_sigma_mm=0.1, legacy_force=1)
-    job = mqhip.job(499.0, 1.0, 0.0, 100.0, 500e3, 0.5, 100e3, n, 1499)
+    job = mqhip.job(499.0, 1.0, 0.0, 100.0, 500e3, 0.5, 100e3, n, 1499, dc_scale=v_dc / 5.0)
     obj = mqhip.Objective(nsteps, 0, 2.0e-3, 2e-5, 1.0e-3, 0.3)
     return job, ph, obj
 
@@ -41,24 +41,35 @@ def test_adjoint_matches_dual_numbers(c1, oracle_mod):
 
 
 def test_adjoint_matches_central_differences(c1):
+    """Central differences with h = 1e-6 relative and their Richardson extrapolation (h, h/2).
+    Operating point with a 0.5 V gate: no ion reaches the barrier top, so the objective is
+    smooth on the scale of h (at 5 V the ions reflected near the barrier top make the
+    pathwise derivative of a 0.02 mm sigmoid ill-conditioned; the dual-number test above
+    covers that case exactly)."""
     g, e = c1
-    job, ph, obj = setting(n=512, nsteps=1500)
+    job, ph, obj = setting(n=512, nsteps=1500, v_dc=0.5)
     print("[gpu] adjoint vs central differences", flush=True)
     e.forward(job, [0, 5, 0, 0], ph, obj)
     grad = e.backward(1.0)
     names = [("drive", "v_rf"), ("drive", "f_rf"), ("drive", "v_ac"), ("drive", "f_ac"), (None, "dc_scale")]
-    for k, (grp, name) in enumerate(names):
+
+    def central(grp, name, h):
         base = getattr(job.drive if grp else job, name)
-        h = 1e-6 * abs(base)
         vals = []
         for sgn in (+1, -1):
             j2 = mqhip.Job()
             j2.sp, j2.drive, j2.n, j2.seed, j2.dc_scale = job.sp, job.drive, job.n, job.seed, job.dc_scale
             setattr(j2.drive if grp else j2, name, base + sgn * h)
             vals.append(e.forward(j2, [0, 5, 0, 0], ph, obj))
-        fd = (vals[0] - vals[1]) / (2 * h)
-        print(f"[gpu]   {name}: adjoint {grad[k]:.8g} central difference {fd:.8g}", flush=True)
-        assert fd == pytest.approx(grad[k], rel=1e-4, abs=1e-9 * np.abs(grad).max())
+        return (vals[0] - vals[1]) / (2 * h)
+
+    for k, (grp, name) in enumerate(names):
+        h = 1e-6 * abs(getattr(job.drive if grp else job, name))
+        d1, d2 = central(grp, name, h), central(grp, name, h / 2)
+        rich = (4 * d2 - d1) / 3
+        print(f"[gpu]   {name}: adjoint {grad[k]:.10g} central {d1:.10g} richardson {rich:.10g}", flush=True)
+        assert rich == pytest.approx(grad[k], rel=1e-4, abs=1e-9 * np.abs(grad).max())
+        assert d1 == pytest.approx(grad[k], rel=1e-3, abs=1e-9 * np.abs(grad).max())
 
 
 def test_checkpoint_interval_does_not_change_gradient(c1):

@@ -75,7 +75,7 @@ def test_uniform_rf_frequency_derivative(oracle_mod):
 def _c1_setting():
     grid = synth.c1_grid()
     ph = mqhip.Physics.correct(0.0).copy(dt=1e-7, injection=mqhip.INJ_GAUSSIAN, inj_sigma_mm=0.1, legacy_force=1)
-    job = mqhip.job(499.0, 1.0, 0.0, 100.0, 500e3, 0.5, 100e3, 48, 1499, dc_scale=1.0)
+    job = mqhip.job(499.0, 1.0, 0.0, 100.0, 500e3, 0.5, 100e3, 48, 1499, dc_scale=0.1)  # 0.5 V gate
     obj = mqhip.Objective(1500, 0, 2.0e-3, 2e-5, 1.0e-3, 0.3)
     return grid, ph, job, obj
 
@@ -85,17 +85,20 @@ def test_dual_numbers_match_central_differences(oracle_mod):
     dc = [0, 5, 0, 0]
     J, grad, _ = oracle_mod.grad_reference(grid, job, dc, ph, obj, 8)
     names = [("drive", "v_rf"), ("drive", "f_rf"), ("drive", "v_ac"), ("drive", "f_ac"), (None, "dc_scale")]
-    for k, (grp, name) in enumerate(names):
+    def central(grp, name, h):
         base = getattr(job.drive if grp else job, name)
-        h = 1e-6 * abs(base)
         vals = []
         for sgn in (+1, -1):
             j2 = mqhip.Job()
             j2.sp, j2.drive, j2.n, j2.seed, j2.dc_scale, j2.pressure_pa = job.sp, job.drive, job.n, job.seed, job.dc_scale, 0.0
             setattr(j2.drive if grp else j2, name, base + sgn * h)
             vals.append(oracle_mod.grad_reference(grid, j2, dc, ph, obj, 8)[0])
-        fd = (vals[0] - vals[1]) / (2 * h)
-        assert fd == pytest.approx(grad[k], rel=1e-4, abs=1e-12 * max(1.0, abs(grad).max())), (name, fd, grad[k])
+        return (vals[0] - vals[1]) / (2 * h)
+
+    for k, (grp, name) in enumerate(names):
+        h = 1e-6 * abs(getattr(job.drive if grp else job, name))
+        rich = (4 * central(grp, name, h / 2) - central(grp, name, h)) / 3
+        assert rich == pytest.approx(grad[k], rel=1e-4, abs=1e-12 * max(1.0, abs(grad).max())), (name, rich, grad[k])
 
 
 def test_coupled_linear_fields_central_differences(oracle_mod):
