@@ -143,28 +143,42 @@ def grad_bench(fd: bool):
                      " - 0.3 sigmoid((r-1 mm)/0.02 mm); sqrt(N) checkpoints"}
     if fd:
         rel = {}
+        n = 65536
+        gi = eng.backward_per_ion(n)   # per-ion adjoint contributions (dJ_i/dtheta / n)
         names = [("drive", "v_rf", "V_rf"), ("drive", "f_rf", "f_rf"), ("drive", "v_ac", "V_exc"),
                  ("drive", "f_ac", "f_exc"), (None, "dc_scale", "V_dc")]
 
         def central(grp, name, h):
             base = getattr(job.drive if grp else job, name)
-            vals = []
+            vals, per = [], []
             for sgn in (+1, -1):
                 j2 = mqhip.Job()
                 j2.sp, j2.drive, j2.n, j2.seed, j2.dc_scale = job.sp, job.drive, job.n, job.seed, job.dc_scale
                 setattr(j2.drive if grp else j2, name, base + sgn * h)
                 vals.append(eng.forward(j2, dc, ph, obj))
-            return (vals[0] - vals[1]) / (2 * h)
+                per.append(eng.forward_per_ion() / n)
+            return (vals[0] - vals[1]) / (2 * h), (per[0] - per[1]) / (2 * h)
 
         for k, (grp, name, label) in enumerate(names):
             h = 1e-6 * abs(getattr(job.drive if grp else job, name))
-            d1, d2 = central(grp, name, h), central(grp, name, h / 2)
-            rich = (4 * d2 - d1) / 3
+            d1, p1 = central(grp, name, h)
+            d2, p2 = central(grp, name, h / 2)
+            rich, prich = (4 * d2 - d1) / 3, (4 * p2 - p1) / 3
             g = float(grad[k])
+            # per-ion agreement: an ion is well conditioned when its Richardson central
+            # difference matches its adjoint contribution within 1e-4 (or 1e-6 of the
+            # largest contribution)
+            tol = 1e-4 * np.abs(gi[k]) + 1e-6 * np.abs(gi[k]).max()
+            ok = np.abs(prich - gi[k]) <= tol
             rel[label] = {"adjoint": g, "central_h": d1, "central_h2": d2, "richardson": rich,
                           "rel_err_central": abs(d1 - g) / max(abs(g), 1e-300),
-                          "rel_err_richardson": abs(rich - g) / max(abs(g), 1e-300)}
-            log(f"gradient check {label}: adjoint {g:.8g} central {d1:.8g} richardson {rich:.8g}")
+                          "rel_err_richardson": abs(rich - g) / max(abs(g), 1e-300),
+                          "ions_agreeing_1e-4": float(ok.mean()),
+                          "adjoint_over_agreeing_ions": float(gi[k][ok].sum()),
+                          "richardson_over_agreeing_ions": float(prich[ok].sum()),
+                          "largest_disagreeing_contributions": sorted(np.abs(gi[k][~ok]).tolist())[-3:]}
+            log(f"gradient check {label}: adjoint {g:.8g} central {d1:.8g} richardson {rich:.8g}; "
+                f"{ok.mean():.5f} of ions agree per ion")
         out["fd_check_h_rel_1e-6"] = rel
     eng.close()
     return out

@@ -208,6 +208,12 @@ int mq_forward(mq_engine* eng, const mq_job* job, const double dc[4], const mq_p
  * ion as 6 arrays of n (x for all ions, then y, ...). */
 int mq_backward(mq_engine* eng, double dJ_up, double grad[5], double* grad_state);
 
+/* Per-ion objective J_i of the last mq_forward (n values, J = mean J_i) and
+ * per-ion gradient contributions of the last mq_backward (5 arrays of n, in
+ * the order of grad; grad = sum over ions). For conditioning diagnostics. */
+int mq_forward_per_ion(mq_engine* eng, double* J_ion);
+int mq_backward_per_ion(mq_engine* eng, double* grad_ion);
+
 /* ------------------------------------------------------------ scan API */
 
 /* One point of an adaptive scan: AC frequency [kHz], transmitted and crashed

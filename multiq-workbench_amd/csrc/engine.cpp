@@ -887,6 +887,26 @@ int mq_backward(mq_engine* eng, double dJ_up, double grad[5], double* grad_state
     })
 }
 
+int mq_forward_per_ion(mq_engine* eng, double* J_ion)
+{
+    MQ_GUARD({
+        if (!eng || !J_ion) throw Err(-1, "NULL argument");
+        if (!eng->adj_valid) throw Err(-1, "needs a preceding mq_forward on this engine");
+        HIPCHK(hipMemcpy(J_ion, eng->adj_buf.Ji, (size_t)eng->adj_n * sizeof(double), hipMemcpyDeviceToHost));
+    })
+}
+
+int mq_backward_per_ion(mq_engine* eng, double* grad_ion)
+{
+    MQ_GUARD({
+        if (!eng || !grad_ion) throw Err(-1, "NULL argument");
+        if (!eng->adj_valid) throw Err(-1, "needs a preceding mq_forward and mq_backward on this engine");
+        for (int k = 0; k < 5; ++k)
+            HIPCHK(hipMemcpy(grad_ion + (size_t)k * eng->adj_n, eng->adj_buf.gp[k], (size_t)eng->adj_n * sizeof(double),
+                             hipMemcpyDeviceToHost));
+    })
+}
+
 int mq_get_tune(const mq_engine* eng, mq_tune* out)
 {
     MQ_GUARD({

@@ -137,6 +137,8 @@ SIGNATURES = {
     "mq_forward": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_double), C.POINTER(Physics),
                              C.POINTER(Objective), C.POINTER(C.c_double)]),
     "mq_backward": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mq_forward_per_ion": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "mq_backward_per_ion": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "mq_cached_batch": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(Job), C.c_size_t, C.POINTER(C.c_double),
                                   C.POINTER(Physics), C.c_int32, C.c_char_p, C.POINTER(Outcome),
                                   C.POINTER(C.c_size_t)]),
@@ -296,6 +298,16 @@ class Engine:
         gs = np.zeros(6 * n) if state else None
         _check(load().mq_backward(self._h, dJ, g, _dptr(gs) if state else None))
         return (np.array(list(g)), gs.reshape(6, n)) if state else np.array(list(g))
+
+    def forward_per_ion(self) -> np.ndarray:
+        out = np.zeros(self.stats()["n_ions"])
+        _check(load().mq_forward_per_ion(self._h, _dptr(out)))
+        return out
+
+    def backward_per_ion(self, n: int) -> np.ndarray:
+        out = np.zeros(5 * n)
+        _check(load().mq_backward_per_ion(self._h, _dptr(out)))
+        return out.reshape(5, n)
 
     def autotune(self, jobs, dc, ph: Physics, sample_ions: int = 0, cache_path: str | None = None,
                  force: bool = False) -> tuple[dict, float]:

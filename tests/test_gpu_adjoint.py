@@ -72,6 +72,17 @@ def test_adjoint_matches_central_differences(c1):
         assert d1 == pytest.approx(grad[k], rel=1e-3, abs=1e-9 * np.abs(grad).max())
 
 
+def test_per_ion_outputs_sum_to_totals(c1):
+    g, e = c1
+    job, ph, obj = setting(n=300, nsteps=500)
+    J = e.forward(job, [0, 5, 0, 0], ph, obj)
+    Ji = e.forward_per_ion()
+    grad = e.backward(1.0)
+    gi = e.backward_per_ion(300)
+    assert Ji.mean() == pytest.approx(J, rel=1e-13)
+    assert np.allclose(gi.sum(axis=1), grad, rtol=1e-12, atol=1e-15 * np.abs(grad).max())
+
+
 def test_checkpoint_interval_does_not_change_gradient(c1):
     g, e = c1
     job, ph, obj = setting(n=128, nsteps=700)
