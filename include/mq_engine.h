@@ -120,9 +120,9 @@ typedef struct {
 } mq_run_stats;
 
 /* Tuning of the batched path (TileCfg, tiled.rs:12-29). mq_set_tune: zero
- * in block, waves_per_cu, refill_period, chunk, chunk_cap, sync_every keeps
- * the current value; resort and graph are always taken. Results per ion do
- * not depend on any of these. */
+ * in block, waves_per_cu, refill_period, chunk, chunk_cap, sync_every and
+ * coll_batch keeps the current value; resort and graph are always taken.
+ * Results per ion do not depend on any of these. */
 typedef struct {
     int32_t block;           /* threads per workgroup (64..256, multiple of 64) */
     int32_t waves_per_cu;    /* persistent waves launched per CU */
@@ -134,6 +134,8 @@ typedef struct {
     int32_t sync_every;      /* launches enqueued per host check of the live count */
     int32_t graph;           /* 1: capture those launches as one hipGraph and replay it
                                 (sync_every must be even) */
+    int32_t coll_batch;      /* lanes of a wavefront whose collision is pending before the
+                                collision code runs for all of them (1: at once) */
 } mq_tune;
 
 typedef struct mq_engine mq_engine;

@@ -133,7 +133,7 @@ struct mq_engine {
     // per-call pools
     DevBuf jobs, acc, list[3], ctr, bin_of, rank_of, hist, bsum;
     DevBuf sx, sy, sz, sphi, svx, svy, svz, sfx, sfy, sfz, ssteps, sncoll, scd, sjob, sstatus;
-    mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0};
+    mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0, 1};
     // differentiable form: context of the last forward
     bool adj_valid = false;
     uint32_t adj_n = 0;
@@ -493,6 +493,7 @@ struct mq_engine {
             a.chunk = chunk;
             a.period = period;
             a.nqueues = resort ? MAX_QUEUES : 1u;
+            a.coll_batch = (uint32_t)std::max(1, tune.coll_batch);
             launch_prepare(a, stream);
             if (timed) HIPCHK(hipEventRecord((*evs)[2 * it], stream));
             launch_batched(a, grid, block, stream);
@@ -851,6 +852,10 @@ int mq_set_tune(mq_engine* eng, const mq_tune* t)
             n.sync_every = t->sync_every;
         }
         if (t->graph < 0 || t->graph > 1) throw Err(-1, "graph must be 0 or 1");
+        if (t->coll_batch) {
+            if (t->coll_batch < 1 || t->coll_batch > 64) throw Err(-1, "coll_batch must be in [1, 64]");
+            n.coll_batch = t->coll_batch;
+        }
         n.graph = t->graph;
         if (n.graph && (n.sync_every % 2)) throw Err(-1, "graph mode needs an even sync_every");
         eng->tune = n;

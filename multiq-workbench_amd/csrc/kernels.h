@@ -39,6 +39,7 @@ struct WorkerArgs {
     uint32_t chunk;     // max steps per ion in this launch (multiple of 4)
     uint32_t period;    // steps between refills (multiple of 4)
     uint32_t nqueues;   // 1, or MAX_QUEUES contiguous segments (one per XCD)
+    uint32_t coll_batch;  // pending collisions per wavefront that trigger the collision code
 };
 
 // Coarse cell bins of the re-sort: bin = ((k >> sz) nby + (j >> sy)) nbx + (i >> sx).

@@ -182,7 +182,8 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
     const float dtf = (float)B.dt;
     Lane L;
     float4 R[CELL_F4];
-    bool have = false, drained = false;
+    bool have = false, drained = false, pending = false;
+    uint32_t pcur = 0;
     L.cell = -1;
     for (;;) {
         // ---- refill idle lanes from the live list (one atomic per wavefront). With
@@ -226,11 +227,18 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
             }
         }
         if (!__any(have)) break;
-        // ---- advance every held ion by up to `period` steps
+        // ---- advance every held ion by up to `period` steps. A lane whose collision
+        // candidate fires (thermal) or whose countdown expires (legacy) becomes
+        // `pending` and pauses; the collision code then runs once for every pending
+        // lane of the wavefront when at least `coll_batch` lanes wait, when no lane
+        // can step, or at the end of the period (the thermal analogue of tc_tail_leg,
+        // channel_tiled.metal:485-585). An ion's own sequence of operations is
+        // unchanged: its classification depends on its position only and runs at
+        // the step, and its next kick waits for its collision.
         bool stop = false;
         int status = 0;
         for (uint32_t r = 0; r < A.period; ++r) {
-            if (have && !stop) {
+            if (have && !stop && !pending) {
                 const float pxy = L.tx * L.ty, pyz = L.ty * L.tz, pxz = L.tx * L.tz, pxyz = pxy * L.tz;
                 float e[3];
 #pragma unroll
@@ -252,7 +260,7 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                 const int32_t c = locate(B, L);
                 if (c != L.cell) { L.cell = c; load_cell(A.cells, c, R); }
                 phases(L);
-                // ---- collisions
+                // ---- collision candidate of this step
                 if (!B.legacy_coll) {
                     if (L.nsig > 0.0f) {
                         const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
@@ -265,44 +273,11 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                         } else {
                             cw = ph == 1 ? L.cw1 : (ph == 2 ? L.cw2 : L.cw3);
                         }
-                        if (u01f(cw) < L.nsig * bound) {
-                            const u32x4 w = philox(L.ion, cur, 0, D_COLL, L.k0, L.k1);
-                            const float r1 = sqrtf(-2.0f * log_f(u01f(w.a))), r2 = sqrtf(-2.0f * log_f(u01f(w.c)));
-                            const float a1 = u01f(w.b), a2 = u01f(w.d);
-                            const float gx = L.vx - L.sgas * (r1 * cos_rev(a1));
-                            const float gy = L.vy - L.sgas * (r1 * sin_rev(a1));
-                            const float gz = L.vz - L.sgas * (r2 * cos_rev(a2));
-                            const float gl = sqrtf(fmaf(gz, gz, fmaf(gy, gy, gx * gx)));
-                            const u32x4 w2 = philox(L.ion, cur, 1, D_COLL, L.k0, L.k1);
-                            if (u01f(w2.a) * bound < gl) {
-                                const float cth = 2.0f * u01f(w2.b) - 1.0f, a3 = u01f(w2.c);
-                                const float sn = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
-                                const float mg = L.mr * gl;
-                                L.vx = fmaf(mg, sn * cos_rev(a3), fmaf(-L.mr, gx, L.vx));
-                                L.vy = fmaf(mg, sn * sin_rev(a3), fmaf(-L.mr, gy, L.vy));
-                                L.vz = fmaf(mg, cth, fmaf(-L.mr, gz, L.vz));
-                                L.ncoll += 1;
-                            }
-                        }
+                        if (u01f(cw) < L.nsig * bound) { pending = true; pcur = cur; }
                     }
                 } else {
                     L.cd -= 1;
-                    if (L.cd == 0) {
-                        const u32x4 w = philox(L.ion, cur, 0, D_COLL, L.k0, L.k1);
-                        const float vm = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx)));
-                        if (vm > L.floorv) {
-                            const float cth = 2.0f * u01f(w.a) - 1.0f, a3 = u01f(w.b);
-                            const float sn = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
-                            const float mv = L.mr * vm;
-                            L.vx = fmaf(mv, sn * cos_rev(a3), L.vx * L.omr);
-                            L.vy = fmaf(mv, sn * sin_rev(a3), L.vy * L.omr);
-                            L.vz = fmaf(mv, cth, L.vz * L.omr);
-                            L.ncoll += 1;
-                        }
-                        const float mfp = -log_f(fmaxf(u01f(w.c), 1e-7f)) * L.mfpm;
-                        const float vm2 = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx)));
-                        L.cd = countdown_f(floorf(mfp / fmaxf(vm2, 1e-3f) / dtf));
-                    }
+                    if (L.cd == 0) { pending = true; pcur = cur; }
                 }
                 L.visit += 1;
                 // ---- classification (channel.metal:114-120)
@@ -316,6 +291,52 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                 }
                 if (status == 0 && L.st >= B.max_steps) status = 5;
                 if (status != 0 || L.visit >= A.chunk) stop = true;
+            }
+            // ---- batched collisions of the pending lanes
+            const uint64_t pm = __ballot(pending);
+            if (pm) {
+                const uint32_t npend = (uint32_t)__popcll(pm);
+                const uint64_t am = __ballot(have && !stop && !pending);
+                if (npend >= A.coll_batch || am == 0 || r + 1 == A.period) {
+                    if (pending) {
+                        if (!B.legacy_coll) {
+                            const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
+                            const u32x4 w = philox(L.ion, pcur, 0, D_COLL, L.k0, L.k1);
+                            const float r1 = sqrtf(-2.0f * log_f(u01f(w.a))), r2 = sqrtf(-2.0f * log_f(u01f(w.c)));
+                            const float a1 = u01f(w.b), a2 = u01f(w.d);
+                            const float gx = L.vx - L.sgas * (r1 * cos_rev(a1));
+                            const float gy = L.vy - L.sgas * (r1 * sin_rev(a1));
+                            const float gz = L.vz - L.sgas * (r2 * cos_rev(a2));
+                            const float gl = sqrtf(fmaf(gz, gz, fmaf(gy, gy, gx * gx)));
+                            const u32x4 w2 = philox(L.ion, pcur, 1, D_COLL, L.k0, L.k1);
+                            if (u01f(w2.a) * bound < gl) {
+                                const float cth = 2.0f * u01f(w2.b) - 1.0f, a3 = u01f(w2.c);
+                                const float sn = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
+                                const float mg = L.mr * gl;
+                                L.vx = fmaf(mg, sn * cos_rev(a3), fmaf(-L.mr, gx, L.vx));
+                                L.vy = fmaf(mg, sn * sin_rev(a3), fmaf(-L.mr, gy, L.vy));
+                                L.vz = fmaf(mg, cth, fmaf(-L.mr, gz, L.vz));
+                                L.ncoll += 1;
+                            }
+                        } else {
+                            const u32x4 w = philox(L.ion, pcur, 0, D_COLL, L.k0, L.k1);
+                            const float vm = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx)));
+                            if (vm > L.floorv) {
+                                const float cth = 2.0f * u01f(w.a) - 1.0f, a3 = u01f(w.b);
+                                const float sn = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
+                                const float mv = L.mr * vm;
+                                L.vx = fmaf(mv, sn * cos_rev(a3), L.vx * L.omr);
+                                L.vy = fmaf(mv, sn * sin_rev(a3), L.vy * L.omr);
+                                L.vz = fmaf(mv, cth, L.vz * L.omr);
+                                L.ncoll += 1;
+                            }
+                            const float mfp = -log_f(fmaxf(u01f(w.c), 1e-7f)) * L.mfpm;
+                            const float vm2 = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx)));
+                            L.cd = countdown_f(floorf(mfp / fmaxf(vm2, 1e-3f) / dtf));
+                        }
+                        pending = false;
+                    }
+                }
             }
         }
         // ---- write back stopped ions; carry unfinished ones to the next launch

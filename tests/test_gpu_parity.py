@@ -126,6 +126,10 @@ def test_batched_independent_of_batch_and_tuning(c1):
         dict(block=256, waves_per_cu=4, refill_period=16, chunk=96, chunk_cap=96, resort=1, sync_every=4, graph=1),
         dict(block=256, waves_per_cu=8, refill_period=8, chunk=20000, chunk_cap=200000, resort=0, sync_every=2,
              graph=1),
+        dict(block=256, waves_per_cu=8, refill_period=16, chunk=256, chunk_cap=256, resort=1, sync_every=16, graph=0,
+             coll_batch=6),
+        dict(block=64, waves_per_cu=4, refill_period=4, chunk=128, chunk_cap=128, resort=1, sync_every=8, graph=0,
+             coll_batch=64),
     ]
     runs = []
     try:
@@ -142,6 +146,26 @@ def test_batched_independent_of_batch_and_tuning(c1):
     again = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
     for k in KEYS + ("steps", "ncoll", "status"):
         assert np.array_equal(full[k], again[k]), k
+
+
+def test_batched_collision_batching_is_exact(c1):
+    """Deferring collisions to a wavefront-wide batch leaves every ion's result unchanged
+    (thermal and legacy collision models)."""
+    g, e = c1
+    t0 = e.get_tune()
+    try:
+        for case in ("half_thermal", "legacy"):
+            ph = physics(case)
+            jobs = jobs_for(case, 600)
+            e.set_tune(coll_batch=1)
+            a = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+            for cb in (4, 16):
+                e.set_tune(coll_batch=cb)
+                b = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+                for k in KEYS + ("steps", "ncoll", "status"):
+                    assert np.array_equal(a[k], b[k]), (case, cb, k)
+    finally:
+        e.set_tune(**t0)
 
 
 def test_plain_independent_of_batch(c1):
