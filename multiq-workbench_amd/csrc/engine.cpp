@@ -616,8 +616,9 @@ bool read_tune_cache(const char* path, const std::string& key, mq_tune& t)
         const size_t tab = l.find('\t');
         if (tab == std::string::npos || l.substr(0, tab) != key) continue;
         mq_tune x{};
-        if (std::sscanf(l.c_str() + tab + 1, "%d,%d,%d,%d,%d,%d,%d,%d", &x.block, &x.waves_per_cu, &x.refill_period,
-                        &x.chunk, &x.chunk_cap, &x.resort, &x.sync_every, &x.graph) == 8) {
+        if (std::sscanf(l.c_str() + tab + 1, "%d,%d,%d,%d,%d,%d,%d,%d,%d", &x.block, &x.waves_per_cu,
+                        &x.refill_period, &x.chunk, &x.chunk_cap, &x.resort, &x.sync_every, &x.graph,
+                        &x.coll_batch) == 9) {
             t = x;  // the last entry for a key wins
             found = true;
         }
@@ -652,19 +653,21 @@ static void autotune(mq_engine& e, const mq_job* jobs, size_t njobs, const doubl
     std::vector<mq_tune> cands;
     for (int resort : {1, 0})
         for (int chunk : {128, 256, 512})
-            for (int waves : {4, 8}) {
-                if (!resort && chunk != 256) continue;
-                mq_tune t = e.tune;
-                t.resort = resort;
-                t.chunk = resort ? chunk : 20000;
-                t.chunk_cap = resort ? chunk : 200000;
-                t.waves_per_cu = waves;
-                t.block = 256;
-                t.refill_period = 8;
-                t.sync_every = 16;
-                t.graph = 0;
-                cands.push_back(t);
-            }
+            for (int waves : {4, 8})
+                for (int cb : {1, 8}) {
+                    if (!resort && (chunk != 256 || cb != 8)) continue;
+                    mq_tune t = e.tune;
+                    t.coll_batch = cb;
+                    t.resort = resort;
+                    t.chunk = resort ? chunk : 20000;
+                    t.chunk_cap = resort ? chunk : 200000;
+                    t.waves_per_cu = waves;
+                    t.block = 256;
+                    t.refill_period = 8;
+                    t.sync_every = 16;
+                    t.graph = 0;
+                    cands.push_back(t);
+                }
     const mq_tune saved = e.tune;
     std::vector<unsigned long long> acc;
     size_t n = 0;
@@ -679,9 +682,9 @@ static void autotune(mq_engine& e, const mq_job* jobs, size_t njobs, const doubl
     if (cache_path) {
         FILE* f = std::fopen(cache_path, "a");
         if (f) {
-            std::fprintf(f, "%s\t%d,%d,%d,%d,%d,%d,%d,%d\t%.6g\n", key.c_str(), best.block, best.waves_per_cu,
+            std::fprintf(f, "%s\t%d,%d,%d,%d,%d,%d,%d,%d,%d\t%.6g\n", key.c_str(), best.block, best.waves_per_cu,
                          best.refill_period, best.chunk, best.chunk_cap, best.resort, best.sync_every, best.graph,
-                         best_rate);
+                         best.coll_batch, best_rate);
             std::fclose(f);
         }
     }
