@@ -133,7 +133,7 @@ struct mq_engine {
     // per-call pools
     DevBuf jobs, acc, list[3], ctr, bin_of, rank_of, hist, bsum;
     DevBuf sx, sy, sz, sphi, svx, svy, svz, sfx, sfy, sfz, ssteps, sncoll, scd, sjob, sstatus;
-    mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0, 1};
+    mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0, 4};
     // differentiable form: context of the last forward
     bool adj_valid = false;
     uint32_t adj_n = 0;
