@@ -108,6 +108,10 @@ typedef struct {
     uint32_t *steps;   /* steps executed (time = steps * dt) */
     uint32_t *ncoll;   /* collisions */
     uint8_t *status;   /* MQ_* status */
+    /* Optional (NULL = not returned): */
+    double *phi_ac0;   /* AC phase at injection [cycles, 0..1); the phase at the
+                          end is frac(phi_ac0 + f_ac * dt * steps) */
+    uint32_t *cd;      /* legacy collision countdown (steps to the next collision) */
 } mq_ion_soa;
 
 /* Statistics of the last run on an engine. */
@@ -255,6 +259,39 @@ int mq_scan_key(const char* geom_key, const mq_job* job, const double dc[4], con
 /* An f64 in Rust Debug ({:?}, display = 0) or Display ({}, display = 1)
  * notation, as used in the cache file; same return convention as mq_scan_key. */
 int mq_format_f64(double x, int32_t display, char* buf, size_t len);
+
+/* ---------------------------------------------------------- record compat */
+
+/* The reference's 64-byte per-ion record, ChIon (channel.rs:292-294,
+ * channel.metal:31-36): p = fixed-point position at MQ_POS_SCALE per metre
+ * (62.5 pm) with the status in p[3]; v = velocity [m/s] with the ion's own
+ * time [s] in v[3]; aux = {AC phase [cycles], collisions, legacy countdown,
+ * RF phase [cycles]}; rng = the random-stream position of this build,
+ * {key lo, key hi, ion index in its job, steps} (the reference holds its
+ * xoshiro128** state there). Little-endian, no padding. */
+#define MQ_POS_SCALE 1.6e10
+typedef struct {
+    int32_t p[4];
+    float v[4];
+    float aux[4];
+    uint32_t rng[4];
+} mq_chion;
+
+/* Writes n ChIon records from final states (ions in job order, as returned by
+ * mq_run_batch_ions). s->phi_ac0 and s->cd are read when non-NULL (else the
+ * AC phase and countdown are written as 0). Positions are rounded to the
+ * nearest quantum, as the reference's drift does (channel.metal:88-90). */
+int mq_export_chion(const mq_ion_soa* s, const uint32_t* job, size_t n, const mq_job* jobs, size_t njobs,
+                    const mq_physics* ph, mq_chion* out);
+
+/* summarize (channel.rs:496-506) over ChIon records: status from p[3],
+ * collisions from aux[1], time from v[3], summed in f64. */
+int mq_summarize_chion(const mq_chion* ions, const uint32_t* job, size_t n, size_t njobs, mq_outcome* out);
+
+/* FNV-1a 64-bit hash of a byte range, as check.rs:100-104 hashes the raw
+ * ChIon bytes of a run (offset basis 14695981039346656037, prime
+ * 1099511628211). */
+uint64_t mq_fnv1a64(const void* data, size_t nbytes);
 
 #ifdef __cplusplus
 }

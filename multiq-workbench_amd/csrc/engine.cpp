@@ -779,6 +779,7 @@ int mq_run_batch_ions(mq_engine* eng, const mq_job* jobs, size_t njobs, const do
         };
         d2h(out->x, S.x, n * 8); d2h(out->y, S.y, n * 8); d2h(out->z, S.z, n * 8);
         d2h(out->steps, S.steps, n * 4); d2h(out->ncoll, S.ncoll, n * 4); d2h(out->status, S.status, n);
+        d2h(out->phi_ac0, S.phi, n * 8); d2h(out->cd, S.cd, n * 4);
         if (out_job) d2h(out_job, S.job, n * 4);
         if (path == MQ_PATH_PLAIN) {
             d2h(out->vx, S.vx, n * 8); d2h(out->vy, S.vy, n * 8); d2h(out->vz, S.vz, n * 8);

@@ -95,7 +95,13 @@ class IonSoA(C.Structure):
         ("x", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double)), ("z", C.POINTER(C.c_double)),
         ("vx", C.POINTER(C.c_double)), ("vy", C.POINTER(C.c_double)), ("vz", C.POINTER(C.c_double)),
         ("steps", C.POINTER(C.c_uint32)), ("ncoll", C.POINTER(C.c_uint32)), ("status", C.POINTER(C.c_uint8)),
+        ("phi_ac0", C.POINTER(C.c_double)), ("cd", C.POINTER(C.c_uint32)),
     ]
+
+
+# numpy view of mq_chion (ChIon, channel.rs:292-294): 64 bytes per ion.
+CHION_DTYPE = np.dtype([("p", "<i4", 4), ("v", "<f4", 4), ("aux", "<f4", 4), ("rng", "<u4", 4)])
+POS_SCALE = 1.6e10
 
 
 class RunStats(C.Structure):
@@ -152,6 +158,11 @@ SIGNATURES = {
     "mq_scan_key": (C.c_int, [C.c_char_p, C.POINTER(Job), C.POINTER(C.c_double), C.POINTER(Physics), C.c_char_p,
                               C.c_size_t]),
     "mq_format_f64": (C.c_int, [C.c_double, C.c_int32, C.c_char_p, C.c_size_t]),
+    "mq_export_chion": (C.c_int, [C.POINTER(IonSoA), C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(Job), C.c_size_t,
+                                  C.POINTER(Physics), C.c_void_p]),
+    "mq_summarize_chion": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_size_t, C.c_size_t,
+                                     C.POINTER(Outcome)]),
+    "mq_fnv1a64": (C.c_uint64, [C.c_void_p, C.c_size_t]),
 }
 
 _lib = None
@@ -262,12 +273,9 @@ class Engine:
         o["ncoll"] = np.zeros(n, np.uint32)
         o["status"] = np.zeros(n, np.uint8)
         o["job"] = np.zeros(n, np.uint32)
-        soa = IonSoA()
-        for k in ("x", "y", "z", "vx", "vy", "vz"):
-            setattr(soa, k, _dptr(o[k]))
-        soa.steps = o["steps"].ctypes.data_as(C.POINTER(C.c_uint32))
-        soa.ncoll = o["ncoll"].ctypes.data_as(C.POINTER(C.c_uint32))
-        soa.status = o["status"].ctypes.data_as(C.POINTER(C.c_uint8))
+        o["phi_ac0"] = np.zeros(n, np.float64)
+        o["cd"] = np.zeros(n, np.uint32)
+        soa = soa_of(o)
         dcv = (C.c_double * 4)(*dc)
         _check(load().mq_run_batch_ions(self._h, ja, len(jobs), dcv, C.byref(ph), path, C.byref(soa),
                                         o["job"].ctypes.data_as(C.POINTER(C.c_uint32))))
@@ -335,6 +343,53 @@ def summarize(status, ncoll, steps, job, njobs: int, dt: float) -> list[Outcome]
     _check(load().mq_summarize(status.ctypes.data_as(C.POINTER(C.c_uint8)), ncoll.ctypes.data_as(u32),
                                steps.ctypes.data_as(u32), job.ctypes.data_as(u32), len(status), njobs, dt, out))
     return list(out)
+
+
+def soa_of(o: dict) -> IonSoA:
+    """mq_ion_soa over the arrays of a dict of final states (optional keys may be absent)."""
+    soa = IonSoA()
+    for k in ("x", "y", "z", "vx", "vy", "vz", "phi_ac0"):
+        if k in o:
+            setattr(soa, k, _dptr(o[k]))
+    for k in ("steps", "ncoll", "cd"):
+        if k in o:
+            setattr(soa, k, o[k].ctypes.data_as(C.POINTER(C.c_uint32)))
+    soa.status = o["status"].ctypes.data_as(C.POINTER(C.c_uint8))
+    return soa
+
+
+def export_chion(o: dict, jobs, ph: Physics) -> np.ndarray:
+    """64-byte ChIon records (channel.rs:292-294) of final states in job order."""
+    n = len(o["status"])
+    for k in ("x", "y", "z", "vx", "vy", "vz", "phi_ac0"):
+        if k in o:
+            o[k] = np.ascontiguousarray(o[k], np.float64)
+    for k in ("steps", "ncoll", "cd", "job"):
+        if k in o:
+            o[k] = np.ascontiguousarray(o[k], np.uint32)
+    o["status"] = np.ascontiguousarray(o["status"], np.uint8)
+    out = np.zeros(n, CHION_DTYPE)
+    ja = jobs_array(jobs)
+    _check(load().mq_export_chion(C.byref(soa_of(o)), o["job"].ctypes.data_as(C.POINTER(C.c_uint32)), n, ja,
+                                  len(jobs), C.byref(ph), out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+def summarize_chion(recs: np.ndarray, job, njobs: int) -> list[Outcome]:
+    """summarize (channel.rs:496-506) over ChIon records."""
+    recs = np.ascontiguousarray(recs, CHION_DTYPE)
+    job = np.ascontiguousarray(job, np.uint32)
+    out = (Outcome * max(njobs, 1))()
+    _check(load().mq_summarize_chion(recs.ctypes.data_as(C.c_void_p), job.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                     len(recs), njobs, out))
+    return list(out)[:njobs]
+
+
+def fnv1a64(data) -> int:
+    """FNV-1a 64 of raw bytes (check.rs:100-104 hashes ChIon bytes this way)."""
+    b = memoryview(np.ascontiguousarray(data)).cast("B") if not isinstance(data, (bytes, bytearray)) else data
+    buf = (C.c_char * len(b)).from_buffer_copy(bytes(b))
+    return int(load().mq_fnv1a64(C.cast(buf, C.c_void_p), len(b)))
 
 
 # ------------------------------------------------------------------ scan API
