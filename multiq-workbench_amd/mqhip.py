@@ -99,6 +99,44 @@ class IonSoA(C.Structure):
     ]
 
 
+class Geom(C.Structure):
+    """mq_geom (include/mq_fieldsolve.h): Geom of channel.rs:18-36, lengths in mm."""
+    _fields_ = [("de", C.c_double), ("coef", C.c_double), ("q1_start", C.c_double), ("q1_len", C.c_double),
+                ("alpha", C.c_double), ("gap", C.c_double), ("plate", C.c_double), ("hole_d", C.c_double),
+                ("q2_len", C.c_double), ("stem_d", C.c_double), ("q2_flare_bug", C.c_int32), ("grow", C.c_double),
+                ("tilt_mode", C.c_int32), ("ac_mode", C.c_int32), ("mis", (C.c_double * 4) * 4),
+                ("n_xy", C.c_int32), ("h_mm", C.c_double)]
+
+    @staticmethod
+    def default(**kw) -> "Geom":
+        """Geom::default (channel.rs:38-43), with keyword overrides."""
+        g = Geom()
+        load().mq_geom_default(C.byref(g))
+        for k, v in kw.items():
+            setattr(g, k, v)
+        return g
+
+    @staticmethod
+    def s14(**kw) -> "Geom":
+        """The corrected S14 geometry (s14_geom "code" variant, channel.rs:605-619): grown rods, parallel Q2."""
+        return Geom.default(grow=0.4, q2_flare_bug=0, **kw)
+
+    def dims(self):
+        nx, ny, nz = C.c_int32(), C.c_int32(), C.c_int32()
+        lay = (C.c_double * 8)()
+        _check(load().mq_geom_dims(C.byref(self), C.byref(nx), C.byref(ny), C.byref(nz), lay))
+        return nx.value, ny.value, nz.value, list(lay)
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("residual", C.c_double), ("converged", C.c_int32), ("ms", C.c_double)]
+
+
+class ChannelFields(C.Structure):
+    _fields_ = [("rf", C.POINTER(C.c_double)), ("ac", C.POINTER(C.c_double)), ("dc", C.POINTER(C.c_double) * 4),
+                ("phi", C.POINTER(C.c_double) * 6), ("mask", C.POINTER(C.c_uint8)), ("stats", SolveStats * 6)]
+
+
 # numpy view of mq_chion (ChIon, channel.rs:292-294): 64 bytes per ion.
 CHION_DTYPE = np.dtype([("p", "<i4", 4), ("v", "<f4", 4), ("aux", "<f4", 4), ("rng", "<u4", 4)])
 POS_SCALE = 1.6e10
@@ -163,6 +201,18 @@ SIGNATURES = {
     "mq_summarize_chion": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_size_t, C.c_size_t,
                                      C.POINTER(Outcome)]),
     "mq_fnv1a64": (C.c_uint64, [C.c_void_p, C.c_size_t]),
+    "mq_geom_default": (None, [C.POINTER(Geom)]),
+    "mq_geom_dims": (C.c_int, [C.POINTER(Geom), C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                               C.POINTER(C.c_double)]),
+    "mq_geom_fields_params": (C.c_int, [C.POINTER(Geom), C.POINTER(Fields)]),
+    "mq_geom_classify": (C.c_int, [C.POINTER(Geom), C.POINTER(C.c_uint8)]),
+    "mq_laplace_solve": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_uint8), C.POINTER(C.c_double),
+                                   C.c_double, C.c_double, C.c_uint32, C.POINTER(C.c_double),
+                                   C.POINTER(SolveStats)]),
+    "mq_efield": (C.c_int, [C.POINTER(C.c_double), C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                            C.POINTER(C.c_double)]),
+    "mq_fields_build": (C.c_int, [C.POINTER(Geom), C.c_double, C.c_char_p, C.POINTER(ChannelFields)]),
+    "mq_field_cache_name": (C.c_int, [C.POINTER(Geom), C.c_int32, C.c_char_p, C.c_size_t]),
 }
 
 _lib = None
@@ -390,6 +440,76 @@ def fnv1a64(data) -> int:
     b = memoryview(np.ascontiguousarray(data)).cast("B") if not isinstance(data, (bytes, bytearray)) else data
     buf = (C.c_char * len(b)).from_buffer_copy(bytes(b))
     return int(load().mq_fnv1a64(C.cast(buf, C.c_void_p), len(b)))
+
+
+# ------------------------------------------------------------------ field build
+
+BASES = ("Rf", "Ac", "DcQ1", "DcPlate", "DcDown", "DcBox")
+
+
+def geom_classify(g: Geom) -> np.ndarray:
+    """MQ_ELEC_* code per node, shape (nz, ny, nx) (Geom::classify on the GPU)."""
+    nx, ny, nz, _ = g.dims()
+    out = np.zeros(nx * ny * nz, np.uint8)
+    _check(load().mq_geom_classify(C.byref(g), out.ctypes.data_as(C.POINTER(C.c_uint8))))
+    return out.reshape(nz, ny, nx)
+
+
+def laplace_solve(mask: np.ndarray, fixed: np.ndarray, outer_bc: float, tol: float, max_iter: int = 0):
+    """fieldsolve::solve on the GPU; mask/fixed of shape (nz, ny, nx). Returns (phi, stats dict)."""
+    nz, ny, nx = mask.shape
+    m = np.ascontiguousarray(mask, np.uint8)
+    f = np.ascontiguousarray(fixed, np.float64)
+    phi = np.zeros(m.shape, np.float64)
+    st = SolveStats()
+    rc = load().mq_laplace_solve(nx, ny, nz, m.ctypes.data_as(C.POINTER(C.c_uint8)), _dptr(f), outer_bc, tol,
+                                 max_iter, _dptr(phi), C.byref(st))
+    stats = {k: getattr(st, k) for k, _ in st._fields_}
+    if rc not in (0, -5):
+        _check(rc)
+    return phi, stats
+
+
+def efield(phi: np.ndarray, h: float) -> np.ndarray:
+    """E = -grad(phi) (channel.rs:168-184) on the GPU; returns shape (nz, ny, nx, 3)."""
+    nz, ny, nx = phi.shape
+    p = np.ascontiguousarray(phi, np.float64)
+    e = np.zeros(p.shape + (3,), np.float64)
+    _check(load().mq_efield(_dptr(p), nx, ny, nz, h, _dptr(e)))
+    return e
+
+
+def field_cache_name(g: Geom, basis: int) -> str:
+    buf = C.create_string_buffer(64)
+    _check(load().mq_field_cache_name(C.byref(g), basis, buf, len(buf)))
+    return buf.value.decode()
+
+
+def channel_grid(g: Geom, tol: float = 1e-7, cache_dir: str | None = None, potentials: bool = False) -> dict:
+    """Fields::build (channel.rs:204-217) on the GPU: a grid dict for Engine(), as synth.py makes."""
+    nx, ny, nz, lay = g.dims()
+    n = nx * ny * nz
+    grid = {"rf": np.zeros(3 * n), "ac": np.zeros(3 * n), "dc": [np.zeros(3 * n) for _ in range(4)],
+            "mask": np.zeros(n, np.uint8)}
+    out = ChannelFields()
+    out.rf, out.ac = _dptr(grid["rf"]), _dptr(grid["ac"])
+    for b in range(4):
+        out.dc[b] = _dptr(grid["dc"][b])
+    phis = [np.zeros(n) for _ in range(6)] if potentials else []
+    for b, p in enumerate(phis):
+        out.phi[b] = _dptr(p)
+    out.mask = grid["mask"].ctypes.data_as(C.POINTER(C.c_uint8))
+    _check(load().mq_fields_build(C.byref(g), tol, cache_dir.encode() if cache_dir else None, C.byref(out)))
+    f = Fields()
+    _check(load().mq_geom_fields_params(C.byref(g), C.byref(f)))
+    for k in ("nx", "ny", "nz", "h", "axis_x", "axis_y", "z_inject", "z_return", "z_exit", "z_return_legacy",
+              "z_exit_legacy", "crash_z0_legacy", "crash_z1_legacy"):
+        grid[k] = getattr(f, k)
+    grid["stats"] = [{k: getattr(out.stats[b], k) for k, _ in SolveStats._fields_} for b in range(6)]
+    grid["layers_mm"] = lay
+    if potentials:
+        grid["phi"] = phis
+    return grid
 
 
 # ------------------------------------------------------------------ scan API

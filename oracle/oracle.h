@@ -34,6 +34,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include "../include/mq_engine.h"
+#include "../include/mq_fieldsolve.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -77,6 +78,29 @@ int or_grad_reference(const mq_fields* f, const mq_job* job, const double dc[4],
 
 /* Trilinear sample of one node-major FP64 basis at grid coordinates g. */
 void or_sample(const double* F, int nx, int ny, int nz, const double g[3], int nearest, double out[3]);
+
+/* ---- basis-field pipeline (oracle_fields.c) ---- */
+
+/* Geom::default, layers(), nz() (channel.rs:38-57). */
+void or_geom_default(mq_geom* g);
+void or_geom_layers(const mq_geom* g, double l[8]);
+int or_geom_nz(const mq_geom* g);
+
+/* Geom::classify (channel.rs:59-103): MQ_ELEC_* code of node (i, j, k), and
+ * of every node of the n_xy x n_xy x nz grid. */
+int or_geom_classify_node(const mq_geom* g, int i, int j, int k);
+void or_geom_classify(const mq_geom* g, uint8_t* codes);
+
+/* basis_value (channel.rs:110-119) of an MQ_ELEC_* code. */
+double or_basis_value(int basis, int code, int ac_mode);
+
+/* The 7-point Laplace problem of fieldsolve::solve by red-black SOR (small
+ * grids): returns 0 when the scaled max-norm residual is <= tol. */
+int or_laplace_sor(int nx, int ny, int nz, const uint8_t* mask, const double* fixed, double outer_bc, double tol,
+                   uint32_t max_iter, double* phi, uint32_t* iters, double* residual);
+
+/* efield (channel.rs:168-184): e[3N] = -grad(phi). */
+void or_efield(const double* phi, int nx, int ny, int nz, double h, double* e);
 
 #ifdef __cplusplus
 }

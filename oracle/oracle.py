@@ -56,6 +56,18 @@ def load() -> C.CDLL:
                                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.or_sample.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int,
                               C.POINTER(C.c_double)]
+    lib.or_geom_default.argtypes = [C.POINTER(mqhip.Geom)]
+    lib.or_geom_nz.argtypes = [C.POINTER(mqhip.Geom)]
+    lib.or_geom_nz.restype = C.c_int
+    lib.or_geom_classify_node.argtypes = [C.POINTER(mqhip.Geom), C.c_int, C.c_int, C.c_int]
+    lib.or_geom_classify_node.restype = C.c_int
+    lib.or_geom_classify.argtypes = [C.POINTER(mqhip.Geom), C.POINTER(C.c_uint8)]
+    lib.or_basis_value.argtypes = [C.c_int, C.c_int, C.c_int]
+    lib.or_basis_value.restype = C.c_double
+    lib.or_laplace_sor.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8), C.POINTER(C.c_double),
+                                   C.c_double, C.c_double, C.c_uint32, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
+    lib.or_efield.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double)]
     _lib = lib
     return lib
 
@@ -158,3 +170,56 @@ def grad_reference(grid: dict, job, dc, ph, obj, nthreads: int = 8):
         raise RuntimeError("oracle gradient failed")
     del keep
     return J.value, np.array(list(g)), gs.reshape(6, n)
+
+
+# ---------------------------------------------------------------- field pipeline
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def geom_default(**kw):
+    g = mqhip.Geom()
+    load().or_geom_default(C.byref(g))
+    for k, v in kw.items():
+        setattr(g, k, v)
+    return g
+
+
+def geom_classify(g) -> np.ndarray:
+    """Geom::classify restated on the CPU: codes of shape (nz, ny, nx)."""
+    nz = load().or_geom_nz(C.byref(g))
+    out = np.zeros(g.n_xy * g.n_xy * nz, np.uint8)
+    load().or_geom_classify(C.byref(g), out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out.reshape(nz, g.n_xy, g.n_xy)
+
+
+def classify_node(g, i, j, k) -> int:
+    return load().or_geom_classify_node(C.byref(g), i, j, k)
+
+
+def basis_values(codes: np.ndarray, basis: int, ac_mode: int) -> np.ndarray:
+    lut = np.array([load().or_basis_value(basis, c, ac_mode) for c in range(23)])
+    return lut[codes]
+
+
+def laplace_sor(mask, fixed, outer_bc, tol, max_iter=200000):
+    nz, ny, nx = mask.shape
+    m = np.ascontiguousarray(mask, np.uint8)
+    f = np.ascontiguousarray(fixed, np.float64)
+    phi = np.zeros(m.shape)
+    it = C.c_uint32()
+    res = C.c_double()
+    rc = load().or_laplace_sor(nx, ny, nz, m.ctypes.data_as(C.POINTER(C.c_uint8)), _dp(f), outer_bc, tol, max_iter,
+                               _dp(phi), C.byref(it), C.byref(res))
+    if rc < 0:
+        raise RuntimeError("oracle SOR failed")
+    return phi, {"iterations": it.value, "residual": res.value, "converged": rc == 0}
+
+
+def efield(phi, h):
+    nz, ny, nx = phi.shape
+    p = np.ascontiguousarray(phi, np.float64)
+    e = np.zeros(p.shape + (3,))
+    load().or_efield(_dp(p), nx, ny, nz, h, _dp(e))
+    return e
