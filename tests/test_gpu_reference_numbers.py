@@ -1,0 +1,120 @@
+"""The whole path on the reference's own channel geometry against numbers the
+reference itself recorded: fields built by the GPU field pipeline
+(Geom::default / the corrected S14 geometry, channel.rs:38-43, 605-624),
+then the integrator.
+
+* REF_CHSIM_GPU (mqviz/src/validate.rs:110-112; tests/golden/chsim_gpu_s14.json):
+  the reference's Metal engine at the S14 calibrated point, 100,000 ions.
+  Every outcome class within 3 sigma (binomial errors of both samples).
+* docs/mass-scan.csv (tests/golden/mass-scan.csv): the legacy Fig. 4 scan.
+  The published columns come from the Java-exported potentials, not
+  reproducible here; on converged fields the peak must sit at m/z 497-501
+  with a height within 3 sigma of 742/1000, both ends of the scan must be
+  empty, and with the full kick no ion may be transmitted (0 of 25,000, the
+  rust_full_kick column).
+* Both paths: the FP32 batched and FP64 plain forms agree within 3 sigma per m/z.
+"""
+import csv
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import mqhip
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def pull(a, na, b, nb):
+    pa, pb = a / na, b / nb
+    se = np.sqrt(pa * (1 - pa) / na + pb * (1 - pb) / nb)
+    return 0.0 if se == 0 else (pa - pb) / se
+
+
+@pytest.fixture(scope="module")
+def s14_engine():
+    say("[gpu] building the corrected S14 channel fields (160x160x242 @ 0.1 mm)")
+    grid = mqhip.channel_grid(mqhip.Geom.s14(), tol=1e-7)
+    e = mqhip.Engine(grid)
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def default_engine():
+    say("[gpu] building the default channel fields (160x160x242 @ 0.1 mm)")
+    grid = mqhip.channel_grid(mqhip.Geom.default(), tol=1e-7)
+    e = mqhip.Engine(grid)
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("path", [mqhip.PATH_BATCHED, mqhip.PATH_PLAIN])
+def test_s14_point_matches_ref_chsim_gpu(s14_engine, path):
+    ref = json.load(open(os.path.join(HERE, "golden", "chsim_gpu_s14.json")))
+    s = ref["setting"]
+    ph = mqhip.Physics.correct(0.1).copy(t_max=s["t_max_s"])
+    j = mqhip.job(s["mass_da"], float(s["z"]), s["ccs_n2"], s["v_rf"], s["f_rf"], s["v_ac"], s["f_ac"], ref["n"],
+                  s["seed"])
+    say(f"[gpu] S14 point, path {path}, {ref['n']} ions")
+    o = s14_engine.run_batch([j], s["dc"], ph, path)[0]
+    n = ref["n"]
+    got = {"transmitted": o.transmitted, "crashed_plus_left": o.crashed + o.left, "returned": o.returned,
+           "timeout": o.timeout}
+    pulls = {k: pull(got[k], n, ref[k], n) for k in got}
+    say(f"   got {got}, pulls {pulls}")
+    assert o.n == n
+    assert all(abs(p) <= 3.0 for p in pulls.values()), pulls
+
+
+def _fig4_jobs(full_kick=False):
+    mzs = [475.0 + 2 * i for i in range(25)]
+    return mzs, [mqhip.job(mz * 5, 5.0, 0.0, 100.0, 500e3, 0.5, 100e3, 1000, 1000 + int(mz)) for mz in mzs]
+
+
+@pytest.fixture(scope="module")
+def fig4(default_engine):
+    mzs, jobs = _fig4_jobs()
+    ph = mqhip.Physics.legacy(1e-3 * 133.322)
+    out = {}
+    for path in (mqhip.PATH_BATCHED, mqhip.PATH_PLAIN):
+        say(f"[gpu] legacy Fig. 4 scan on converged fields, path {path}")
+        out[path] = [o.transmitted for o in default_engine.run_batch(jobs, [0, 5, -5, 0], ph, path)]
+    return mzs, out
+
+
+def test_fig4_legacy_peak_on_converged_fields(fig4):
+    mzs, out = fig4
+    rows = list(csv.DictReader(open(os.path.join(HERE, "golden", "mass-scan.csv"))))
+    published = {float(r["mz"]): int(r["rust_legacy"]) for r in rows}
+    assert [float(r["mz"]) for r in rows] == mzs
+    for path, t in out.items():
+        say(f"   path {path}: {t}")
+        k = int(np.argmax(t))
+        assert 497 <= mzs[k] <= 501, mzs[k]
+        assert abs(pull(max(t), 1000, published[499.0], 1000)) <= 3.0, (max(t), published[499.0])
+        # both ends empty, as in the published columns (at most a few ions)
+        assert sum(t[:5]) <= 3 and sum(t[-3:]) <= 10
+
+
+def test_fig4_batched_and_plain_agree(fig4):
+    mzs, out = fig4
+    for a, b, mz in zip(out[mqhip.PATH_BATCHED], out[mqhip.PATH_PLAIN], mzs):
+        assert abs(pull(a, 1000, b, 1000)) <= 3.0 or abs(a - b) <= 3, (mz, a, b)
+
+
+def test_fig4_full_kick_transmits_nothing(default_engine):
+    mzs, jobs = _fig4_jobs()
+    ph = mqhip.Physics.legacy(1e-3 * 133.322).copy(legacy_force=0)
+    say("[gpu] legacy Fig. 4 scan with the full kick")
+    out = default_engine.run_batch(jobs, [0, 5, -5, 0], ph, mqhip.PATH_BATCHED)
+    assert sum(o.transmitted for o in out) == 0
+    assert sum(o.n for o in out) == 25000
