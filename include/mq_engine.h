@@ -179,8 +179,8 @@ int mq_set_tune(mq_engine* eng, const mq_tune* t);
 int mq_get_tune(const mq_engine* eng, mq_tune* out);
 
 /* Autotuner of the batched path (the reference tunes by environment
- * variables, tiled.rs:24-28): times candidate tunings (re-sort interval,
- * waves per CU, re-sort on/off) on an evenly spaced subset of the jobs of
+ * variables, tiled.rs:24-28): times candidate tunings (re-sort on/off,
+ * chunk, waves per CU, collision batch) on an evenly spaced subset of the jobs of
  * about sample_ions ions (0: 262144), sets the fastest on the engine and
  * appends it to the TSV file cache_path (NULL: no file) keyed by grid and
  * physics switches; a later call with the same key reads it back instead of
