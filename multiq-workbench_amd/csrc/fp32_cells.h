@@ -1,0 +1,43 @@
+// fp32_cells.h -- FP32 cell-record access shared by the batched and tandem
+// workers: record load into VGPRs, polynomial evaluation, hardware
+// transcendental functions in revolutions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mq_types.h"
+
+namespace mq {
+
+__device__ __forceinline__ float sin_rev(float p) { return __builtin_amdgcn_sinf(p); }
+__device__ __forceinline__ float cos_rev(float p) { return __builtin_amdgcn_cosf(p); }
+__device__ __forceinline__ float log_f(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945f; }
+
+__device__ __forceinline__ void load_cell(const float4* __restrict__ cells, int32_t c, float4 R[CELL_F4])
+{
+    const float4* src = cells + (size_t)c * CELL_F4;
+#pragma unroll
+    for (int q = 0; q < CELL_F4; ++q) R[q] = src[q];
+}
+
+__device__ __forceinline__ float rget(const float4 R[CELL_F4], int k)
+{
+    const float4 v = R[k >> 2];
+    return (k & 3) == 0 ? v.x : ((k & 3) == 1 ? v.y : ((k & 3) == 2 ? v.z : v.w));
+}
+
+__device__ __forceinline__ float poly(const float4 R[CELL_F4], int base, float tx, float ty, float tz, float pxy,
+                                      float pyz, float pxz, float pxyz)
+{
+    float f = rget(R, base);
+    f = fmaf(tx, rget(R, base + 3), f);
+    f = fmaf(ty, rget(R, base + 6), f);
+    f = fmaf(tz, rget(R, base + 9), f);
+    f = fmaf(pxy, rget(R, base + 12), f);
+    f = fmaf(pyz, rget(R, base + 15), f);
+    f = fmaf(pxz, rget(R, base + 18), f);
+    f = fmaf(pxyz, rget(R, base + 21), f);
+    return f;
+}
+
+}  // namespace mq

@@ -8,6 +8,7 @@
 // Built with -ffp-contract=off; fused multiply-adds are explicit fma().
 #include <hip/hip_runtime.h>
 
+#include "fp64_sample.h"
 #include "kernels.h"
 #include "mq_math.h"
 #include "mq_types.h"
@@ -27,7 +28,6 @@ __device__ __forceinline__ uint32_t job_of(const JobP* jobs, uint32_t njobs, uin
     return lo;
 }
 
-__device__ __forceinline__ double norm3(double a, double b, double c) { return sqrt((a * a + b * b) + c * c); }
 
 // ---------------------------------------------------------------- inject
 // Engine::inject (channel.rs:342-374) with the Philox stream: three blocks
@@ -113,33 +113,6 @@ __global__ void k_combine_dc(double* __restrict__ out, const double* d0, const d
     if (d2 && v2 != 0.0) e = e + v2 * d2[i];
     if (d3 && v3 != 0.0) e = e + v3 * d3[i];
     out[i] = e;
-}
-
-// ------------------------------------------------------ FP64 sampling
-__device__ __forceinline__ void loc_d(double g, int n, int& i, double& t, int nearest)
-{
-    double gc = g;
-    if (!(gc >= 0.0)) gc = 0.0; /* also maps NaN to 0 */
-    if (gc > (double)(n - 1)) gc = (double)(n - 1);
-    int ii = (int)gc;
-    if (ii > n - 2) ii = n - 2;
-    i = ii;
-    t = nearest ? 0.0 : gc - (double)ii;
-}
-
-// Trilinear interpolation with the corner order and lerp chain of `interp`
-// (channel.metal:63-68): x first, then y, then z.
-__device__ __forceinline__ void sample_d(const double* __restrict__ F, uint32_t b, uint32_t sy, uint32_t sz,
-                                         double tx, double ty, double tz, double out[3])
-{
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        double c00 = lerp_d(F[3 * b + d], F[3 * (b + 1) + d], tx);
-        double c10 = lerp_d(F[3 * (b + sy) + d], F[3 * (b + 1 + sy) + d], tx);
-        double c01 = lerp_d(F[3 * (b + sz) + d], F[3 * (b + 1 + sz) + d], tx);
-        double c11 = lerp_d(F[3 * (b + sy + sz) + d], F[3 * (b + 1 + sy + sz) + d], tx);
-        out[d] = lerp_d(lerp_d(c00, c10, ty), lerp_d(c01, c11, ty), tz);
-    }
 }
 
 // ---------------------------------------------------- FP64 plain step

@@ -1,0 +1,493 @@
+// kernels_tandem.hip -- tandem multi-domain batch: one clock, a splitter,
+// nine channel ports and a combiner (include/mq_tandem.h).
+//
+// Reference: trapsim Model::advance, enter, leave
+// (simulation/trapsim/src/lib.rs:176-385). Per step an ion samples the grid
+// of its region (cube for splitter/combiner, channel for ports), takes chsim's
+// kick-drift step and thermal collision, then is classified in its region;
+// a crossing moves it to the next region with the coordinate map of
+// enter/leave and counts the crossing in per-port u64 counters (integer
+// atomics: exact and order-independent).
+//
+//  * k_tandem_plain: FP64, one thread per ion, relaunched every nsteps
+//    (the plain path; bit-identical to the oracle).
+//  * k_tandem_batched: FP32 fields and push with FP64 position and clock, in
+//    persistent wavefronts with lane refill and carry, as k_batched; the cell
+//    record is reloaded from the new region's table on a crossing.
+#include <hip/hip_runtime.h>
+
+#include "fp32_cells.h"
+#include "fp64_sample.h"
+#include "kernels.h"
+#include "mq_math.h"
+#include "mq_types.h"
+
+namespace mq {
+
+// a[k] without a dynamically indexed load (which would copy the parameter
+// block to scratch): an unrolled select chain.
+__device__ __forceinline__ double sel9(const double (&a)[9], uint32_t k)
+{
+    double r = a[0];
+#pragma unroll
+    for (uint32_t j = 1; j < 9; ++j) r = (k == j) ? a[j] : r;
+    return r;
+}
+
+// Classification and region crossing after a step (lib.rs:206-265 with
+// enter 326-356 and leave 357-377). `crash` tests the conductor mask of a
+// grid at a position inside that grid's box. Returns the new status
+// (0 alive, 1 lost, 2 back, 3 detected).
+template <class V, class Crash>
+__device__ __forceinline__ int tandem_cross(const TandemP& T, const TGrid& cube, const TGrid& port, uint32_t& reg,
+                                            uint32_t& last, double& x, double& y, double& z, V& vz, uint32_t job,
+                                            unsigned long long* __restrict__ ctr, Crash crash)
+{
+    if (reg == REG_SPLIT || reg == REG_COMBINE) {
+        if (x < 0.0 || y < 0.0 || z < 0.0 || x > cube.lim0 || y > cube.lim1 || z > cube.lim2 || crash(false))
+            return 1;
+        if (z > T.seam) {
+            uint32_t p = 0;
+            double best = 0.0, mx = 0.0, my = 0.0;
+#pragma unroll
+            for (uint32_t k = 0; k < 9; ++k) {
+                const double dx = x - T.mx[k], dy = y - T.my[k];
+                const double d2 = dx * dx + dy * dy;
+                if (k == 0 || d2 < best) { best = d2; p = k; mx = T.mx[k]; my = T.my[k]; }
+            }
+            if (!(best < T.ap_r2)) return 1;  // on the board between the mouths
+            unsigned long long* c = ctr + (size_t)p * TC_PORT;
+            x = x + (T.ax_p - mx);
+            y = y + (T.ay_p - my);
+            if (reg == REG_SPLIT) {
+                z = T.z_ret + (z - T.seam);
+                atomicAdd(&c[TC_ENTERED], 1ull);
+                if (last != NO_PORT) {
+                    atomicAdd(&c[TC_REVISITS], 1ull);
+                    atomicAdd(&c[TC_RETRY + last], 1ull);
+                    if (last != p) atomicAdd(&c[TC_FROM_OTHER], 1ull);
+                }
+            } else {
+                z = T.z_exit - (z - T.seam);
+                vz = -vz;
+                atomicAdd(&c[TC_REVERSE], 1ull);
+            }
+            reg = 1u + p;
+            return 0;
+        }
+        if (z < T.z_back) return reg == REG_SPLIT ? 2 : 3;
+        return 0;
+    }
+    const uint32_t p = reg - 1u;
+    unsigned long long* c = ctr + (size_t)p * TC_PORT;
+    const double mx = sel9(T.mx, p), my = sel9(T.my, p);
+    if (x < 0.0 || y < 0.0 || z < 0.0 || x > port.lim0 || y > port.lim1 || z > port.lim2 || crash(true)) {
+        atomicAdd(&c[TC_LOST], 1ull);
+        return 1;
+    }
+    if (z < T.z_ret) {
+        x = x + (mx - T.ax_p);
+        y = y + (my - T.ay_p);
+        z = T.seam + (z - T.z_ret);
+        last = p;
+        reg = REG_SPLIT;
+        atomicAdd(&c[TC_RETURNED], 1ull);
+    } else if (z > T.z_exit) {
+        x = x + (mx - T.ax_p);
+        y = y + (my - T.ay_p);
+        z = T.seam - (z - T.z_exit);
+        vz = -vz;
+        last = NO_PORT;
+        reg = REG_COMBINE;
+        atomicAdd(&c[TC_FORWARDED], 1ull);
+        if (job < (uint32_t)TC_MAX_JOBS) atomicAdd(&c[TC_BYJOB + job], 1ull);
+    }
+    return 0;
+}
+
+// Excitation of a region in cycles per step (0 for the cube and idle ports).
+__device__ __forceinline__ double w_ac_of(const TandemP& T, uint32_t reg)
+{
+    return (reg == REG_SPLIT || reg == REG_COMBINE) ? 0.0 : sel9(T.w_ac, reg - 1u);
+}
+
+__device__ __forceinline__ bool port_driven(const TandemP& T, uint32_t reg) { return w_ac_of(T, reg) > 0.0; }
+
+// --------------------------------------------------------------- FP64 plain
+struct TPlainArgs {
+    IonState S;
+    const JobP* jobs;
+    TandemP T;
+    TGrid cube, port;
+    const double* dc[4];  // combined DC: splitter, combiner, driven port, idle port
+    unsigned long long* ctr;
+    uint32_t n, nsteps;
+    uint32_t* live;
+};
+
+__global__ __launch_bounds__(256) void k_tandem_plain(TPlainArgs A)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const IonState& S = A.S;
+    const TandemP& T = A.T;
+    bool alive = false;
+    if (i < A.n && S.status[i] == 0) {
+        const uint32_t jk = S.job[i];
+        const JobP J = A.jobs[jk];
+        const uint32_t ion = (uint32_t)(i - J.off);
+        double x = S.x[i], y = S.y[i], z = S.z[i], vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
+        uint32_t st = S.steps[i], ncoll = S.ncoll[i];
+        uint32_t reg = S.region[i], last = S.last[i];
+        const double dt = T.dt;
+        int status = 0;
+        u32x4 cw = {0, 0, 0, 0};
+        for (uint32_t s = 0; s < A.nsteps; ++s) {
+            const bool cube = reg == REG_SPLIT || reg == REG_COMBINE;
+            const TGrid& G = cube ? A.cube : A.port;
+            const double* dcp = reg == REG_SPLIT ? A.dc[0] : reg == REG_COMBINE ? A.dc[1]
+                              : port_driven(T, reg) ? A.dc[2] : A.dc[3];
+            int ii, jj, kk;
+            double tx, ty, tz;
+            loc_d(x * G.inv_h, G.nx, ii, tx, (int)T.nearest);
+            loc_d(y * G.inv_h, G.ny, jj, ty, (int)T.nearest);
+            loc_d(z * G.inv_h, G.nz, kk, tz, (int)T.nearest);
+            const uint32_t sy = (uint32_t)G.nx, sz = (uint32_t)G.nx * (uint32_t)G.ny;
+            const uint32_t b = (uint32_t)ii + sy * ((uint32_t)jj + (uint32_t)G.ny * (uint32_t)kk);
+            double er[3], ed[3], ea[3] = {0.0, 0.0, 0.0};
+            sample_d(G.rf, b, sy, sz, tx, ty, tz, er);
+            sample_d(dcp, b, sy, sz, tx, ty, tz, ed);
+            const double a_rf = T.v_rf * sin2pi(frac_d(T.w_rf * (double)st));
+            double a_ac = 0.0;
+            if (!cube) {
+                sample_d(G.ac, b, sy, sz, tx, ty, tz, ea);
+                const double wa = w_ac_of(T, reg);
+                if (wa > 0.0) a_ac = T.v_ac * sin2pi(frac_d(wa * (double)st));
+            }
+            double E;
+            E = (a_rf * er[0] + ed[0]) + a_ac * ea[0]; vx = fma(J.kq, E, vx); x = fma(vx, dt, x);
+            E = (a_rf * er[1] + ed[1]) + a_ac * ea[1]; vy = fma(J.kq, E, vy); y = fma(vy, dt, y);
+            E = (a_rf * er[2] + ed[2]) + a_ac * ea[2]; vz = fma(J.kq, E, vz); z = fma(vz, dt, z);
+            const uint32_t cur = st;
+            st += 1;
+            // thermal bath (channel.metal:93-99 in the form of k_plain)
+            if (J.nsig_dt > 0.0) {
+                const double bound = norm3(vx, vy, vz) + J.v_bound;
+                if ((cur & 3) == 0 || s == 0) cw = philox(ion, cur >> 2, 0, D_CAND, J.key0, J.key1);
+                if (u01(pick(cw, cur & 3)) < J.nsig_dt * bound) {
+                    u32x4 w = philox(ion, cur, 0, D_COLL, J.key0, J.key1);
+                    double r1 = sqrt(-2.0 * log_d(u01(w.a))), r2 = sqrt(-2.0 * log_d(u01(w.c)));
+                    double s1, c1, s2, c2;
+                    sincos2pi(u01(w.b), s1, c1);
+                    sincos2pi(u01(w.d), s2, c2);
+                    const double gx = vx - J.s_gas * (r1 * c1), gy = vy - J.s_gas * (r1 * s1),
+                                 gz = vz - J.s_gas * (r2 * c2);
+                    const double gl = norm3(gx, gy, gz);
+                    u32x4 w2 = philox(ion, cur, 1, D_COLL, J.key0, J.key1);
+                    if (u01(w2.a) * bound < gl) {
+                        double c = 2.0 * u01(w2.b) - 1.0, sp, cp;
+                        sincos2pi(u01(w2.c), sp, cp);
+                        double sn = sqrt(fmax(0.0, 1.0 - c * c));
+                        double mg = J.mr * gl;
+                        vx = (vx - J.mr * gx) + mg * (sn * cp);
+                        vy = (vy - J.mr * gy) + mg * (sn * sp);
+                        vz = (vz - J.mr * gz) + mg * c;
+                        ncoll += 1;
+                    }
+                }
+            }
+            auto crash = [&](bool in_port) {
+                const TGrid& H = in_port ? A.port : A.cube;
+                const uint32_t hy = (uint32_t)H.nx;
+                return H.mask[(uint32_t)rint(x * H.inv_h) +
+                              hy * ((uint32_t)rint(y * H.inv_h) + (uint32_t)H.ny * (uint32_t)rint(z * H.inv_h))] != 0;
+            };
+            status = tandem_cross(T, A.cube, A.port, reg, last, x, y, z, vz, jk, A.ctr, crash);
+            if (status == 0 && st >= T.max_steps) status = 5;
+            if (status) break;
+        }
+        S.x[i] = x; S.y[i] = y; S.z[i] = z; S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
+        S.steps[i] = st; S.ncoll[i] = ncoll; S.status[i] = (uint8_t)status;
+        S.region[i] = (uint8_t)reg; S.last[i] = (uint8_t)last;
+        alive = status == 0;
+    }
+    const uint64_t m = __ballot(alive);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(A.live, (uint32_t)__popcll(m));
+}
+
+// ------------------------------------------------------------- FP32 batched
+struct TLane {
+    double x, y, z;
+    float vx, vy, vz;
+    float kq, nsig, sgas, vb, mr;
+    uint32_t st, ncoll, k0, k1, ion, id, job, visit, reg, last;
+    float tx, ty, tz, fx, fy, fz, a_rf, a_ac;
+    uint32_t cw1, cw2, cw3;
+    int32_t cell, tab;
+};
+
+struct TBatchArgs {
+    IonState S;
+    const JobP* jobs;
+    TandemP T;
+    TGrid cube, port;
+    const float4* cells[4];  // splitter, combiner, driven port, idle port
+    unsigned long long* ctr;
+    const uint32_t* list_in;
+    uint32_t* list_out;
+    uint32_t* ctr_in;
+    uint32_t* ctr_out;
+    uint32_t chunk, period;
+};
+
+__device__ __forceinline__ int32_t tab_of(const TandemP& T, uint32_t reg)
+{
+    return reg == REG_SPLIT ? 0 : reg == REG_COMBINE ? 1 : (port_driven(T, reg) ? 2 : 3);
+}
+
+__device__ __forceinline__ int32_t tlocate(const TGrid& cube, const TGrid& port, uint32_t nearest, TLane& L)
+{
+    const bool in_cube = L.reg == REG_SPLIT || L.reg == REG_COMBINE;
+    const double inv_h = in_cube ? cube.inv_h : port.inv_h;
+    const int nx = in_cube ? cube.nx : port.nx, ny = in_cube ? cube.ny : port.ny, nz = in_cube ? cube.nz : port.nz;
+    const double gx = fmin(fmax(L.x * inv_h, 0.0), (double)(nx - 1));
+    const double gy = fmin(fmax(L.y * inv_h, 0.0), (double)(ny - 1));
+    const double gz = fmin(fmax(L.z * inv_h, 0.0), (double)(nz - 1));
+    const int i = min((int)gx, nx - 2), j = min((int)gy, ny - 2), k = min((int)gz, nz - 2);
+    L.fx = (float)(gx - (double)i);
+    L.fy = (float)(gy - (double)j);
+    L.fz = (float)(gz - (double)k);
+    L.tx = nearest ? 0.0f : L.fx;
+    L.ty = nearest ? 0.0f : L.fy;
+    L.tz = nearest ? 0.0f : L.fz;
+    return i + (nx - 1) * (j + (ny - 1) * k);
+}
+
+// The four cell tables: splitter, combiner, driven port, idle port.
+struct Tables {
+    const float4 *c0, *c1, *c2, *c3;
+};
+
+// Locate in the lane's region, reload the record on a cell or table change.
+__device__ __forceinline__ void tfetch(const TandemP& T, const TGrid& cube, const TGrid& port, const Tables& C,
+                                       TLane& L, float4 R[CELL_F4])
+{
+    const int32_t t = tab_of(T, L.reg);
+    const int32_t c = tlocate(cube, port, T.nearest, L);
+    if (c != L.cell || t != L.tab) {
+        L.cell = c;
+        L.tab = t;
+        load_cell(t == 0 ? C.c0 : t == 1 ? C.c1 : t == 2 ? C.c2 : C.c3, c, R);
+    }
+}
+
+__device__ __forceinline__ void tphases(const TandemP& T, TLane& L)
+{
+    double pr = T.w_rf * (double)L.st;
+    pr = pr - floor(pr);
+    L.a_rf = (float)T.v_rf * sin_rev((float)pr);
+    L.a_ac = 0.0f;
+    const double wa = w_ac_of(T, L.reg);
+    if (wa > 0.0) {
+        double pa = wa * (double)L.st;
+        pa = pa - floor(pa);
+        L.a_ac = (float)T.v_ac * sin_rev((float)pa);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tandem_batched(TBatchArgs A)
+{
+    const IonState& S = A.S;
+    const TandemP& T = A.T;
+    const TGrid& cube = A.cube;
+    const TGrid& port = A.port;
+    const Tables C = {A.cells[0], A.cells[1], A.cells[2], A.cells[3]};
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t count = A.ctr_in[CTR_COUNT];
+    const double dt = T.dt;
+    TLane L;
+    float4 R[CELL_F4];
+    bool have = false, drained = false;
+    L.cell = -1;
+    L.tab = -1;
+    for (;;) {
+        const uint64_t idle = __ballot(!have);
+        if (idle && !drained) {
+            const uint32_t nidle = (uint32_t)__popcll(idle);
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&A.ctr_in[0], nidle);
+            base = __shfl(base, (int)leader);
+            if (base >= count) {
+                drained = true;
+            } else if (!have) {
+                const uint32_t slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (slot < count) {
+                    const uint32_t id = A.list_in[slot];
+                    const uint32_t jk = S.job[id];
+                    const JobP& J = A.jobs[jk];
+                    L.id = id;
+                    L.job = jk;
+                    L.ion = (uint32_t)(id - J.off);
+                    L.x = S.x[id]; L.y = S.y[id]; L.z = S.z[id];
+                    L.vx = S.fvx[id]; L.vy = S.fvy[id]; L.vz = S.fvz[id];
+                    L.st = S.steps[id]; L.ncoll = S.ncoll[id];
+                    L.reg = S.region[id]; L.last = S.last[id];
+                    L.kq = (float)J.kq; L.nsig = (float)J.nsig_dt; L.sgas = (float)J.s_gas;
+                    L.vb = (float)J.v_bound; L.mr = (float)J.mr;
+                    L.k0 = J.key0; L.k1 = J.key1;
+                    L.visit = 0;
+                    L.cw1 = L.cw2 = L.cw3 = 0;
+                    tfetch(T, cube, port, C, L, R);
+                    tphases(T, L);
+                    have = true;
+                }
+            }
+        }
+        if (!__any(have)) break;
+        bool stop = false;
+        int status = 0;
+        for (uint32_t r = 0; r < A.period; ++r) {
+            if (have && !stop) {
+                const float pxy = L.tx * L.ty, pyz = L.ty * L.tz, pxz = L.tx * L.tz, pxyz = pxy * L.tz;
+                float e[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const float er = poly(R, d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+                    const float ed = poly(R, 24 + d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+                    const float ea = poly(R, 48 + d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+                    e[d] = fmaf(L.a_ac, ea, fmaf(L.a_rf, er, ed));
+                }
+                L.vx = fmaf(L.kq, e[0], L.vx);
+                L.vy = fmaf(L.kq, e[1], L.vy);
+                L.vz = fmaf(L.kq, e[2], L.vz);
+                L.x = fma((double)L.vx, dt, L.x);
+                L.y = fma((double)L.vy, dt, L.y);
+                L.z = fma((double)L.vz, dt, L.z);
+                const uint32_t cur = L.st;
+                L.st += 1;
+                tfetch(T, cube, port, C, L, R);  // record at the new position (same region): mask bits and next kick
+                if (L.nsig > 0.0f) {
+                    const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
+                    uint32_t cw;
+                    const uint32_t ph = cur & 3;
+                    if (ph == 0 || L.visit == 0) {
+                        const u32x4 w = philox(L.ion, cur >> 2, 0, D_CAND, L.k0, L.k1);
+                        cw = pick(w, ph);
+                        L.cw1 = w.b; L.cw2 = w.c; L.cw3 = w.d;
+                    } else {
+                        cw = ph == 1 ? L.cw1 : (ph == 2 ? L.cw2 : L.cw3);
+                    }
+                    if (u01f(cw) < L.nsig * bound) {
+                        const u32x4 w = philox(L.ion, cur, 0, D_COLL, L.k0, L.k1);
+                        const float r1 = sqrtf(-2.0f * log_f(u01f(w.a))), r2 = sqrtf(-2.0f * log_f(u01f(w.c)));
+                        const float a1 = u01f(w.b), a2 = u01f(w.d);
+                        const float gx = L.vx - L.sgas * (r1 * cos_rev(a1));
+                        const float gy = L.vy - L.sgas * (r1 * sin_rev(a1));
+                        const float gz = L.vz - L.sgas * (r2 * cos_rev(a2));
+                        const float gl = sqrtf(fmaf(gz, gz, fmaf(gy, gy, gx * gx)));
+                        const u32x4 w2 = philox(L.ion, cur, 1, D_COLL, L.k0, L.k1);
+                        if (u01f(w2.a) * bound < gl) {
+                            const float cth = 2.0f * u01f(w2.b) - 1.0f, a3 = u01f(w2.c);
+                            const float sn = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
+                            const float mg = L.mr * gl;
+                            L.vx = fmaf(mg, sn * cos_rev(a3), fmaf(-L.mr, gx, L.vx));
+                            L.vy = fmaf(mg, sn * sin_rev(a3), fmaf(-L.mr, gy, L.vy));
+                            L.vz = fmaf(mg, cth, fmaf(-L.mr, gz, L.vz));
+                            L.ncoll += 1;
+                        }
+                    }
+                }
+                L.visit += 1;
+                const uint32_t corner = (L.fx >= 0.5f ? 1u : 0u) | (L.fy >= 0.5f ? 2u : 0u) | (L.fz >= 0.5f ? 4u : 0u);
+                const bool on_conductor = ((__float_as_uint(R[18].x) >> corner) & 1u) != 0u;
+                auto crash = [on_conductor](bool) { return on_conductor; };
+                const uint32_t reg0 = L.reg;
+                status = tandem_cross(T, cube, port, L.reg, L.last, L.x, L.y, L.z, L.vz, L.job, A.ctr, crash);
+                if (status == 0 && L.reg != reg0) tfetch(T, cube, port, C, L, R);
+                if (status == 0) tphases(T, L);
+                if (status == 0 && L.st >= T.max_steps) status = 5;
+                if (status != 0 || L.visit >= A.chunk) stop = true;
+            }
+        }
+        const bool carry = have && stop && status == 0;
+        const uint64_t cm = __ballot(carry);
+        if (cm) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)cm) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&A.ctr_out[CTR_COUNT], (uint32_t)__popcll(cm));
+            base = __shfl(base, (int)leader);
+            if (carry) A.list_out[base + (uint32_t)__popcll(cm & ((1ull << lane) - 1ull))] = L.id;
+        }
+        if (have && stop) {
+            const uint32_t id = L.id;
+            S.x[id] = L.x; S.y[id] = L.y; S.z[id] = L.z;
+            S.fvx[id] = L.vx; S.fvy[id] = L.vy; S.fvz[id] = L.vz;
+            S.steps[id] = L.st; S.ncoll[id] = L.ncoll;
+            S.status[id] = (uint8_t)status;
+            S.region[id] = (uint8_t)L.reg; S.last[id] = (uint8_t)L.last;
+            have = false;
+        }
+    }
+}
+
+// Region and status of every ion at the start, and the final totals.
+__global__ void k_tandem_init(IonState S, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    S.region[i] = (uint8_t)REG_SPLIT;
+    S.last[i] = (uint8_t)NO_PORT;
+    S.status[i] = 0;
+}
+
+__global__ void k_tandem_totals(IonState S, uint32_t n, unsigned long long* __restrict__ tot)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t st = S.status[i], reg = S.region[i];
+    atomicAdd(&tot[0], 1ull);
+    int k;
+    if (st == 1) k = 4;
+    else if (st == 2) k = 5;
+    else if (st == 3) k = 6;
+    else k = reg == REG_SPLIT ? 1 : reg == REG_COMBINE ? 3 : 2;
+    atomicAdd(&tot[k], 1ull);
+    atomicAdd(&tot[7], (unsigned long long)S.steps[i]);
+}
+
+// ------------------------------------------------------------- launchers
+void launch_tandem_init(const IonState& S, uint32_t n, hipStream_t st)
+{
+    if (n) k_tandem_init<<<(n + 255) / 256, 256, 0, st>>>(S, n);
+}
+
+void launch_tandem_plain(const IonState& S, const JobP* jobs, const TandemP& T, const TGrid& cube, const TGrid& port,
+                         const double* const dc[4], unsigned long long* ctr, uint32_t n, uint32_t nsteps,
+                         uint32_t* live, hipStream_t st)
+{
+    TPlainArgs a;
+    a.S = S; a.jobs = jobs; a.T = T; a.cube = cube; a.port = port;
+    for (int k = 0; k < 4; ++k) a.dc[k] = dc[k];
+    a.ctr = ctr; a.n = n; a.nsteps = nsteps; a.live = live;
+    if (n) k_tandem_plain<<<(n + 255) / 256, 256, 0, st>>>(a);
+}
+
+void launch_tandem_batched(const IonState& S, const JobP* jobs, const TandemP& T, const TGrid& cube,
+                           const TGrid& port, const float4* const cells[4], unsigned long long* ctr,
+                           const uint32_t* list_in, uint32_t* list_out, uint32_t* ctr_in, uint32_t* ctr_out,
+                           uint32_t chunk, uint32_t period, uint32_t grid, uint32_t block, hipStream_t st)
+{
+    TBatchArgs a;
+    a.S = S; a.jobs = jobs; a.T = T; a.cube = cube; a.port = port;
+    for (int k = 0; k < 4; ++k) a.cells[k] = cells[k];
+    a.ctr = ctr; a.list_in = list_in; a.list_out = list_out; a.ctr_in = ctr_in; a.ctr_out = ctr_out;
+    a.chunk = chunk; a.period = period;
+    k_tandem_batched<<<grid, block, 0, st>>>(a);
+}
+
+void launch_tandem_totals(const IonState& S, uint32_t n, unsigned long long* tot, hipStream_t st)
+{
+    if (n) k_tandem_totals<<<(n + 255) / 256, 256, 0, st>>>(S, n, tot);
+}
+
+}  // namespace mq

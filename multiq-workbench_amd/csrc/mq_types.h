@@ -39,6 +39,7 @@ struct IonState {
     float *fvx, *fvy, *fvz;
     uint32_t *steps, *ncoll, *cd, *job;
     uint8_t *status;
+    uint8_t *region = nullptr, *last = nullptr;  // tandem batch only: domain and previous port
 };
 
 // Per-job accumulators (u64): [0] n, [1..5] crashed, returned, transmitted,
@@ -50,5 +51,33 @@ constexpr int ACC = 8;
 // RF, combined-DC and AC bases, then the conductor mask of the 8 corners
 // (bit dx + 2 dy + 4 dz), padded to 19 float4 = 304 B.
 constexpr int CELL_F4 = 19;
+
+// ---- tandem batch (kernels_tandem.hip)
+// One grid of the tandem: FP64 bases and mask (plain path), geometry.
+struct TGrid {
+    const double *rf, *ac;
+    const uint8_t* mask;
+    double inv_h, lim0, lim1, lim2;
+    int32_t nx, ny, nz, pad;
+};
+
+// Coupling geometry and drive, FP64; w_* = f * dt (cycles per step).
+struct TandemP {
+    double dt, seam, ap_r2, z_back, z_ret, z_exit, ax_p, ay_p;
+    double mx[9], my[9];
+    double v_rf, w_rf, v_ac;
+    double w_ac[9];
+    uint32_t max_steps, nearest, pad0, pad1;
+};
+
+// Per-port u64 counters: entered, returned, forwarded, revisits, from_other,
+// reverse entries, lost, retry_from[9], by_job[8].
+constexpr int TC_PORT = 24;
+constexpr int TC_ENTERED = 0, TC_RETURNED = 1, TC_FORWARDED = 2, TC_REVISITS = 3, TC_FROM_OTHER = 4,
+              TC_REVERSE = 5, TC_LOST = 6, TC_RETRY = 7, TC_BYJOB = 16;
+constexpr int TC_MAX_JOBS = 8;
+// Totals accumulated at the end: n, split, channel, combine, lost, back, detected, steps.
+constexpr int TT_N = 8;
+constexpr uint32_t REG_SPLIT = 0, REG_COMBINE = 10, NO_PORT = 255;
 
 }  // namespace mq

@@ -137,6 +137,39 @@ class ChannelFields(C.Structure):
                 ("phi", C.POINTER(C.c_double) * 6), ("mask", C.POINTER(C.c_uint8)), ("stats", SolveStats * 6)]
 
 
+class TandemGeom(C.Structure):
+    """mq_tandem_geom (include/mq_tandem.h), metres in the cube grid's frame."""
+    _fields_ = [("seam_z", C.c_double), ("mouth", (C.c_double * 2) * 9), ("ap_r", C.c_double), ("z_back", C.c_double)]
+
+
+class TandemDrive(C.Structure):
+    """mq_tandem_drive: one RF, one excitation amplitude, a tone per port (0 = not driven), DC sets."""
+    _fields_ = [("v_rf", C.c_double), ("f_rf", C.c_double), ("v_ac", C.c_double), ("tones", C.c_double * 9),
+                ("dc_split", C.c_double * 4), ("dc_combine", C.c_double * 4), ("dc_port", C.c_double * 4),
+                ("bias_idle", C.c_double * 4)]
+
+
+class PortCounts(C.Structure):
+    """mq_port_counts (Counts, trapsim lib.rs:28-39)."""
+    _fields_ = [("entered", C.c_uint64), ("returned", C.c_uint64), ("forwarded", C.c_uint64),
+                ("revisits", C.c_uint64), ("from_other", C.c_uint64), ("reverse_entries", C.c_uint64),
+                ("lost", C.c_uint64), ("retry_from", C.c_uint64 * 9), ("by_job", C.c_uint64 * 8)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_[:7]}
+        d["retry_from"] = list(self.retry_from)
+        d["by_job"] = list(self.by_job)
+        return d
+
+
+class TandemTotals(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("split", C.c_uint64), ("channel", C.c_uint64), ("combine", C.c_uint64),
+                ("lost", C.c_uint64), ("back", C.c_uint64), ("detected", C.c_uint64), ("ion_steps", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # numpy view of mq_chion (ChIon, channel.rs:292-294): 64 bytes per ion.
 CHION_DTYPE = np.dtype([("p", "<i4", 4), ("v", "<f4", 4), ("aux", "<f4", 4), ("rng", "<u4", 4)])
 POS_SCALE = 1.6e10
@@ -213,6 +246,16 @@ SIGNATURES = {
                             C.POINTER(C.c_double)]),
     "mq_fields_build": (C.c_int, [C.POINTER(Geom), C.c_double, C.c_char_p, C.POINTER(ChannelFields)]),
     "mq_field_cache_name": (C.c_int, [C.POINTER(Geom), C.c_int32, C.c_char_p, C.c_size_t]),
+    "mq_tandem_create": (C.c_int, [C.POINTER(Fields), C.POINTER(Fields), C.POINTER(TandemGeom),
+                                   C.POINTER(C.c_void_p)]),
+    "mq_tandem_destroy": (None, [C.c_void_p]),
+    "mq_tandem_run": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_size_t, C.POINTER(TandemDrive), C.POINTER(Physics),
+                                C.c_int32, C.POINTER(PortCounts), C.POINTER(TandemTotals)]),
+    "mq_tandem_run_ions": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_size_t, C.POINTER(TandemDrive),
+                                     C.POINTER(Physics), C.c_int32, C.POINTER(PortCounts), C.POINTER(TandemTotals),
+                                     C.POINTER(IonSoA), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
+                                     C.POINTER(C.c_uint32)]),
+    "mq_tandem_last_stats": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
 }
 
 _lib = None
@@ -265,7 +308,7 @@ def make_fields(grid: dict) -> tuple[Fields, list]:
               "crash_z0_legacy", "crash_z1_legacy"):
         setattr(f, k, grid[k])
     f.rf = _dptr(arr(grid["rf"], np.float64))
-    f.ac = _dptr(arr(grid["ac"], np.float64))
+    f.ac = _dptr(arr(grid["ac"], np.float64)) if grid.get("ac") is not None else C.POINTER(C.c_double)()
     for b in range(4):
         d = grid["dc"][b]
         f.dc[b] = _dptr(arr(d, np.float64)) if d is not None else C.POINTER(C.c_double)()
@@ -510,6 +553,85 @@ def channel_grid(g: Geom, tol: float = 1e-7, cache_dir: str | None = None, poten
     if potentials:
         grid["phi"] = phis
     return grid
+
+
+# ------------------------------------------------------------------ tandem batch
+
+TANDEM_REGION_SPLIT, TANDEM_REGION_COMBINE, TANDEM_NO_PORT = 0, 10, 255
+
+
+def tandem_geom(seam_z: float, mouths, ap_r: float, z_back: float) -> TandemGeom:
+    g = TandemGeom()
+    g.seam_z, g.ap_r, g.z_back = seam_z, ap_r, z_back
+    for k, (x, y) in enumerate(mouths):
+        g.mouth[k][0], g.mouth[k][1] = x, y
+    return g
+
+
+def tandem_drive(v_rf, f_rf, v_ac, tones, dc_split, dc_combine, dc_port, bias_idle) -> TandemDrive:
+    d = TandemDrive()
+    d.v_rf, d.f_rf, d.v_ac = v_rf, f_rf, v_ac
+    for k in range(9):
+        d.tones[k] = tones[k]
+    for b in range(4):
+        d.dc_split[b], d.dc_combine[b] = dc_split[b], dc_combine[b]
+        d.dc_port[b], d.bias_idle[b] = dc_port[b], bias_idle[b]
+    return d
+
+
+class Tandem:
+    """The tandem multi-domain batch (include/mq_tandem.h) over the C ABI."""
+
+    def __init__(self, cube: dict, port: dict, geom: TandemGeom):
+        fc, self._kc = make_fields(cube)
+        fp, self._kp = make_fields(port)
+        h = C.c_void_p()
+        _check(load().mq_tandem_create(C.byref(fc), C.byref(fp), C.byref(geom), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            load().mq_tandem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, jobs, drive: TandemDrive, ph: Physics, path: int = PATH_BATCHED):
+        """(per-port counts as dicts, totals dict)."""
+        ja = jobs_array(jobs)
+        pc = (PortCounts * 9)()
+        tt = TandemTotals()
+        _check(load().mq_tandem_run(self._h, ja, len(jobs), C.byref(drive), C.byref(ph), path, pc, C.byref(tt)))
+        return [p.as_dict() for p in pc], tt.as_dict()
+
+    def run_ions(self, jobs, drive: TandemDrive, ph: Physics, path: int = PATH_BATCHED):
+        """(counts, totals, final states dict with region / last / job)."""
+        ja = jobs_array(jobs)
+        n = int(sum(j.n for j in jobs))
+        o = {k: np.zeros(n, np.float64) for k in ("x", "y", "z", "vx", "vy", "vz")}
+        o["steps"] = np.zeros(n, np.uint32)
+        o["ncoll"] = np.zeros(n, np.uint32)
+        o["status"] = np.zeros(n, np.uint8)
+        o["region"] = np.zeros(n, np.uint8)
+        o["last"] = np.zeros(n, np.uint8)
+        o["job"] = np.zeros(n, np.uint32)
+        soa = soa_of(o)
+        pc = (PortCounts * 9)()
+        tt = TandemTotals()
+        u8 = C.POINTER(C.c_uint8)
+        _check(load().mq_tandem_run_ions(self._h, ja, len(jobs), C.byref(drive), C.byref(ph), path, pc, C.byref(tt),
+                                         C.byref(soa), o["region"].ctypes.data_as(u8), o["last"].ctypes.data_as(u8),
+                                         o["job"].ctypes.data_as(C.POINTER(C.c_uint32))))
+        return [p.as_dict() for p in pc], tt.as_dict(), o
+
+    def stats(self) -> dict:
+        s = RunStats()
+        _check(load().mq_tandem_last_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
 
 
 # ------------------------------------------------------------------ scan API

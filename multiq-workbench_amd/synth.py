@@ -112,3 +112,102 @@ def c3_jobs(mod, n_settings: int = 4096, n: int = 256, seed: int = 3):
         p = (0.5 + 1.5 * u[i, 4]) * 0.133322
         jobs.append(mod.job(499.0, 1.0, 230.0, v_rf, 500e3, v_exc, f_exc, n, 7000 + i, v_dc, p))
     return jobs
+
+
+# ---------------------------------------------------------------- tandem batch
+
+AMU = 1.66053906660e-27
+QE = 1.602176634e-19
+UBQ_MASS = 8565.0                       # ubiquitin, Da
+UBQ_CHARGES = (6.0, 7.0, 8.0, 9.0)      # m/z 1428.5, 1224.6, 1071.6, 952.7
+DRIVEN_PORTS = (0, 2, 6, 8)             # the four driven ports of trapsim's default tones (lib.rs:13)
+
+
+def secular_frequency(mass_da: float, z: float, v_rf: float, f_rf: float, r0: float = R0_REFERENCE,
+                      terms: int = 30) -> float:
+    """Radial secular frequency [Hz] of an ideal quadrupole, a = 0, full kick:
+    q = 4 z e V / (m r0^2 Omega^2), f = beta f_rf / 2 with beta from the
+    Mathieu continued fraction (March, Quadrupole Ion Trap Mass Spectrometry)."""
+    om = 2.0 * np.pi * f_rf
+    q = 4.0 * z * QE * v_rf / (mass_da * AMU * r0 * r0 * om * om)
+    beta = np.sqrt(q * q / 2.0)
+    for _ in range(300):
+        def cf(b, sgn):
+            v = 0.0
+            for k in range(terms, 0, -1):
+                m = b + sgn * 2 * k
+                v = q * q / (m * m - v)
+            return v
+        nb = np.sqrt(max(cf(beta, 1) + cf(beta, -1), 1e-18))
+        if abs(nb - beta) < 1e-14:
+            break
+        beta = nb
+    return beta * f_rf / 2.0
+
+
+def tandem_cube(nx: int, ny: int, nz: int, h: float, seam_z: float, r0_rf: float = 6.0e-3) -> dict:
+    """Splitter/combiner grid of the tandem batch (a stand-in for trapsim's
+    486-post cube, geometry.rs): a confining RF quadrupole E = 2 (Y, X) / r0_rf^2
+    per volt about the box axis, no excitation, and one DC basis (slot 0)
+    that is a uniform axial push E_z = 1 / seam_z per volt (+V drives ions
+    toward the port board in the splitter, -V toward the detector in the
+    combiner). No electrodes inside the box; its faces are the walls."""
+    ax = (nx // 2) * h
+    ay = (ny // 2) * h
+    x = np.arange(nx) * h - ax
+    y = np.arange(ny) * h - ay
+    Z, Y, X = np.meshgrid(np.arange(nz) * h, y, x, indexing="ij")
+    N = nx * ny * nz
+    rf = np.empty((nz, ny, nx, 3))
+    rf[..., 0] = 2.0 * Y / r0_rf**2
+    rf[..., 1] = 2.0 * X / r0_rf**2
+    rf[..., 2] = 0.0
+    push = np.zeros((nz, ny, nx, 3))
+    push[..., 2] = 1.0 / seam_z
+    return {
+        "nx": nx, "ny": ny, "nz": nz, "h": h,
+        "axis_x": ax, "axis_y": ay, "z_inject": 1.0e-3,
+        "z_return": 0.5e-3, "z_exit": seam_z,
+        "z_return_legacy": 1.0e-3, "z_exit_legacy": seam_z,
+        "crash_z0_legacy": 0.0, "crash_z1_legacy": 0.0,
+        "rf": rf.reshape(3 * N), "ac": None,
+        "dc": [push.reshape(3 * N), None, None, None],
+        "mask": None,
+    }
+
+
+def tandem_setup(mod, size: str = "c5", n_per_charge: int | None = None, v_rf: float = 115.0,
+                 v_push: float = 2.0, v_gate: float = 1.0, v_ac: float = 0.5, pitch: float = 2.0e-3,
+                 ap_r: float = 0.8e-3, pressure_pa: float = 0.133322, max_steps: int = 30000):
+    """The tandem configuration. size "c5": cube and ports 128 x 128 x 512 at
+    0.05 mm, 4 x 262144 ions (1M); size "small": cube 48 x 48 x 176 at
+    0.15 mm, ports 64 x 64 x 256 at 0.1 mm, for parity checks.
+
+    Ions: ubiquitin (8565 Da) 6+ to 9+, one job per charge state, injected 1 mm
+    above the splitter floor (Gaussian spot, 300 K). Ports: nine mouths on a
+    3 x 3 lattice of the given pitch about the cube axis; ports 0, 2, 6, 8 are
+    driven at the secular frequencies of 6+, 7+, 8+, 9+ (full kick); the
+    others are idle with the gate raised by 4 V (trapsim lib.rs:196-203).
+    Returns (cube, port, geom, drive, jobs, physics)."""
+    if size == "c5":
+        cube = tandem_cube(128, 128, 512, 0.5e-4, seam_z=24.0e-3)
+        port = ideal_channel(128, 128, 512, 0.5e-4)
+        n = 262144 if n_per_charge is None else n_per_charge
+    elif size == "small":
+        cube = tandem_cube(48, 48, 176, 1.5e-4, seam_z=24.0e-3)
+        port = ideal_channel(64, 64, 256, 1.0e-4)
+        n = 256 if n_per_charge is None else n_per_charge
+    else:
+        raise ValueError(size)
+    ax, ay = cube["axis_x"], cube["axis_y"]
+    mouths = [(ax + (c - 1) * pitch, ay + (r - 1) * pitch) for r in range(3) for c in range(3)]
+    geom = mod.tandem_geom(24.0e-3, mouths, ap_r, 0.5e-3)
+    tones = [0.0] * 9
+    for p, z in zip(DRIVEN_PORTS, UBQ_CHARGES):
+        tones[p] = secular_frequency(UBQ_MASS, z, v_rf, 500e3)
+    drive = mod.tandem_drive(v_rf, 500e3, v_ac, tones, [v_push, 0, 0, 0], [-v_push, 0, 0, 0], [0, v_gate, 0, 0],
+                             [0, 4.0, 0, 0])
+    ccs = {6.0: 1000.0, 7.0: 1150.0, 8.0: 1300.0, 9.0: 1450.0}
+    jobs = [mod.job(UBQ_MASS, z, ccs[z], v_rf, 500e3, v_ac, 0.0, n, 6000 + int(z)) for z in UBQ_CHARGES]
+    ph = mod.Physics.correct(pressure_pa).copy(injection=mod.INJ_GAUSSIAN, inj_sigma_mm=1.0, max_steps=max_steps)
+    return cube, port, geom, drive, jobs, ph

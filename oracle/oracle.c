@@ -686,3 +686,296 @@ int or_grad_reference(const mq_fields* f, const mq_job* job, const double dc[4],
     free(dcc);
     return 0;
 }
+
+/* ================================================================ tandem batch
+ * Restates trapsim's Model::advance / enter / leave
+ * (simulation/trapsim/src/lib.rs:176-385) with chsim's step and bath, in the
+ * operation order of the GPU's FP64 tandem kernel. Counters per port:
+ * entered, returned, forwarded, revisits, from_other, reverse entries, lost,
+ * retry_from[9], by_job[8] (TC_* layout of the product). */
+
+enum { TC_PORT = 24, TC_ENTERED = 0, TC_RETURNED = 1, TC_FORWARDED = 2, TC_REVISITS = 3, TC_FROM_OTHER = 4,
+       TC_REVERSE = 5, TC_LOST = 6, TC_RETRY = 7, TC_BYJOB = 16 };
+#define OR_NO_PORT 255u
+
+typedef struct {
+    const double *rf, *ac;
+    const uint8_t* mask;
+    double inv_h, lim[3];
+    int nx, ny, nz;
+} or_tgrid;
+
+typedef struct {
+    or_tgrid cube, port;
+    const double* dc[4]; /* splitter, combiner, driven port, idle port */
+    double dt, seam, ap_r2, z_back, z_ret, z_exit, ax_p, ay_p, mx[9], my[9];
+    double v_rf, w_rf, v_ac, w_ac[9];
+    uint64_t max_steps;
+    int nearest;
+} or_tctx;
+
+static void tgrid_init(or_tgrid* G, const mq_fields* f)
+{
+    G->rf = f->rf; G->ac = f->ac; G->mask = f->mask;
+    G->inv_h = 1.0 / f->h;
+    G->lim[0] = (double)(f->nx - 1) * f->h;
+    G->lim[1] = (double)(f->ny - 1) * f->h;
+    G->lim[2] = (double)(f->nz - 1) * f->h;
+    G->nx = f->nx; G->ny = f->ny; G->nz = f->nz;
+}
+
+static int tgrid_crash(const or_tgrid* G, const double x[3])
+{
+    if (!G->mask) return 0;
+    return G->mask[(size_t)rint(x[0] * G->inv_h) +
+                   (size_t)G->nx * ((size_t)rint(x[1] * G->inv_h) + (size_t)G->ny * (size_t)rint(x[2] * G->inv_h))] != 0;
+}
+
+static int outside(const or_tgrid* G, const double x[3])
+{
+    return x[0] < 0.0 || x[1] < 0.0 || x[2] < 0.0 || x[0] > G->lim[0] || x[1] > G->lim[1] || x[2] > G->lim[2];
+}
+
+/* classification and region crossing after a step (lib.rs:206-265, 326-377) */
+static int tandem_cross(const or_tctx* T, uint32_t* reg, uint32_t* last, double x[3], double v[3], uint32_t job,
+                        uint64_t* ctr)
+{
+    if (*reg == 0 || *reg == 10) {
+        if (outside(&T->cube, x) || tgrid_crash(&T->cube, x)) return 1;
+        if (x[2] > T->seam) {
+            uint32_t p = 0;
+            double best = 0.0;
+            for (uint32_t k = 0; k < 9; ++k) {
+                double dx = x[0] - T->mx[k], dy = x[1] - T->my[k];
+                double d2 = dx * dx + dy * dy;
+                if (k == 0 || d2 < best) { best = d2; p = k; }
+            }
+            if (!(best < T->ap_r2)) return 1; /* the board between the mouths */
+            uint64_t* c = ctr + (size_t)p * TC_PORT;
+            x[0] = x[0] + (T->ax_p - T->mx[p]);
+            x[1] = x[1] + (T->ay_p - T->my[p]);
+            if (*reg == 0) {
+                x[2] = T->z_ret + (x[2] - T->seam);
+                c[TC_ENTERED] += 1;
+                if (*last != OR_NO_PORT) {
+                    c[TC_REVISITS] += 1;
+                    c[TC_RETRY + *last] += 1;
+                    if (*last != p) c[TC_FROM_OTHER] += 1;
+                }
+            } else {
+                x[2] = T->z_exit - (x[2] - T->seam);
+                v[2] = -v[2];
+                c[TC_REVERSE] += 1;
+            }
+            *reg = 1 + p;
+            return 0;
+        }
+        if (x[2] < T->z_back) return *reg == 0 ? 2 : 3;
+        return 0;
+    }
+    const uint32_t p = *reg - 1;
+    uint64_t* c = ctr + (size_t)p * TC_PORT;
+    if (outside(&T->port, x) || tgrid_crash(&T->port, x)) {
+        c[TC_LOST] += 1;
+        return 1;
+    }
+    if (x[2] < T->z_ret) {
+        x[0] = x[0] + (T->mx[p] - T->ax_p);
+        x[1] = x[1] + (T->my[p] - T->ay_p);
+        x[2] = T->seam + (x[2] - T->z_ret);
+        *last = p;
+        *reg = 0;
+        c[TC_RETURNED] += 1;
+    } else if (x[2] > T->z_exit) {
+        x[0] = x[0] + (T->mx[p] - T->ax_p);
+        x[1] = x[1] + (T->my[p] - T->ay_p);
+        x[2] = T->seam - (x[2] - T->z_exit);
+        v[2] = -v[2];
+        *last = OR_NO_PORT;
+        *reg = 10;
+        c[TC_FORWARDED] += 1;
+        if (job < 8) c[TC_BYJOB + job] += 1;
+    }
+    return 0;
+}
+
+static void tandem_advance(const or_tctx* T, const or_jp* J, or_ion* s, uint32_t ion, uint32_t job, uint8_t* reg_out,
+                           uint8_t* last_out, uint64_t* ctr)
+{
+    const double dt = T->dt;
+    double x[3] = {s->x[0], s->x[1], s->x[2]}, v[3] = {s->v[0], s->v[1], s->v[2]};
+    uint32_t n = s->steps, ncoll = s->ncoll, reg = 0, last = OR_NO_PORT;
+    int status = 0;
+    while (status == 0) {
+        const int cube = reg == 0 || reg == 10;
+        const or_tgrid* G = cube ? &T->cube : &T->port;
+        const double wa = cube ? 0.0 : T->w_ac[reg - 1];
+        const double* dcp = reg == 0 ? T->dc[0] : reg == 10 ? T->dc[1] : (wa > 0.0 ? T->dc[2] : T->dc[3]);
+        double g[3] = {x[0] * G->inv_h, x[1] * G->inv_h, x[2] * G->inv_h};
+        double er[3], ed[3], ea[3] = {0.0, 0.0, 0.0};
+        or_sample(G->rf, G->nx, G->ny, G->nz, g, T->nearest, er);
+        or_sample(dcp, G->nx, G->ny, G->nz, g, T->nearest, ed);
+        double a_rf = T->v_rf * sin2pi(frac(T->w_rf * (double)n));
+        double a_ac = 0.0;
+        if (!cube) {
+            or_sample(G->ac, G->nx, G->ny, G->nz, g, T->nearest, ea);
+            if (wa > 0.0) a_ac = T->v_ac * sin2pi(frac(wa * (double)n));
+        }
+        for (int d = 0; d < 3; ++d) {
+            double E = (a_rf * er[d] + ed[d]) + a_ac * ea[d];
+            v[d] = fma(J->kq, E, v[d]);
+            x[d] = fma(v[d], dt, x[d]);
+        }
+        const uint32_t st = n;
+        n += 1;
+        if (J->nsig_dt > 0.0) {
+            double bound = norm3(v) + J->v_bound;
+            uint32_t w[4];
+            blk(J->key, ion, st >> 2, 0, D_CAND, w);
+            if (u01(w[st & 3]) < J->nsig_dt * bound) {
+                blk(J->key, ion, st, 0, D_COLL, w);
+                double r1 = sqrt(-2.0 * or_log(u01(w[0]))), r2 = sqrt(-2.0 * or_log(u01(w[2])));
+                double s1, c1, s2, c2;
+                or_sincos2pi(u01(w[1]), &s1, &c1);
+                or_sincos2pi(u01(w[3]), &s2, &c2);
+                double gv[3] = {v[0] - J->s_gas * (r1 * c1), v[1] - J->s_gas * (r1 * s1), v[2] - J->s_gas * (r2 * c2)};
+                double gl = norm3(gv);
+                blk(J->key, ion, st, 1, D_COLL, w);
+                if (u01(w[0]) * bound < gl) {
+                    double c = 2.0 * u01(w[1]) - 1.0, sp, cp;
+                    or_sincos2pi(u01(w[2]), &sp, &cp);
+                    double sn = sqrt(fmax(0.0, 1.0 - c * c));
+                    double nn[3] = {sn * cp, sn * sp, c};
+                    double mg = J->mr * gl;
+                    for (int d = 0; d < 3; ++d) v[d] = (v[d] - J->mr * gv[d]) + mg * nn[d];
+                    ncoll += 1;
+                }
+            }
+        }
+        status = tandem_cross(T, &reg, &last, x, v, job, ctr);
+        if (status == 0 && (uint64_t)n >= T->max_steps) status = 5;
+    }
+    s->x[0] = x[0]; s->x[1] = x[1]; s->x[2] = x[2];
+    s->v[0] = v[0]; s->v[1] = v[1]; s->v[2] = v[2];
+    s->steps = n;
+    s->ncoll = ncoll;
+    s->status = (uint8_t)status;
+    *reg_out = (uint8_t)reg;
+    *last_out = (uint8_t)last;
+}
+
+typedef struct {
+    const or_tctx* T;
+    const mq_fields* cube;
+    const mq_job* jobs;
+    const or_jp* jp;
+    const uint64_t* off;
+    size_t njobs, total;
+    const mq_physics* ph;
+    or_ion* out;
+    uint32_t* out_job;
+    uint8_t *region, *last;
+    uint64_t* ctr; /* per thread: 9 * TC_PORT */
+    atomic_size_t next;
+} or_twork;
+
+typedef struct {
+    or_twork* W;
+    uint64_t* ctr;
+} or_targ;
+
+static void* tandem_worker(void* arg)
+{
+    or_targ* A = (or_targ*)arg;
+    or_twork* W = A->W;
+    for (;;) {
+        size_t a = atomic_fetch_add(&W->next, 16);
+        if (a >= W->total) break;
+        size_t b = a + 16 < W->total ? a + 16 : W->total;
+        for (size_t i = a; i < b; ++i) {
+            size_t j = 0;
+            while (j + 1 < W->njobs && W->off[j + 1] <= i) ++j;
+            uint32_t ion = (uint32_t)(i - W->off[j]);
+            or_inject(W->cube, &W->jobs[j], W->ph, ion, &W->out[i]);
+            tandem_advance(W->T, &W->jp[j], &W->out[i], ion, (uint32_t)j, &W->region[i], &W->last[i], A->ctr);
+            if (W->out_job) W->out_job[i] = (uint32_t)j;
+        }
+    }
+    return NULL;
+}
+
+int or_tandem_run(const mq_fields* cube, const mq_fields* port, const mq_tandem_geom* g, const mq_job* jobs,
+                  size_t njobs, const mq_tandem_drive* d, const mq_physics* ph, int nthreads, or_ion* out,
+                  uint32_t* out_job, uint8_t* region, uint8_t* last, uint64_t* counts, uint64_t* totals)
+{
+    or_tctx T;
+    memset(&T, 0, sizeof T);
+    tgrid_init(&T.cube, cube);
+    tgrid_init(&T.port, port);
+    double idle[4];
+    for (int b = 0; b < 4; ++b) idle[b] = d->dc_port[b] + d->bias_idle[b];
+    double* dcs[4] = {combine_dc(cube, d->dc_split), combine_dc(cube, d->dc_combine), combine_dc(port, d->dc_port),
+                      combine_dc(port, idle)};
+    for (int k = 0; k < 4; ++k)
+        if (!dcs[k]) return -1;
+    for (int k = 0; k < 4; ++k) T.dc[k] = dcs[k];
+    T.dt = ph->dt;
+    T.seam = g->seam_z;
+    T.ap_r2 = g->ap_r * g->ap_r;
+    T.z_back = g->z_back;
+    T.z_ret = port->z_return;
+    T.z_exit = port->z_exit;
+    T.ax_p = port->axis_x;
+    T.ay_p = port->axis_y;
+    for (int k = 0; k < 9; ++k) {
+        T.mx[k] = g->mouth[k][0];
+        T.my[k] = g->mouth[k][1];
+        T.w_ac[k] = d->tones[k] * ph->dt;
+    }
+    T.v_rf = d->v_rf;
+    T.w_rf = d->f_rf * ph->dt;
+    T.v_ac = d->v_ac;
+    T.max_steps = max_steps_of(ph);
+    T.nearest = ph->legacy_fields != 0;
+    or_jp* jp = (or_jp*)malloc(njobs * sizeof(or_jp));
+    uint64_t* off = (uint64_t*)malloc((njobs + 1) * sizeof(uint64_t));
+    off[0] = 0;
+    for (size_t j = 0; j < njobs; ++j) {
+        job_params(&jobs[j], ph, &jp[j]);
+        off[j + 1] = off[j] + jobs[j].n;
+    }
+    if (nthreads < 1) nthreads = 1;
+    or_twork W;
+    W.T = &T; W.cube = cube; W.jobs = jobs; W.jp = jp; W.off = off; W.njobs = njobs; W.total = (size_t)off[njobs];
+    W.ph = ph; W.out = out; W.out_job = out_job; W.region = region; W.last = last;
+    atomic_init(&W.next, 0);
+    uint64_t* ctrs = (uint64_t*)calloc((size_t)nthreads * 9 * TC_PORT, sizeof(uint64_t));
+    or_targ* args = (or_targ*)malloc((size_t)nthreads * sizeof(or_targ));
+    pthread_t* th = (pthread_t*)malloc((size_t)nthreads * sizeof(pthread_t));
+    for (int t = 0; t < nthreads; ++t) {
+        args[t].W = &W;
+        args[t].ctr = ctrs + (size_t)t * 9 * TC_PORT;
+        pthread_create(&th[t], NULL, tandem_worker, &args[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    for (int k = 0; k < 9 * TC_PORT; ++k) {
+        uint64_t s = 0;
+        for (int t = 0; t < nthreads; ++t) s += ctrs[(size_t)t * 9 * TC_PORT + k];
+        counts[k] = s;
+    }
+    for (int k = 0; k < 8; ++k) totals[k] = 0;
+    for (size_t i = 0; i < W.total; ++i) {
+        totals[0] += 1;
+        int st = out[i].status, r = region[i];
+        int k = st == 1 ? 4 : st == 2 ? 5 : st == 3 ? 6 : (r == 0 ? 1 : r == 10 ? 3 : 2);
+        totals[k] += 1;
+        totals[7] += out[i].steps;
+    }
+    free(th);
+    free(args);
+    free(ctrs);
+    free(jp);
+    free(off);
+    for (int k = 0; k < 4; ++k) free(dcs[k]);
+    return 0;
+}

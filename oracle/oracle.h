@@ -35,6 +35,7 @@
 #include <stddef.h>
 #include "../include/mq_engine.h"
 #include "../include/mq_fieldsolve.h"
+#include "../include/mq_tandem.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -78,6 +79,16 @@ int or_grad_reference(const mq_fields* f, const mq_job* job, const double dc[4],
 
 /* Trilinear sample of one node-major FP64 basis at grid coordinates g. */
 void or_sample(const double* F, int nx, int ny, int nz, const double g[3], int nearest, double out[3]);
+
+/* Tandem multi-domain batch (trapsim Model::advance / enter / leave,
+ * simulation/trapsim/src/lib.rs:176-385) in the FP64 operation order of the
+ * GPU plain path. out/out_job/region/last: sum(n) entries in job order;
+ * counts: 9 x 24 u64 (entered, returned, forwarded, revisits, from_other,
+ * reverse entries, lost, retry_from[9], by_job[8]); totals: n, split,
+ * channel, combine, lost, back, detected, ion-steps. */
+int or_tandem_run(const mq_fields* cube, const mq_fields* port, const mq_tandem_geom* g, const mq_job* jobs,
+                  size_t njobs, const mq_tandem_drive* d, const mq_physics* ph, int nthreads, or_ion* out,
+                  uint32_t* out_job, uint8_t* region, uint8_t* last, uint64_t* counts, uint64_t* totals);
 
 /* ---- basis-field pipeline (oracle_fields.c) ---- */
 

@@ -68,6 +68,11 @@ def load() -> C.CDLL:
                                    C.c_double, C.c_double, C.c_uint32, C.POINTER(C.c_double),
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
     lib.or_efield.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double)]
+    lib.or_tandem_run.argtypes = [C.POINTER(mqhip.Fields), C.POINTER(mqhip.Fields), C.POINTER(mqhip.TandemGeom),
+                                  C.POINTER(mqhip.Job), C.c_size_t, C.POINTER(mqhip.TandemDrive),
+                                  C.POINTER(mqhip.Physics), C.c_int, C.POINTER(OrIon), u32p,
+                                  C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.POINTER(C.c_uint64),
+                                  C.POINTER(C.c_uint64)]
     _lib = lib
     return lib
 
@@ -223,3 +228,43 @@ def efield(phi, h):
     e = np.zeros(p.shape + (3,))
     load().or_efield(_dp(p), nx, ny, nz, h, _dp(e))
     return e
+
+
+# ---------------------------------------------------------------- tandem batch
+
+PORT_KEYS = ("entered", "returned", "forwarded", "revisits", "from_other", "reverse_entries", "lost")
+TOTAL_KEYS = ("n", "split", "channel", "combine", "lost", "back", "detected", "ion_steps")
+
+
+def tandem_run(cube: dict, port: dict, geom, jobs, drive, ph, nthreads: int = 8):
+    """(per-port counts as dicts, totals dict, final states dict) of the FP64 tandem restatement."""
+    fc, kc = mqhip.make_fields(cube)
+    fp, kp = mqhip.make_fields(port)
+    ja = mqhip.jobs_array(jobs)
+    n = int(sum(j.n for j in jobs))
+    ions = (OrIon * max(n, 1))()
+    job = (C.c_uint32 * max(n, 1))()
+    region = np.zeros(max(n, 1), np.uint8)
+    last = np.zeros(max(n, 1), np.uint8)
+    counts = np.zeros(9 * 24, np.uint64)
+    totals = np.zeros(8, np.uint64)
+    u8 = C.POINTER(C.c_uint8)
+    u64 = C.POINTER(C.c_uint64)
+    rc = load().or_tandem_run(C.byref(fc), C.byref(fp), C.byref(geom), ja, len(jobs), C.byref(drive), C.byref(ph),
+                              nthreads, ions, job, region.ctypes.data_as(u8), last.ctypes.data_as(u8),
+                              counts.ctypes.data_as(u64), totals.ctypes.data_as(u64))
+    if rc != 0:
+        raise RuntimeError("oracle tandem run failed")
+    del kc, kp
+    d = _ions_to_dict(ions, job)
+    d = {k: v[:n] for k, v in d.items()}
+    d["region"] = region[:n].copy()
+    d["last"] = last[:n].copy()
+    cs = []
+    for p in range(9):
+        a = counts[24 * p: 24 * (p + 1)]
+        c = {k: int(a[i]) for i, k in enumerate(PORT_KEYS)}
+        c["retry_from"] = [int(v) for v in a[7:16]]
+        c["by_job"] = [int(v) for v in a[16:24]]
+        cs.append(c)
+    return cs, {k: int(totals[i]) for i, k in enumerate(TOTAL_KEYS)}, d
