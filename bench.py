@@ -12,7 +12,10 @@ batched mass scan: m/z 475..523 step 2 (25 masses) x 64 RF frequencies
 Prints one JSON line: value = FP32 batched ion-steps/s; plus the plain FP64
 form on the same inputs, the outcome agreement between them, the roofline
 object of the batched worker kernel and the CPU baseline (the FP64 oracle on
-the host's cores over a bounded sample of the same jobs).
+the host's cores over a bounded sample of the same jobs). Sub-objects: the
+differentiable form (gradient evaluations/s), the tandem multi-domain batch
+(1M ions) and the reference channel geometry solved on the GPU with the two
+numbers the reference recorded for it.
 """
 from __future__ import annotations
 
@@ -184,6 +187,91 @@ def grad_bench(fd: bool):
     return out
 
 
+def tandem_bench(steps: int = 2):
+    """Tandem multi-domain batch (C5): ubiquitin 6+..9+ (4 x 262144 ions) through a
+    splitter, nine ports (four driven) and a combiner on one clock; FP32 batched
+    vs FP64 plain on the same inputs, per-port counts and the retry matrix."""
+    cube, port, geom, d, jobs, ph = synth.tandem_setup(mqhip, "c5")
+    t = mqhip.Tandem(cube, port, geom)
+    t.run(jobs, d, ph, mqhip.PATH_BATCHED)  # warmup: cell tables
+    n_steps = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cb, tb = t.run(jobs, d, ph, mqhip.PATH_BATCHED)
+        n_steps += t.stats()["ion_steps"]
+    wall = time.perf_counter() - t0
+    kms = t.stats()["kernel_ms"]
+    t0 = time.perf_counter()
+    cp, tp = t.run(jobs, d, ph, mqhip.PATH_PLAIN)
+    pwall = time.perf_counter() - t0
+    psteps = t.stats()["ion_steps"]
+    t.close()
+    log(f"tandem: batched {n_steps / wall:.3e} ion-steps/s, plain {psteps / pwall:.3e}")
+
+    def pull(a, b):
+        return (a - b) / np.sqrt(max(a + b, 1))
+
+    keys = ("entered", "returned", "forwarded", "lost", "revisits", "from_other")
+    port_pulls = [pull(cb[p][k], cp[p][k]) for p in range(9) for k in keys]
+    end_pulls = [pull(tb[k], tp[k]) for k in ("split", "channel", "combine", "lost", "back", "detected")]
+    return {
+        "config": "ubiquitin (8565 Da) 6+..9+, 4 x 262144 = 1048576 ions; splitter/combiner grid 128x128x512 @ "
+                  "0.05 mm (confining RF, axial push), nine ports on one 128x128x512 @ 0.05 mm channel grid; "
+                  "115 V RF @ 500 kHz shared clock; ports 0, 2, 6, 8 driven at the 6+..9+ secular frequencies "
+                  f"{[round(x / 1e3, 1) for x in d.tones if x > 0]} kHz (0.5 V), idle ports +4 V gate; "
+                  "thermal N2 1 mTorr; dt 40 ns; cap 30000 steps",
+        "value": n_steps / wall, "unit": "ion-steps/s", "kernel_ion_steps_per_s": n_steps / max(kms * 1e-3 * steps, 1e-9),
+        "plain_fp64": psteps / pwall, "speedup_vs_plain_fp64": (n_steps / wall) / (psteps / pwall),
+        "ms_per_run": 1e3 * wall / steps,
+        "totals_batched": tb, "totals_plain": tp,
+        "ports_batched": [{k: cb[p][k] for k in keys + ("reverse_entries",)} | {"by_job": cb[p]["by_job"][:4]}
+                          for p in range(9)],
+        "ports_plain": [{k: cp[p][k] for k in keys + ("reverse_entries",)} | {"by_job": cp[p]["by_job"][:4]}
+                        for p in range(9)],
+        "retry_matrix_batched": [cb[p]["retry_from"] for p in range(9)],
+        "retry_matrix_plain": [cp[p]["retry_from"] for p in range(9)],
+        "count_pulls_within_4sigma": float(np.mean(np.abs(np.array(port_pulls + end_pulls)) <= 4.0)),
+        "max_abs_count_pull": float(np.max(np.abs(np.array(port_pulls + end_pulls)))),
+        "fp64_exactness": "the FP64 path's per-port counts and retry matrix equal the CPU oracle's exactly "
+                          "(tests/test_gpu_tandem.py)",
+    }
+
+
+def reference_geometry_bench():
+    """The reference's own channel geometry, solved on the GPU, and two numbers the
+    reference recorded: REF_CHSIM_GPU (validate.rs:110-112) and the legacy Fig. 4 peak."""
+    out = {}
+    t0 = time.perf_counter()
+    g14 = mqhip.channel_grid(mqhip.Geom.s14(), tol=1e-7)
+    out["field_build_s"] = time.perf_counter() - t0
+    out["field_build_cg_iterations"] = [st["iterations"] for st in g14["stats"]]
+    e = mqhip.Engine(g14)
+    ph = mqhip.Physics.correct(0.1).copy(t_max=3e-3)
+    j = mqhip.job(2 * 863.473, 2.0, 425.0, 70.0, 500e3, 0.35, 84.25e3, 100000, 4242)
+    o = e.run_batch([j], [18, 24, 15, 20], ph, mqhip.PATH_BATCHED)[0]
+    st = e.stats()
+    e.close()
+    ref = {"transmitted": 19669, "crashed_plus_left": 3412, "returned": 75797, "timeout": 1122}
+    got = {"transmitted": o.transmitted, "crashed_plus_left": o.crashed + o.left, "returned": o.returned,
+           "timeout": o.timeout}
+    p = {k: (got[k] - ref[k]) / 1e5 / np.sqrt((got[k] / 1e5 * (1 - got[k] / 1e5) + ref[k] / 1e5 * (1 - ref[k] / 1e5))
+                                             / 1e5) for k in ref}
+    out["s14_ref_chsim_gpu"] = {"batched": got, "reference": ref, "pulls": p,
+                                "ion_steps_per_s": st["ion_steps"] / (st["wall_ms"] * 1e-3)}
+    gd = mqhip.channel_grid(mqhip.Geom.default(), tol=1e-7)
+    e = mqhip.Engine(gd)
+    mzs = [475.0 + 2 * i for i in range(25)]
+    jobs = [mqhip.job(mz * 5, 5.0, 0.0, 100.0, 500e3, 0.5, 100e3, 1000, 1000 + int(mz)) for mz in mzs]
+    outs = e.run_batch(jobs, [0, 5, -5, 0], mqhip.Physics.legacy(1e-3 * 133.322), mqhip.PATH_BATCHED)
+    e.close()
+    tr = [o.transmitted for o in outs]
+    k = int(np.argmax(tr))
+    out["fig4_legacy_converged"] = {"transmitted_per_1000": tr, "peak_mz": mzs[k], "peak": tr[k],
+                                    "reference_rust_legacy_peak": 742}
+    log(f"reference geometry: S14 pulls {p}, Fig. 4 peak {tr[k]} at m/z {mzs[k]}")
+    return out
+
+
 def read_traffic(name: str):
     """HBM bytes per worker launch from the committed rocprofv3 PMC summary, if any."""
     path = os.path.join(ROOT, "profiles", f"k_batched_{name}_traffic.md")
@@ -206,6 +294,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-grad", action="store_true")
     ap.add_argument("--grad-fd", type=int, default=1, help="check the adjoint against central differences")
+    ap.add_argument("--skip-tandem", action="store_true")
+    ap.add_argument("--skip-refgeom", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one torch elementwise copy of 1 GiB (read 1 GiB, write 1 GiB) first, as the "
                          "known-size reference for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -289,6 +379,10 @@ def main():
     eng.close()
     if not a.skip_grad:
         result["grad"] = grad_bench(bool(a.grad_fd))
+    if not a.skip_tandem:
+        result["tandem_c5"] = tandem_bench()
+    if not a.skip_refgeom:
+        result["reference_geometry"] = reference_geometry_bench()
     if not a.skip_cpu:
         result["cpu_baseline"] = cpu_baseline(grid, jobs, dc, ph, a.cpu_seconds)
     print(json.dumps(result), flush=True)
