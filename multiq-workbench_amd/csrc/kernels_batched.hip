@@ -103,7 +103,8 @@ struct Lane {
     float vx, vy, vz;
     float kq, vrf, vac, sdc, nsig, sgas, vb, mr, omr, floorv, mfpm;
     uint32_t st, ncoll, cd, k0, k1, ion, id, visit;
-    float tx, ty, tz, fx, fy, fz, a_rf, a_ac;
+    float tx, ty, tz, a_rf, a_ac;
+    uint32_t corner;  // nearest corner of the cell (bit dx + 2 dy + 4 dz), for the crash test
     uint32_t cw1, cw2, cw3;
     int32_t cell;
 };
@@ -114,12 +115,11 @@ __device__ __forceinline__ int32_t locate(const BatchP& B, Lane& L)
     double gy = fmin(fmax(L.y * B.inv_h, 0.0), (double)(B.ny - 1));
     double gz = fmin(fmax(L.z * B.inv_h, 0.0), (double)(B.nz - 1));
     int i = min((int)gx, B.nx - 2), j = min((int)gy, B.ny - 2), k = min((int)gz, B.nz - 2);
-    L.fx = (float)(gx - (double)i);
-    L.fy = (float)(gy - (double)j);
-    L.fz = (float)(gz - (double)k);
-    L.tx = B.nearest ? 0.0f : L.fx;
-    L.ty = B.nearest ? 0.0f : L.fy;
-    L.tz = B.nearest ? 0.0f : L.fz;
+    const float fx = (float)(gx - (double)i), fy = (float)(gy - (double)j), fz = (float)(gz - (double)k);
+    L.corner = (fx >= 0.5f ? 1u : 0u) | (fy >= 0.5f ? 2u : 0u) | (fz >= 0.5f ? 4u : 0u);
+    L.tx = B.nearest ? 0.0f : fx;
+    L.ty = B.nearest ? 0.0f : fy;
+    L.tz = B.nearest ? 0.0f : fz;
     return i + (B.nx - 1) * (j + (B.ny - 1) * k);
 }
 
@@ -140,6 +140,9 @@ __device__ __forceinline__ uint32_t countdown_f(float x)
     return (uint32_t)x;
 }
 
+// LEGACY selects the collision model at compile time (legacy countdown or the
+// thermal bath), so each instantiation keeps only its own values live.
+template <bool LEGACY>
 __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
 {
     const BatchP& B = A.B;
@@ -180,7 +183,8 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                     L.ion = (uint32_t)(id - J.off);
                     L.x = S.x[id]; L.y = S.y[id]; L.z = S.z[id]; L.phi = S.phi[id];
                     L.vx = S.fvx[id]; L.vy = S.fvy[id]; L.vz = S.fvz[id];
-                    L.st = S.steps[id]; L.ncoll = S.ncoll[id]; L.cd = S.cd[id];
+                    L.st = S.steps[id]; L.ncoll = S.ncoll[id];
+                    if constexpr (LEGACY) L.cd = S.cd[id];
                     L.w_rf = J.w_rf; L.w_ac = J.w_ac;
                     L.kq = (float)J.kq; L.vrf = (float)J.v_rf; L.vac = (float)J.v_ac; L.sdc = (float)J.s_dc;
                     L.nsig = (float)J.nsig_dt; L.sgas = (float)J.s_gas; L.vb = (float)J.v_bound;
@@ -231,7 +235,7 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                 if (c != L.cell) { L.cell = c; load_cell(A.cells, c, R); }
                 phases(L);
                 // ---- collision candidate of this step
-                if (!B.legacy_coll) {
+                if constexpr (!LEGACY) {
                     if (L.nsig > 0.0f) {
                         const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
                         uint32_t cw;
@@ -256,8 +260,7 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                 else if (L.x < 0.0 || L.y < 0.0 || L.z < 0.0 || L.x > B.lim0 || L.y > B.lim1 || L.z > B.lim2) status = 4;
                 else if (L.z > B.cz0 && L.z < B.cz1) {
                     const uint32_t bits = __float_as_uint(R[18].x);
-                    const uint32_t corner = (L.fx >= 0.5f ? 1u : 0u) | (L.fy >= 0.5f ? 2u : 0u) | (L.fz >= 0.5f ? 4u : 0u);
-                    if ((bits >> corner) & 1u) status = 1;
+                    if ((bits >> L.corner) & 1u) status = 1;
                 }
                 if (status == 0 && L.st >= B.max_steps) status = 5;
                 if (status != 0 || L.visit >= A.chunk) stop = true;
@@ -269,7 +272,7 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                 const uint64_t am = __ballot(have && !stop && !pending);
                 if (npend >= A.coll_batch || am == 0 || r + 1 == A.period) {
                     if (pending) {
-                        if (!B.legacy_coll) {
+                        if constexpr (!LEGACY) {
                             const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
                             const u32x4 w = philox(L.ion, pcur, 0, D_COLL, L.k0, L.k1);
                             const float r1 = sqrtf(-2.0f * log_f(u01f(w.a))), r2 = sqrtf(-2.0f * log_f(u01f(w.c)));
@@ -323,7 +326,8 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
             const uint32_t id = L.id;
             S.x[id] = L.x; S.y[id] = L.y; S.z[id] = L.z;
             S.fvx[id] = L.vx; S.fvy[id] = L.vy; S.fvz[id] = L.vz;
-            S.steps[id] = L.st; S.ncoll[id] = L.ncoll; S.cd[id] = L.cd;
+            S.steps[id] = L.st; S.ncoll[id] = L.ncoll;
+            if constexpr (LEGACY) S.cd[id] = L.cd;
             S.status[id] = (uint8_t)status;
             have = false;
         }
@@ -444,7 +448,8 @@ void launch_prepare(const WorkerArgs& a, hipStream_t st) { k_prepare<<<1, 64, 0,
 
 void launch_batched(const WorkerArgs& a, uint32_t grid, uint32_t block, hipStream_t st)
 {
-    k_batched<<<grid, block, 0, st>>>(a);
+    if (a.B.legacy_coll) k_batched<true><<<grid, block, 0, st>>>(a);
+    else k_batched<false><<<grid, block, 0, st>>>(a);
 }
 
 }  // namespace mq
