@@ -272,15 +272,20 @@ def reference_geometry_bench():
     return out
 
 
-def read_traffic(name: str):
-    """HBM bytes per worker launch from the committed rocprofv3 PMC summary, if any."""
+def read_profile_value(name: str, key: str):
+    """A figure from the committed rocprofv3 PMC summary of the worker kernel, if any."""
     path = os.path.join(ROOT, "profiles", f"k_batched_{name}_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def read_traffic(name: str):
+    """HBM bytes per worker launch from the committed rocprofv3 PMC summary, if any."""
+    return read_profile_value(name, "traffic_bytes_per_launch")
 
 
 def main():
@@ -350,6 +355,21 @@ def main():
                           "bytes_per_ion_step": BYTES_PER_STEP_FP32,
                           "ion_steps_per_launch": steps / max(launches, 1),
                           "avg_launch_ms": kms / max(launches, 1)}
+
+    # achievable random-gather bandwidth of this config's cell table (304-B records, worker occupancy):
+    # the whole table, a 128 MB window (Infinity Cache sized) and a 2.5 MB window (L2 sized)
+    ncell = (grid["nx"] - 1) * (grid["ny"] - 1) * (grid["nz"] - 1)
+    gath = {"table": eng.gather_probe(0) / 1e9, "window_128MB": eng.gather_probe(128 * 2**20 // 304) / 1e9,
+            "window_2.5MB": eng.gather_probe(8192) / 1e9}
+    traffic = result["roofline"]["traffic"]
+    r = result["roofline"]
+    r["achievable_gather_gbs"] = gath
+    r["table_bytes"] = ncell * 304
+    if traffic:
+        r["traffic_gbs"] = traffic / (r["avg_launch_ms"] * 1e-3) / 1e9
+        r["traffic_frac_of_achievable_table_gather"] = r["traffic_gbs"] / gath["table"]
+    r["valu_busy"] = read_profile_value(a.config, "valu_busy")
+    log(f"gather probe: {gath}")
 
     if a.plain_steps > 0:
         psteps = pk = 0

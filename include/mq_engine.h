@@ -174,6 +174,14 @@ int mq_summarize(const uint8_t* status, const uint32_t* ncoll, const uint32_t* s
 /* Statistics of the last run_batch / run_batch_ions call on this engine. */
 int mq_last_stats(const mq_engine* eng, mq_run_stats* out);
 
+/* Achievable gather bandwidth of the engine's FP32 cell table (no reference
+ * counterpart: the measurement that a roofline of the batched path needs).
+ * Random whole-record reads (304 B, the worker's access unit) by persistent
+ * wavefronts at the worker's occupancy, over the first window_cells records
+ * of the table built by the last batched run (0 = the whole table); iters
+ * records per lane. *bytes_per_s receives the bandwidth. */
+int mq_gather_probe(mq_engine* eng, uint64_t window_cells, uint32_t iters, double* bytes_per_s);
+
 /* Tuning of the batched path; replaces TileCfg::from_env (tiled.rs:24-28). */
 int mq_set_tune(mq_engine* eng, const mq_tune* t);
 int mq_get_tune(const mq_engine* eng, mq_tune* out);

@@ -743,6 +743,31 @@ int mq_last_stats(const mq_engine* eng, mq_run_stats* out)
     })
 }
 
+int mq_gather_probe(mq_engine* eng, uint64_t window_cells, uint32_t iters, double* bytes_per_s)
+{
+    MQ_GUARD({
+        if (!eng || !bytes_per_s) throw Err(-1, "NULL argument");
+        if (!eng->cells_valid) throw Err(-1, "no cell table yet: run a batched batch first");
+        const uint64_t nc = (uint64_t)(eng->nx - 1) * (eng->ny - 1) * (eng->nz - 1);
+        const uint64_t window = window_cells ? std::min<uint64_t>(window_cells, nc) : nc;
+        if (iters == 0 || iters > 4096) throw Err(-1, "iters must be in [1, 4096]");
+        const uint32_t block = 256;
+        const uint32_t grid = std::max<uint32_t>(1, (uint32_t)((uint64_t)eng->cus * eng->tune.waves_per_cu * 64 / block));
+        DevBuf out;
+        out.ensure((size_t)grid * block * sizeof(float));
+        launch_gather_probe(eng->cells.as<float4>(), window, iters, out.as<float>(), grid, block, eng->stream);  // warm
+        HIPCHK(hipEventRecord(eng->ev0, eng->stream));
+        launch_gather_probe(eng->cells.as<float4>(), window, iters, out.as<float>(), grid, block, eng->stream);
+        HIPCHK(hipEventRecord(eng->ev1, eng->stream));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventSynchronize(eng->ev1));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, eng->ev0, eng->ev1));
+        out.release();
+        *bytes_per_s = (double)grid * block * iters * CELL_F4 * 16.0 / (ms * 1e-3);
+    })
+}
+
 int mq_set_tune(mq_engine* eng, const mq_tune* t)
 {
     MQ_GUARD({

@@ -50,6 +50,8 @@ void launch_sort(const IonState& S, const BatchP& B, const uint32_t* list, const
                  uint32_t* bin_of, uint32_t* rank_of, uint32_t* hist, uint32_t* bsum, const SortGeom& G,
                  uint32_t nmax, hipStream_t st);
 void launch_prepare(const WorkerArgs& a, hipStream_t st);
+void launch_gather_probe(const float4* cells, uint64_t window, uint32_t iters, float* out, uint32_t grid,
+                         uint32_t block, hipStream_t st);
 void launch_batched(const WorkerArgs& a, uint32_t grid, uint32_t block, hipStream_t st);
 // Differentiable form (kernels_adjoint.hip).
 struct AdjGrids {

@@ -334,6 +334,35 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
     }
 }
 
+// ------------------------------------------------- gather probe
+// Random whole-record reads over the first `window` records: each lane reads
+// `iters` records at hashed indices (splitmix-style 64-bit mixer of lane and
+// iteration) and folds them into one float, so no load is dead code.
+__global__ __launch_bounds__(256) void k_gather_probe(const float4* __restrict__ cells, uint64_t window, uint32_t iters,
+                                                      float* __restrict__ out)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float acc = 0.0f;
+    for (uint32_t it = 0; it < iters; ++it) {
+        uint64_t h = t * 0x9E3779B97F4A7C15ull + (uint64_t)it * 0xD1B54A32D192ED03ull;
+        h ^= h >> 31;
+        h *= 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 29;
+        const uint64_t c = h % window;
+        float4 R[CELL_F4];
+        load_cell(cells, (int32_t)c, R);
+#pragma unroll
+        for (int q = 0; q < CELL_F4; ++q) acc += (R[q].x + R[q].y) + (R[q].z + R[q].w);
+    }
+    out[t] = acc;
+}
+
+void launch_gather_probe(const float4* cells, uint64_t window, uint32_t iters, float* out, uint32_t grid,
+                         uint32_t block, hipStream_t st)
+{
+    k_gather_probe<<<grid, block, 0, st>>>(cells, window, iters, out);
+}
+
 // ------------------------------------------------- re-sort by grid cell
 // Counting sort of the live list by coarse cell bin (z-major, x fastest):
 // k_bin counts, the three scan kernels turn counts into offsets, k_scatter

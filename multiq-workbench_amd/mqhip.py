@@ -246,6 +246,7 @@ SIGNATURES = {
                             C.POINTER(C.c_double)]),
     "mq_fields_build": (C.c_int, [C.POINTER(Geom), C.c_double, C.c_char_p, C.POINTER(ChannelFields)]),
     "mq_field_cache_name": (C.c_int, [C.POINTER(Geom), C.c_int32, C.c_char_p, C.c_size_t]),
+    "mq_gather_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
     "mq_tandem_create": (C.c_int, [C.POINTER(Fields), C.POINTER(Fields), C.POINTER(TandemGeom),
                                    C.POINTER(C.c_void_p)]),
     "mq_tandem_destroy": (None, [C.c_void_p]),
@@ -378,6 +379,12 @@ class Engine:
         s = RunStats()
         _check(load().mq_last_stats(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def gather_probe(self, window_cells: int = 0, iters: int = 64) -> float:
+        """Achievable random-gather bandwidth [B/s] of the cell table (whole table, or its first window_cells)."""
+        b = C.c_double()
+        _check(load().mq_gather_probe(self._h, window_cells, iters, C.byref(b)))
+        return b.value
 
     def set_tune(self, **kw) -> None:
         t = Tune()
