@@ -43,6 +43,15 @@ enum { MQ_PATH_BATCHED = 0, MQ_PATH_PLAIN = 1 };
  * sigma inj_sigma_mm about the axis, Maxwell-Boltzmann velocity at temp_k. */
 enum { MQ_INJ_REFERENCE = 0, MQ_INJ_GAUSSIAN = 1 };
 
+/* Random streams. PHILOX: Philox4x32-10 keyed by the job seed, counter
+ * (ion, step, ...): results depend on (seed, ion, step) only. COMPAT: the
+ * reference's own streams: one xorshift64 stream per job drives injection
+ * exactly as Engine::inject (channel.rs:342-374, on the host, with its
+ * fixed-point positions and f32 velocities), and a per-ion xoshiro128**
+ * stream seeded from it drives collisions in the draw order of
+ * channel.metal:38-53, 93-113. */
+enum { MQ_RNG_PHILOX = 0, MQ_RNG_COMPAT = 1 };
+
 /* Species (channel.rs:236): mass [Da], charge number, N2 CCS [A^2]. */
 typedef struct { double mass, z, ccs_n2; } mq_species;
 
@@ -75,6 +84,8 @@ typedef struct {
     double pressure_pa, temp_k, dt, t_max, r_inject_mm;
     double inj_sigma_mm;
     uint64_t max_steps;
+    int32_t rng;                /* MQ_RNG_* */
+    int32_t pad_;
 } mq_physics;
 
 /* Per-job outcome counts (channel.rs:297). mean_time in s. */
@@ -165,6 +176,15 @@ int mq_run_batch(mq_engine* eng, const mq_job* jobs, size_t njobs, const double 
  * jobs[k].n entries) and the job index of each ion (out_job may be NULL). */
 int mq_run_batch_ions(mq_engine* eng, const mq_job* jobs, size_t njobs, const double dc[4],
                       const mq_physics* ph, int32_t path, const mq_ion_soa* out, uint32_t* out_job);
+
+/* Engine::inject (channel.rs:342-374) on the host with the reference's
+ * xorshift64 stream (the injection of MQ_RNG_COMPAT): the initial state of
+ * the n ions of one job in s (positions after the reference's fixed-point
+ * truncation, velocities rounded to f32, phi_ac0 the AC phase in cycles
+ * rounded to f32, cd the legacy countdown) and the four xoshiro128** seed
+ * words of each ion in rng[4n]. Host only, no GPU. */
+int mq_inject_reference(const mq_fields* f, const mq_job* job, const mq_physics* ph, const mq_ion_soa* s,
+                        uint32_t* rng);
 
 /* Replaces summarize (channel.rs:496-506): per-job counts and means from final
  * states. dt converts steps to time. */

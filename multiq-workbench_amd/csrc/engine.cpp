@@ -49,7 +49,7 @@ struct mq_engine {
     DevBuf cells;  // FP32 cell records
     // per-call pools
     DevBuf jobs, acc, list[3], ctr, bin_of, rank_of, hist, bsum;
-    DevBuf sx, sy, sz, sphi, svx, svy, svz, sfx, sfy, sfz, ssteps, sncoll, scd, sjob, sstatus;
+    DevBuf sx, sy, sz, sphi, svx, svy, svz, sfx, sfy, sfz, ssteps, sncoll, scd, sjob, sstatus, srng;
     mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0, 4};
     // differentiable form: context of the last forward
     bool adj_valid = false;
@@ -106,7 +106,7 @@ struct mq_engine {
     {
         for (DevBuf* b : {&rf, &ac, &dcb[0], &dcb[1], &dcb[2], &dcb[3], &mask, &dcc, &cells, &jobs, &acc, &list[0],
                           &list[1], &list[2], &ctr, &bin_of, &rank_of, &hist, &bsum, &sx, &sy, &sz, &sphi, &svx, &svy, &svz, &sfx, &sfy, &sfz, &ssteps,
-                          &sncoll, &scd, &sjob, &sstatus, &adj_mem, &adj_cols, &adj_part, &adj_out})
+                          &sncoll, &scd, &sjob, &sstatus, &srng, &adj_mem, &adj_cols, &adj_part, &adj_out})
             b->release();
         for (auto e : evs) (void)hipEventDestroy(e);
         evs.clear();
@@ -160,6 +160,41 @@ struct mq_engine {
         return S;
     }
 
+    // The reference's injection and xoshiro128** seeds (MQ_RNG_COMPAT), on the
+    // host as Engine::inject does it, uploaded into the ion state.
+    void inject_compat(IonState& S, const mq_job* jobs_h, size_t njobs, const mq_physics& ph, uint32_t n)
+    {
+        std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n), phi(n);
+        std::vector<uint32_t> cd(n), rng(4 * (size_t)n), job(n), zero(n, 0u);
+        std::vector<float> fv(n);
+        std::vector<uint8_t> st(n, 0);
+        uint64_t off = 0;
+        for (size_t k = 0; k < njobs; ++k) {
+            const uint64_t m = jobs_h[k].n;
+            inject_reference(meta, jobs_h[k], ph, m, &x[off], &y[off], &z[off], &vx[off], &vy[off], &vz[off],
+                             &phi[off], &cd[off], &rng[4 * off]);
+            for (uint64_t i = 0; i < m; ++i) job[off + i] = (uint32_t)k;
+            off += m;
+        }
+        srng.ensure((size_t)n * 16);
+        S.rng = srng.as<uint32_t>();
+        auto up = [&](void* dst, const void* src, size_t bytes) {
+            HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+        };
+        up(S.x, x.data(), n * 8); up(S.y, y.data(), n * 8); up(S.z, z.data(), n * 8); up(S.phi, phi.data(), n * 8);
+        up(S.vx, vx.data(), n * 8); up(S.vy, vy.data(), n * 8); up(S.vz, vz.data(), n * 8);
+        up(S.cd, cd.data(), n * 4); up(S.rng, rng.data(), (size_t)n * 16); up(S.job, job.data(), n * 4);
+        up(S.steps, zero.data(), n * 4); up(S.ncoll, zero.data(), n * 4); up(S.status, st.data(), n);
+        float* fdst[3] = {S.fvx, S.fvy, S.fvz};
+        const std::vector<double>* vsrc[3] = {&vx, &vy, &vz};
+        for (int d = 0; d < 3; ++d) {
+            for (uint32_t i = 0; i < n; ++i) fv[i] = (float)(*vsrc[d])[i];
+            up(fdst[d], fv.data(), n * 4);
+            HIPCHK(hipStreamSynchronize(stream));  // fv is reused
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
     BatchP batch_params(const mq_physics& ph, uint32_t njobs) const
     {
         BatchP B{};
@@ -176,6 +211,7 @@ struct mq_engine {
         B.nx = nx; B.ny = ny; B.nz = nz;
         B.nearest = ph.legacy_fields ? 1 : 0;
         B.legacy_coll = ph.legacy_collisions ? 1 : 0;
+        B.compat = ph.rng == MQ_RNG_COMPAT ? 1 : 0;
         B.max_steps = max_steps_of(ph);
         B.njobs = njobs;
         return B;
@@ -191,6 +227,7 @@ struct mq_engine {
         if (!jobs_h || njobs == 0) throw Err(-1, "batch has no jobs");
         if (!(ph.dt > 0.0)) throw Err(-1, "dt must be positive");
         if (path != MQ_PATH_BATCHED && path != MQ_PATH_PLAIN) throw Err(-1, "unknown path");
+        if (ph.rng != MQ_RNG_PHILOX && ph.rng != MQ_RNG_COMPAT) throw Err(-1, "unknown random stream (rng)");
         std::vector<JobP> jp(njobs);
         uint64_t off = 0;
         for (size_t k = 0; k < njobs; ++k) {
@@ -210,7 +247,11 @@ struct mq_engine {
         ip.axis_x = meta.axis_x; ip.axis_y = meta.axis_y; ip.z_inject = meta.z_inject;
         ip.temp_k = ph.temp_k; ip.r_inject_mm = ph.r_inject_mm; ip.inj_sigma_mm = ph.inj_sigma_mm; ip.dt = ph.dt;
         ip.model = ph.injection == MQ_INJ_GAUSSIAN ? 2 : (ph.legacy_injection ? 1 : 0);
-        launch_inject(S, jobs.as<JobP>(), (uint32_t)njobs, ip, n, stream);
+        if (ph.rng == MQ_RNG_COMPAT) {
+            if (n) inject_compat(S, jobs_h, njobs, ph, n);
+        } else {
+            launch_inject(S, jobs.as<JobP>(), (uint32_t)njobs, ip, n, stream);
+        }
         HIPCHK(hipGetLastError());
         const BatchP B = batch_params(ph, (uint32_t)njobs);
         stats = mq_run_stats{};
@@ -283,6 +324,7 @@ struct mq_engine {
         if (!(o.scale > 0.0)) throw Err(-1, "objective scale must be positive");
         if (ph.pressure_pa > 0.0 || job.pressure_pa > 0.0)
             throw Err(-1, "the differentiable form is collision-free: set pressure_pa to 0");
+        if (ph.rng != MQ_RNG_PHILOX) throw Err(-1, "the differentiable form injects with the Philox stream (rng = 0)");
         const uint32_t n = (uint32_t)job.n;
         const uint32_t K = o.ckpt_every ? o.ckpt_every : (uint32_t)std::ceil(std::sqrt((double)o.nsteps));
         const uint32_t slots = (o.nsteps + K - 1) / K;

@@ -102,5 +102,11 @@ inline uint32_t max_steps_of(const mq_physics& ph)
 }
 
 
+// Engine::inject with the reference's xorshift64 stream (inject_ref.cpp): the
+// n ions of one job, as the reference stores them, plus 4 xoshiro128** seed
+// words per ion.
+void inject_reference(const mq_fields& f, const mq_job& job, const mq_physics& ph, uint64_t n, double* x, double* y,
+                      double* z, double* vx, double* vy, double* vz, double* phi, uint32_t* cd, uint32_t* rng);
+
 }  // namespace host
 }  // namespace mq

@@ -106,6 +106,7 @@ struct Lane {
     float tx, ty, tz, a_rf, a_ac;
     uint32_t corner;  // nearest corner of the cell (bit dx + 2 dy + 4 dz), for the crash test
     uint32_t cw1, cw2, cw3;
+    uint32_t xs0, xs1, xs2, xs3;  // xoshiro128** state (compat streams)
     int32_t cell;
 };
 
@@ -142,7 +143,9 @@ __device__ __forceinline__ uint32_t countdown_f(float x)
 
 // LEGACY selects the collision model at compile time (legacy countdown or the
 // thermal bath), so each instantiation keeps only its own values live.
-template <bool LEGACY>
+// COMPAT selects the reference's per-ion xoshiro128** stream (draw order of
+// channel.metal:93-113) instead of Philox.
+template <bool LEGACY, bool COMPAT>
 __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
 {
     const BatchP& B = A.B;
@@ -193,6 +196,10 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                     L.k0 = J.key0; L.k1 = J.key1;
                     L.visit = 0;
                     L.cw1 = L.cw2 = L.cw3 = 0;
+                    if constexpr (COMPAT) {
+                        L.xs0 = S.rng[4 * (size_t)id]; L.xs1 = S.rng[4 * (size_t)id + 1];
+                        L.xs2 = S.rng[4 * (size_t)id + 2]; L.xs3 = S.rng[4 * (size_t)id + 3];
+                    }
                     const int32_t c = locate(B, L);
                     if (c != L.cell) { L.cell = c; load_cell(A.cells, c, R); }
                     phases(L);
@@ -235,7 +242,11 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                 if (c != L.cell) { L.cell = c; load_cell(A.cells, c, R); }
                 phases(L);
                 // ---- collision candidate of this step
-                if constexpr (!LEGACY) {
+                if constexpr (!LEGACY && COMPAT) {
+                    // the reference draws a candidate uniform every step (channel.metal:94-95)
+                    const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
+                    if (xo_u01f(xo_next(L.xs0, L.xs1, L.xs2, L.xs3)) < L.nsig * bound) { pending = true; pcur = cur; }
+                } else if constexpr (!LEGACY) {
                     if (L.nsig > 0.0f) {
                         const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
                         uint32_t cw;
@@ -271,7 +282,46 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                 const uint32_t npend = (uint32_t)__popcll(pm);
                 const uint64_t am = __ballot(have && !stop && !pending);
                 if (npend >= A.coll_batch || am == 0 || r + 1 == A.period) {
-                    if (pending) {
+                    if (pending && COMPAT) {
+                        if constexpr (!LEGACY) {
+                            const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
+                            const float u1 = fmaxf(xo_u01f(xo_next(L.xs0, L.xs1, L.xs2, L.xs3)), 1e-7f);
+                            const float u2 = xo_u01f(xo_next(L.xs0, L.xs1, L.xs2, L.xs3));
+                            const float u3 = fmaxf(xo_u01f(xo_next(L.xs0, L.xs1, L.xs2, L.xs3)), 1e-7f);
+                            const float u4 = xo_u01f(xo_next(L.xs0, L.xs1, L.xs2, L.xs3));
+                            const float r1 = sqrtf(-2.0f * log_f(u1)), r2 = sqrtf(-2.0f * log_f(u3));
+                            const float gx = L.vx - L.sgas * (r1 * cos_rev(u2));
+                            const float gy = L.vy - L.sgas * (r1 * sin_rev(u2));
+                            const float gz = L.vz - L.sgas * (r2 * cos_rev(u4));
+                            const float gl = sqrtf(fmaf(gz, gz, fmaf(gy, gy, gx * gx)));
+                            if (xo_u01f(xo_next(L.xs0, L.xs1, L.xs2, L.xs3)) * bound < gl) {
+                                const float cth = 2.0f * xo_u01f(xo_next(L.xs0, L.xs1, L.xs2, L.xs3)) - 1.0f;
+                                const float a3 = xo_u01f(xo_next(L.xs0, L.xs1, L.xs2, L.xs3));
+                                const float sn = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
+                                const float mg = L.mr * gl;
+                                L.vx = fmaf(mg, sn * cos_rev(a3), fmaf(-L.mr, gx, L.vx));
+                                L.vy = fmaf(mg, sn * sin_rev(a3), fmaf(-L.mr, gy, L.vy));
+                                L.vz = fmaf(mg, cth, fmaf(-L.mr, gz, L.vz));
+                                L.ncoll += 1;
+                            }
+                        } else {
+                            const float vm = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx)));
+                            if (vm > L.floorv) {
+                                const float cth = 2.0f * xo_u01f(xo_next(L.xs0, L.xs1, L.xs2, L.xs3)) - 1.0f;
+                                const float a3 = xo_u01f(xo_next(L.xs0, L.xs1, L.xs2, L.xs3));
+                                const float sn = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
+                                const float mv = L.mr * vm;
+                                L.vx = fmaf(mv, sn * cos_rev(a3), L.vx * L.omr);
+                                L.vy = fmaf(mv, sn * sin_rev(a3), L.vy * L.omr);
+                                L.vz = fmaf(mv, cth, L.vz * L.omr);
+                                L.ncoll += 1;
+                            }
+                            const float mfp = -log_f(fmaxf(xo_u01f(xo_next(L.xs0, L.xs1, L.xs2, L.xs3)), 1e-7f)) * L.mfpm;
+                            const float vm2 = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx)));
+                            L.cd = countdown_f(floorf(mfp / fmaxf(vm2, 1e-3f) / dtf));
+                        }
+                        pending = false;
+                    } else if (pending) {
                         if constexpr (!LEGACY) {
                             const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
                             const u32x4 w = philox(L.ion, pcur, 0, D_COLL, L.k0, L.k1);
@@ -328,6 +378,10 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
             S.fvx[id] = L.vx; S.fvy[id] = L.vy; S.fvz[id] = L.vz;
             S.steps[id] = L.st; S.ncoll[id] = L.ncoll;
             if constexpr (LEGACY) S.cd[id] = L.cd;
+            if constexpr (COMPAT) {
+                S.rng[4 * (size_t)id] = L.xs0; S.rng[4 * (size_t)id + 1] = L.xs1;
+                S.rng[4 * (size_t)id + 2] = L.xs2; S.rng[4 * (size_t)id + 3] = L.xs3;
+            }
             S.status[id] = (uint8_t)status;
             have = false;
         }
@@ -477,8 +531,13 @@ void launch_prepare(const WorkerArgs& a, hipStream_t st) { k_prepare<<<1, 64, 0,
 
 void launch_batched(const WorkerArgs& a, uint32_t grid, uint32_t block, hipStream_t st)
 {
-    if (a.B.legacy_coll) k_batched<true><<<grid, block, 0, st>>>(a);
-    else k_batched<false><<<grid, block, 0, st>>>(a);
+    if (a.B.compat) {
+        if (a.B.legacy_coll) k_batched<true, true><<<grid, block, 0, st>>>(a);
+        else k_batched<false, true><<<grid, block, 0, st>>>(a);
+    } else {
+        if (a.B.legacy_coll) k_batched<true, false><<<grid, block, 0, st>>>(a);
+        else k_batched<false, false><<<grid, block, 0, st>>>(a);
+    }
 }
 
 }  // namespace mq

@@ -135,6 +135,11 @@ __global__ __launch_bounds__(256) void k_plain(IonState S, const JobP* __restric
         const double dt = B.dt;
         int status = 0;
         u32x4 cw = {0, 0, 0, 0};
+        uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // xoshiro128** state (compat streams)
+        if (B.compat) {
+            s0 = S.rng[4 * (size_t)i]; s1 = S.rng[4 * (size_t)i + 1];
+            s2 = S.rng[4 * (size_t)i + 2]; s3 = S.rng[4 * (size_t)i + 3];
+        }
         for (uint32_t s = 0; s < nsteps; ++s) {
             int ii, jj, kk;
             double tx, ty, tz;
@@ -154,7 +159,55 @@ __global__ __launch_bounds__(256) void k_plain(IonState S, const JobP* __restric
             E = (a_rf * er[2] + J.s_dc * ed[2]) + a_ac * ea[2]; vz = fma(J.kq, E, vz); z = fma(vz, dt, z);
             const uint32_t cur = st;
             st += 1;
-            if (!B.legacy_coll) {
+            if (B.compat) {
+                // the reference's per-ion xoshiro128** stream in the draw order of
+                // channel.metal:93-113 (rand01, gauss3, iso_dir), in FP64
+                if (!B.legacy_coll) {
+                    const double bound = norm3(vx, vy, vz) + J.v_bound;
+                    if (xo_u01(xo_next(s0, s1, s2, s3)) < J.nsig_dt * bound) {
+                        const double u1 = fmax(xo_u01(xo_next(s0, s1, s2, s3)), 1e-7);
+                        const double u2 = xo_u01(xo_next(s0, s1, s2, s3));
+                        const double u3 = fmax(xo_u01(xo_next(s0, s1, s2, s3)), 1e-7);
+                        const double u4 = xo_u01(xo_next(s0, s1, s2, s3));
+                        const double r1 = sqrt(-2.0 * log_d(u1)), r2 = sqrt(-2.0 * log_d(u3));
+                        double s1n, c1n, s2n, c2n;
+                        sincos2pi(u2, s1n, c1n);
+                        sincos2pi(u4, s2n, c2n);
+                        const double gx = vx - J.s_gas * (r1 * c1n), gy = vy - J.s_gas * (r1 * s1n),
+                                     gz = vz - J.s_gas * (r2 * c2n);
+                        const double gl = norm3(gx, gy, gz);
+                        if (xo_u01(xo_next(s0, s1, s2, s3)) * bound < gl) {
+                            const double c = 2.0 * xo_u01(xo_next(s0, s1, s2, s3)) - 1.0;
+                            double sp, cp;
+                            sincos2pi(xo_u01(xo_next(s0, s1, s2, s3)), sp, cp);
+                            const double sn = sqrt(fmax(0.0, 1.0 - c * c));
+                            const double mg = J.mr * gl;
+                            vx = (vx - J.mr * gx) + mg * (sn * cp);
+                            vy = (vy - J.mr * gy) + mg * (sn * sp);
+                            vz = (vz - J.mr * gz) + mg * c;
+                            ncoll += 1;
+                        }
+                    }
+                } else {
+                    cd -= 1;
+                    if (cd == 0) {
+                        const double vm = norm3(vx, vy, vz);
+                        if (vm > J.floor_v) {
+                            const double c = 2.0 * xo_u01(xo_next(s0, s1, s2, s3)) - 1.0;
+                            double sp, cp;
+                            sincos2pi(xo_u01(xo_next(s0, s1, s2, s3)), sp, cp);
+                            const double sn = sqrt(fmax(0.0, 1.0 - c * c));
+                            const double mv = J.mr * vm;
+                            vx = vx * J.one_m_mr + mv * (sn * cp);
+                            vy = vy * J.one_m_mr + mv * (sn * sp);
+                            vz = vz * J.one_m_mr + mv * c;
+                            ncoll += 1;
+                        }
+                        const double mfp = -log_d(fmax(xo_u01(xo_next(s0, s1, s2, s3)), 1e-7)) * J.mfp_mean;
+                        cd = countdown(floor(mfp / fmax(norm3(vx, vy, vz), 1e-3) / dt));
+                    }
+                }
+            } else if (!B.legacy_coll) {
                 if (J.nsig_dt > 0.0) {
                     const double bound = norm3(vx, vy, vz) + J.v_bound;
                     if ((cur & 3) == 0 || s == 0) cw = philox(ion, cur >> 2, 0, D_CAND, J.key0, J.key1);
@@ -211,6 +264,10 @@ __global__ __launch_bounds__(256) void k_plain(IonState S, const JobP* __restric
         }
         S.x[i] = x; S.y[i] = y; S.z[i] = z; S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
         S.steps[i] = st; S.ncoll[i] = ncoll; S.cd[i] = cd; S.status[i] = (uint8_t)status;
+        if (B.compat) {
+            S.rng[4 * (size_t)i] = s0; S.rng[4 * (size_t)i + 1] = s1;
+            S.rng[4 * (size_t)i + 2] = s2; S.rng[4 * (size_t)i + 3] = s3;
+        }
         alive = status == 0;
     }
     const uint64_t m = __ballot(alive);

@@ -44,6 +44,26 @@ MQ_HD uint32_t pick(const u32x4& w, uint32_t i)
     return i == 0 ? w.a : (i == 1 ? w.b : (i == 2 ? w.c : w.d));
 }
 
+// xoshiro128** (Blackman and Vigna, "Scrambled linear pseudorandom number
+// generators", 2018): the per-ion stream of the reference's kernel
+// (next_u32, channel.metal:38-42). State s[0..3] = uint4 (x, y, z, w).
+MQ_HD uint32_t rotl32(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+MQ_HD uint32_t xo_next(uint32_t& s0, uint32_t& s1, uint32_t& s2, uint32_t& s3)
+{
+    const uint32_t r = rotl32(s1 * 5u, 7) * 9u;
+    const uint32_t t = s1 << 9;
+    s2 ^= s0;
+    s3 ^= s1;
+    s1 ^= s2;
+    s0 ^= s3;
+    s2 ^= t;
+    s3 = rotl32(s3, 11);
+    return r;
+}
+// rand01 (channel.metal:43): top 24 bits, (w >> 8) 2^-24 in [0, 1) (exact in FP32 and FP64).
+MQ_HD float xo_u01f(uint32_t w) { return (float)(w >> 8) * (1.0f / 16777216.0f); }
+MQ_HD double xo_u01(uint32_t w) { return (double)(w >> 8) * (1.0 / 16777216.0); }
+
 // Uniform in (0,1) from one 32-bit word: (w + 1/2) 2^-32 (exact in FP64).
 MQ_HD double u01(uint32_t w) { return ((double)w + 0.5) * 0x1p-32; }
 // FP32 uniform in (0,1): ((w >> 9) + 1/2) 2^-23 (exact in FP32).

@@ -19,7 +19,7 @@ struct JobP {
 struct BatchP {
     double dt, h, inv_h, lim0, lim1, lim2;
     double z_ret, z_exit, cz0, cz1;
-    int32_t nx, ny, nz, nearest, legacy_coll, pad;
+    int32_t nx, ny, nz, nearest, legacy_coll, compat;  // compat: MQ_RNG_COMPAT streams
     uint32_t max_steps;
     uint32_t njobs;
 };
@@ -40,6 +40,7 @@ struct IonState {
     uint32_t *steps, *ncoll, *cd, *job;
     uint8_t *status;
     uint8_t *region = nullptr, *last = nullptr;  // tandem batch only: domain and previous port
+    uint32_t* rng = nullptr;  // MQ_RNG_COMPAT only: xoshiro128** state, 4 words per ion [4 i + k]
 };
 
 // Per-job accumulators (u64): [0] n, [1..5] crashed, returned, transmitted,

@@ -245,6 +245,7 @@ struct mq_tandem {
         if (ph.legacy_collisions || ph.legacy_criteria)
             throw Err(-1, "the tandem batch runs the thermal bath with geometric planes (no legacy collisions/criteria)");
         if (path != MQ_PATH_BATCHED && path != MQ_PATH_PLAIN) throw Err(-1, "unknown path");
+        if (ph.rng != MQ_RNG_PHILOX) throw Err(-1, "the tandem batch uses the Philox streams (rng = 0)");
         for (int k = 0; k < 9; ++k)
             if (!(d->tones[k] >= 0.0) || !std::isfinite(d->tones[k])) throw Err(-1, "tones must be finite and >= 0");
         std::vector<JobP> jp(njobs);

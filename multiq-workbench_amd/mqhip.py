@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(HERE, "libmqhip.so")
 ALIVE, CRASHED, RETURNED, TRANSMITTED, LEFT, TIMEOUT = range(6)
 PATH_BATCHED, PATH_PLAIN = 0, 1
 INJ_REFERENCE, INJ_GAUSSIAN = 0, 1
+RNG_PHILOX, RNG_COMPAT = 0, 1
 
 AMU = 1.6605390666e-27
 QE = 1.602176634e-19
@@ -45,6 +46,7 @@ class Physics(C.Structure):
         ("legacy_injection", C.c_int32), ("legacy_criteria", C.c_int32), ("injection", C.c_int32),
         ("pressure_pa", C.c_double), ("temp_k", C.c_double), ("dt", C.c_double), ("t_max", C.c_double),
         ("r_inject_mm", C.c_double), ("inj_sigma_mm", C.c_double), ("max_steps", C.c_uint64),
+        ("rng", C.c_int32), ("pad_", C.c_int32),
     ]
 
     @staticmethod
@@ -246,6 +248,8 @@ SIGNATURES = {
                             C.POINTER(C.c_double)]),
     "mq_fields_build": (C.c_int, [C.POINTER(Geom), C.c_double, C.c_char_p, C.POINTER(ChannelFields)]),
     "mq_field_cache_name": (C.c_int, [C.POINTER(Geom), C.c_int32, C.c_char_p, C.c_size_t]),
+    "mq_inject_reference": (C.c_int, [C.POINTER(Fields), C.POINTER(Job), C.POINTER(Physics), C.POINTER(IonSoA),
+                                      C.POINTER(C.c_uint32)]),
     "mq_gather_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
     "mq_tandem_create": (C.c_int, [C.POINTER(Fields), C.POINTER(Fields), C.POINTER(TandemGeom),
                                    C.POINTER(C.c_void_p)]),
@@ -430,6 +434,24 @@ class Engine:
         t = Tune()
         _check(load().mq_get_tune(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
+
+
+def inject_reference(grid: dict, j: Job, ph: Physics) -> dict:
+    """Engine::inject (channel.rs:342-374) with the reference's xorshift64 stream, on the host:
+    initial states of every ion of one job, plus their xoshiro128** seed words (shape (n, 4))."""
+    n = int(j.n)
+    o = {k: np.zeros(n, np.float64) for k in ("x", "y", "z", "vx", "vy", "vz", "phi_ac0")}
+    o["cd"] = np.zeros(n, np.uint32)
+    o["steps"] = np.zeros(n, np.uint32)
+    o["ncoll"] = np.zeros(n, np.uint32)
+    o["status"] = np.zeros(n, np.uint8)
+    rng = np.zeros((n, 4), np.uint32)
+    f, keep = make_fields(grid)
+    _check(load().mq_inject_reference(C.byref(f), C.byref(j), C.byref(ph), C.byref(soa_of(o)),
+                                      rng.ctypes.data_as(C.POINTER(C.c_uint32))))
+    del keep
+    o["rng"] = rng
+    return o
 
 
 def summarize(status, ncoll, steps, job, njobs: int, dt: float) -> list[Outcome]:

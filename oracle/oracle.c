@@ -175,7 +175,7 @@ typedef struct {
     double dt, inv_h, lim[3];
     double z_ret, z_exit, cz0, cz1;
     uint64_t max_steps;
-    int nearest, legacy_coll;
+    int nearest, legacy_coll, compat;
 } or_ctx;
 
 /* channel.rs:376-381 */
@@ -237,6 +237,7 @@ static void ctx_init(or_ctx* C, const mq_fields* f, const double* dcc, const mq_
     C->max_steps = max_steps_of(ph);
     C->nearest = ph->legacy_fields != 0;
     C->legacy_coll = ph->legacy_collisions != 0;
+    C->compat = ph->rng == MQ_RNG_COMPAT;
 }
 
 /* --------------------------------------------------------------- inject */
@@ -305,6 +306,102 @@ void or_inject(const mq_fields* f, const mq_job* job, const mq_physics* ph, uint
     o->status = MQ_ALIVE;
 }
 
+/* ------------------------------------------------ reference streams (COMPAT) */
+
+uint64_t or_xorshift64(uint64_t* s)
+{
+    uint64_t x = *s;
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    *s = x;
+    return x;
+}
+
+uint32_t or_xoshiro128ss(uint32_t s[4])
+{
+    const uint32_t r = ((s[1] * 5u) << 7 | (s[1] * 5u) >> 25) * 9u;
+    const uint32_t t = s[1] << 9;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = (s[3] << 11) | (s[3] >> 21);
+    return r;
+}
+
+static double xs_u(uint64_t* s) { return (double)(or_xorshift64(s) >> 11) / 9007199254740992.0; }
+static double xo_u(uint32_t s[4]) { return (double)(or_xoshiro128ss(s) >> 8) / 16777216.0; }
+
+/* channel.rs:342-374: draws per ion in order (model draws, AC phase, first
+ * free path, four seed words), stored as the reference's ChIon. */
+void or_inject_reference(const mq_fields* f, const mq_job* job, const mq_physics* ph, or_ion* out)
+{
+    const double PI_ = 3.14159265358979323846;
+    uint64_t st = job->seed * 0x9E3779B97F4A7C15ull ^ 0xD1B54A32D192ED03ull;
+    const double m = job->sp.mass * AMU;
+    const double mfp_mean = legacy_mfp_mean(&job->sp, job->pressure_pa > 0.0 ? job->pressure_pa : ph->pressure_pa);
+    for (uint64_t i = 0; i < job->n; ++i) {
+        double pos[3], vel[3];
+        if (ph->injection == MQ_INJ_GAUSSIAN) {
+            double sig = ph->inj_sigma_mm * 1e-3, s = sqrt(KB * ph->temp_k / m);
+            double r = sqrt(-2.0 * log(fmax(xs_u(&st), 1e-300))), th = 2.0 * PI_ * xs_u(&st);
+            pos[0] = f->axis_x + sig * (r * cos(th));
+            pos[1] = f->axis_y + sig * (r * sin(th));
+            pos[2] = f->z_inject;
+            r = sqrt(-2.0 * log(fmax(xs_u(&st), 1e-300)));
+            th = 2.0 * PI_ * xs_u(&st);
+            vel[0] = s * (r * cos(th));
+            vel[1] = s * (r * sin(th));
+            r = sqrt(-2.0 * log(fmax(xs_u(&st), 1e-300)));
+            th = 2.0 * PI_ * xs_u(&st);
+            vel[2] = s * (r * cos(th));
+        } else if (ph->legacy_injection) {
+            double kin = 0.1 * (0.75 + 0.75 * xs_u(&st)) * job->sp.z;
+            double vin = sqrt(2.0 * kin * QE / m);
+            double ang = (-10.0 + 20.0 * xs_u(&st)) * (PI_ / 180.0);
+            double vxy = 0.707 * vin * sin(ang);
+            pos[0] = f->axis_x; pos[1] = f->axis_y; pos[2] = f->z_inject;
+            vel[0] = vxy; vel[1] = vxy; vel[2] = vin * cos(ang);
+        } else {
+            double s = sqrt(KB * ph->temp_k / m);
+            double r = ph->r_inject_mm * 1e-3 * sqrt(xs_u(&st));
+            double th = 2.0 * PI_ * xs_u(&st);
+            double vz = s * sqrt(-2.0 * log(fmax(xs_u(&st), 1e-300)));
+            pos[0] = f->axis_x + r * cos(th);
+            pos[1] = f->axis_y + r * sin(th);
+            pos[2] = f->z_inject;
+            double u1 = xs_u(&st), u2 = xs_u(&st);
+            vel[0] = s * (sqrt(-2.0 * log(fmax(u1, 1e-300))) * cos(2.0 * PI_ * u2));
+            double u3 = xs_u(&st), u4 = xs_u(&st);
+            vel[1] = s * (sqrt(-2.0 * log(fmax(u3, 1e-300))) * cos(2.0 * PI_ * u4));
+            vel[2] = vz;
+        }
+        double phase = 2.0 * PI_ * xs_u(&st);
+        double vmag = sqrt(vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+        double first_mfp = -log(fmax(xs_u(&st), 1e-7)) * mfp_mean / 10.0;
+        double steps0 = floor(first_mfp / vmag / ph->dt) + 1.0;
+        or_ion* o = &out[i];
+        for (int d = 0; d < 3; ++d) {
+            double q = pos[d] * MQ_POS_SCALE;
+            int32_t p = !(q == q) ? 0 : q >= 2147483647.0 ? 2147483647 : q <= -2147483648.0 ? INT32_MIN : (int32_t)q;
+            o->x[d] = (double)p / MQ_POS_SCALE;
+            o->v[d] = (double)(float)vel[d];
+        }
+        o->phi_ac = (double)(float)(phase / (2.0 * PI_));
+        float s0 = (float)steps0;
+        o->cd = !(s0 >= 1.0f) ? 1u : (s0 >= 4294967040.0f ? 4294967040u : (uint32_t)s0);
+        o->rng[0] = (uint32_t)or_xorshift64(&st) | 1u;
+        o->rng[1] = (uint32_t)or_xorshift64(&st);
+        o->rng[2] = (uint32_t)or_xorshift64(&st);
+        o->rng[3] = (uint32_t)or_xorshift64(&st);
+        o->steps = 0;
+        o->ncoll = 0;
+        o->status = MQ_ALIVE;
+    }
+}
+
 /* ---------------------------------------------------------------- advance */
 
 static void advance(const or_ctx* C, const or_jp* J, or_ion* s, uint32_t ion)
@@ -330,7 +427,47 @@ static void advance(const or_ctx* C, const or_jp* J, or_ion* s, uint32_t ion)
         const uint32_t st = n;
         n += 1;
         /* collisions (channel.metal:93-113) */
-        if (!C->legacy_coll) {
+        if (C->compat) {
+            /* the reference's xoshiro128** draws: rand01, gauss3, rand01, iso_dir */
+            if (!C->legacy_coll) {
+                double bound = norm3(v) + J->v_bound;
+                if (xo_u(s->rng) < J->nsig_dt * bound) {
+                    double u1 = fmax(xo_u(s->rng), 1e-7), u2 = xo_u(s->rng);
+                    double u3 = fmax(xo_u(s->rng), 1e-7), u4 = xo_u(s->rng);
+                    double r1 = sqrt(-2.0 * or_log(u1)), r2 = sqrt(-2.0 * or_log(u3));
+                    double s1, c1, s2, c2;
+                    or_sincos2pi(u2, &s1, &c1);
+                    or_sincos2pi(u4, &s2, &c2);
+                    double gv[3] = {v[0] - J->s_gas * (r1 * c1), v[1] - J->s_gas * (r1 * s1), v[2] - J->s_gas * (r2 * c2)};
+                    double gl = norm3(gv);
+                    if (xo_u(s->rng) * bound < gl) {
+                        double c = 2.0 * xo_u(s->rng) - 1.0, sp, cp;
+                        or_sincos2pi(xo_u(s->rng), &sp, &cp);
+                        double sn = sqrt(fmax(0.0, 1.0 - c * c));
+                        double nn[3] = {sn * cp, sn * sp, c};
+                        double mg = J->mr * gl;
+                        for (int d = 0; d < 3; ++d) v[d] = (v[d] - J->mr * gv[d]) + mg * nn[d];
+                        ncoll += 1;
+                    }
+                }
+            } else {
+                cd -= 1;
+                if (cd == 0) {
+                    double vm = norm3(v);
+                    if (vm > J->floor_v) {
+                        double c = 2.0 * xo_u(s->rng) - 1.0, sp, cp;
+                        or_sincos2pi(xo_u(s->rng), &sp, &cp);
+                        double sn = sqrt(fmax(0.0, 1.0 - c * c));
+                        double nn[3] = {sn * cp, sn * sp, c};
+                        double mv = J->mr * vm;
+                        for (int d = 0; d < 3; ++d) v[d] = v[d] * J->one_m_mr + mv * nn[d];
+                        ncoll += 1;
+                    }
+                    double mfp = -or_log(fmax(xo_u(s->rng), 1e-7)) * J->mfp_mean;
+                    cd = cd_from(floor(mfp / fmax(norm3(v), 1e-3) / dt));
+                }
+            }
+        } else if (!C->legacy_coll) {
             if (J->nsig_dt > 0.0) {
                 double bound = norm3(v) + J->v_bound;
                 uint32_t w[4];
@@ -433,7 +570,7 @@ static void* worker(void* arg)
         for (size_t i = a; i < b; ++i) {
             while (j + 1 < W->njobs && W->off[j + 1] <= i) ++j;
             uint32_t ion = (uint32_t)(i - W->off[j]) + W->i0;
-            or_inject(W->f, &W->jobs[j], W->ph, ion, &W->out[i]);
+            if (!W->C->compat) or_inject(W->f, &W->jobs[j], W->ph, ion, &W->out[i]);
             advance(W->C, &W->jp[j], &W->out[i], ion);
             if (W->out_job) W->out_job[i] = (uint32_t)j;
         }
@@ -454,6 +591,14 @@ static int run_common(const mq_fields* f, const mq_job* jobs, size_t njobs, cons
     for (size_t j = 0; j < njobs; ++j) {
         job_params(&jobs[j], ph, &jp[j]);
         off[j + 1] = off[j] + jobs[j].n;
+    }
+    if (C.compat) {
+        /* the reference's injection stream is sequential per job: inject on one thread first */
+        if (i0 != 0 || total_override) {
+            free(jp); free(off); free(dcc);
+            return -2; /* a sub-range of a compat stream needs the whole job */
+        }
+        for (size_t j = 0; j < njobs; ++j) or_inject_reference(f, &jobs[j], ph, out + off[j]);
     }
     or_work W;
     W.C = &C; W.jobs = jobs; W.jp = jp; W.off = off; W.njobs = njobs;

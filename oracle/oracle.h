@@ -20,7 +20,8 @@
  * Deliberate, specified differences from the reference (the new design):
  *   - Random numbers: Philox4x32-10 keyed by the job seed with counter
  *     (ion index in job, step, sub-block, domain) instead of xorshift64 +
- *     xoshiro128** (channel.rs:343-373, channel.metal:38-44).
+ *     xoshiro128** (channel.rs:343-373, channel.metal:38-44); with
+ *     rng = MQ_RNG_COMPAT the reference's own streams and draw order.
  *   - FP64 position and velocity (the reference stores 62.5 pm fixed-point
  *     positions and FP32 velocities, channel.metal:31-36), RF/AC phases
  *     computed from the step counter in FP64 (the reference accumulates them
@@ -54,9 +55,19 @@ typedef struct {
     double phi_ac;
     uint32_t steps, ncoll, cd;
     uint8_t status;
+    uint32_t rng[4]; /* xoshiro128** state (MQ_RNG_COMPAT) */
 } or_ion;
 
 void or_inject(const mq_fields* f, const mq_job* job, const mq_physics* ph, uint32_t ion, or_ion* out);
+
+/* Engine::inject (channel.rs:342-374) with the reference's xorshift64 stream:
+ * all job->n ions of one job, in order (the stream is sequential). */
+void or_inject_reference(const mq_fields* f, const mq_job* job, const mq_physics* ph, or_ion* out);
+
+/* The generators of MQ_RNG_COMPAT: one xorshift64 step (13, 7, 17) and one
+ * xoshiro128** output with its state update. */
+uint64_t or_xorshift64(uint64_t* s);
+uint32_t or_xoshiro128ss(uint32_t s[4]);
 
 /* Runs every ion of every job to completion with `nthreads` CPU threads.
  * out_* are arrays of sum(n) entries, job order. Returns 0 on success. */

@@ -21,7 +21,8 @@ import mqhip  # noqa: E402  (C struct definitions of include/mq_engine.h)
 
 class OrIon(C.Structure):
     _fields_ = [("x", C.c_double * 3), ("v", C.c_double * 3), ("phi_ac", C.c_double),
-                ("steps", C.c_uint32), ("ncoll", C.c_uint32), ("cd", C.c_uint32), ("status", C.c_uint8)]
+                ("steps", C.c_uint32), ("ncoll", C.c_uint32), ("cd", C.c_uint32), ("status", C.c_uint8),
+                ("rng", C.c_uint32 * 4)]
 
 
 _lib = None
@@ -56,6 +57,12 @@ def load() -> C.CDLL:
                                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.or_sample.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int,
                               C.POINTER(C.c_double)]
+    lib.or_xorshift64.argtypes = [C.POINTER(C.c_uint64)]
+    lib.or_xorshift64.restype = C.c_uint64
+    lib.or_xoshiro128ss.argtypes = [C.POINTER(C.c_uint32)]
+    lib.or_xoshiro128ss.restype = C.c_uint32
+    lib.or_inject_reference.argtypes = [C.POINTER(mqhip.Fields), C.POINTER(mqhip.Job), C.POINTER(mqhip.Physics),
+                                        C.POINTER(OrIon)]
     lib.or_geom_default.argtypes = [C.POINTER(mqhip.Geom)]
     lib.or_geom_nz.argtypes = [C.POINTER(mqhip.Geom)]
     lib.or_geom_nz.restype = C.c_int
@@ -101,7 +108,7 @@ def _ions_to_dict(ions, job=None) -> dict:
         "x": a["x"][:, 0].copy(), "y": a["x"][:, 1].copy(), "z": a["x"][:, 2].copy(),
         "vx": a["v"][:, 0].copy(), "vy": a["v"][:, 1].copy(), "vz": a["v"][:, 2].copy(),
         "phi_ac": a["phi_ac"].copy(), "steps": a["steps"].copy(), "ncoll": a["ncoll"].copy(),
-        "cd": a["cd"].copy(), "status": a["status"].copy(),
+        "cd": a["cd"].copy(), "status": a["status"].copy(), "rng": a["rng"].copy(),
     }
     if job is not None:
         out["job"] = np.ctypeslib.as_array(job).copy()
@@ -139,6 +146,27 @@ def inject(grid: dict, job, ph, ion: int) -> dict:
     load().or_inject(C.byref(f), C.byref(job), C.byref(ph), ion, C.byref(o))
     del keep
     return {"x": list(o.x), "v": list(o.v), "phi_ac": o.phi_ac, "cd": o.cd, "status": o.status}
+
+
+def xorshift64(state: int, n: int) -> list[int]:
+    s = C.c_uint64(state)
+    return [load().or_xorshift64(C.byref(s)) for _ in range(n)]
+
+
+def xoshiro128ss(state, n: int) -> list[int]:
+    s = (C.c_uint32 * 4)(*state)
+    return [load().or_xoshiro128ss(s) for _ in range(n)]
+
+
+def inject_reference(grid: dict, job, ph) -> dict:
+    """Engine::inject with the reference's xorshift64 stream: every ion of one job."""
+    f, keep = mqhip.make_fields(grid)
+    n = int(job.n)
+    ions = (OrIon * max(n, 1))()
+    load().or_inject_reference(C.byref(f), C.byref(job), C.byref(ph), ions)
+    del keep
+    d = _ions_to_dict(ions)
+    return {k: v[:n] for k, v in d.items()}
 
 
 def summarize(res: dict, njobs: int, dt: float) -> list:

@@ -118,3 +118,26 @@ def test_fig4_full_kick_transmits_nothing(default_engine):
     out = default_engine.run_batch(jobs, [0, 5, -5, 0], ph, mqhip.PATH_BATCHED)
     assert sum(o.transmitted for o in out) == 0
     assert sum(o.n for o in out) == 25000
+
+
+@pytest.mark.parametrize("legacy_force", [0, 1])
+def test_c1_on_the_solved_channel_matches_oracle(default_engine, oracle_mod, legacy_force):
+    """The Fig-4 plain configuration (m = 499 Da, z = +1; 100 V RF at 500 kHz, 0.5 V AC at 100 kHz, 5 V on the
+    plate, N2 1 mTorr, dt 100 ns, cap 20,000 steps, Gaussian 0.1 mm spot, 300 K) on the solved reference channel
+    (Geom::default at 0.1 mm): the FP64 plain path is bit-identical to the oracle ion by ion, in full and half kick."""
+    grid = mqhip.channel_grid(mqhip.Geom.default(), tol=1e-7)
+    ph = mqhip.Physics.correct(0.133322).copy(dt=1e-7, max_steps=20000, injection=mqhip.INJ_GAUSSIAN,
+                                              inj_sigma_mm=0.1, legacy_force=legacy_force)
+    jobs = [mqhip.job(499.0, 1.0, 230.0, 100.0, 500e3, 0.5, 100e3, 200, 1499)]
+    dc = [0.0, 5.0, 0.0, 0.0]
+    say(f"[gpu] C1 on the solved channel, legacy_force={legacy_force}")
+    got = default_engine.run_batch_ions(jobs, dc, ph, mqhip.PATH_PLAIN)
+    want = oracle_mod.run_batch(grid, jobs, dc, ph, 8)
+    for k in ("status", "steps", "ncoll", "x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(got[k], want[k]), k
+    # the FP32 batched path on 1000 ions: class counts within 3 sigma of the FP64 path
+    jobs = [mqhip.job(499.0, 1.0, 230.0, 100.0, 500e3, 0.5, 100e3, 1000, 1499)]
+    b = default_engine.run_batch(jobs, dc, ph, mqhip.PATH_BATCHED)[0]
+    p = default_engine.run_batch(jobs, dc, ph, mqhip.PATH_PLAIN)[0]
+    for k in ("crashed", "returned", "transmitted", "left", "timeout"):
+        assert abs(pull(getattr(b, k), 1000, getattr(p, k), 1000)) <= 3.0 or getattr(b, k) == getattr(p, k), k
