@@ -197,6 +197,31 @@ class ScanPoint(C.Structure):
     _fields_ = [("f_khz", C.c_double), ("transmitted", C.c_double), ("crashed", C.c_double), ("n", C.c_uint64)]
 
 
+class LegacyOpts(C.Structure):
+    """mq_legacy_opts: the Java sketch's potential settings (scl nodes/mm, stop threshold %, sweep cap)."""
+    _fields_ = [("scl", C.c_float), ("precision_pct", C.c_float), ("max_iter", C.c_uint32), ("pad_", C.c_int32)]
+
+    @classmethod
+    def default(cls, **kw) -> "LegacyOpts":
+        o = cls()
+        load().mq_legacy_default(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
+class LegacyStats(C.Structure):
+    _fields_ = [("sweeps", C.c_uint32), ("dif_pct", C.c_float), ("check", C.c_int32 * 3), ("pad_", C.c_int32),
+                ("node_updates", C.c_uint64), ("ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {"sweeps": self.sweeps, "dif_pct": self.dif_pct, "check": list(self.check),
+                "node_updates": self.node_updates, "ms": self.ms}
+
+
+LEGACY_RF, LEGACY_DC, LEGACY_AC = 0, 1, 2
+
+
 # Exported symbols of include/mq_engine.h with their ctypes signatures.
 SIGNATURES = {
     "mq_last_error": (C.c_char_p, []),
@@ -261,6 +286,16 @@ SIGNATURES = {
                                      C.POINTER(IonSoA), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
                                      C.POINTER(C.c_uint32)]),
     "mq_tandem_last_stats": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
+    "mq_legacy_default": (None, [C.POINTER(LegacyOpts)]),
+    "mq_legacy_dims": (C.c_int, [C.POINTER(LegacyOpts), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "mq_legacy_rasterize": (C.c_int, [C.POINTER(LegacyOpts), C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_uint8)]),
+    "mq_legacy_sweeps": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_uint8),
+                                   C.POINTER(C.c_int32), C.c_float, C.c_uint32, C.POINTER(LegacyStats)]),
+    "mq_legacy_potential": (C.c_int, [C.POINTER(LegacyOpts), C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_uint8),
+                                      C.POINTER(LegacyStats)]),
+    "mq_legacy_export": (C.c_int, [C.POINTER(LegacyOpts), C.c_char_p, C.POINTER(LegacyStats)]),
+    "mq_fields_from_theirs": (C.c_int, [C.c_char_p, C.POINTER(ChannelFields)]),
 }
 
 _lib = None
@@ -578,6 +613,93 @@ def channel_grid(g: Geom, tol: float = 1e-7, cache_dir: str | None = None, poten
               "z_exit_legacy", "crash_z0_legacy", "crash_z1_legacy"):
         grid[k] = getattr(f, k)
     grid["stats"] = [{k: getattr(out.stats[b], k) for k, _ in SolveStats._fields_} for b in range(6)]
+    grid["layers_mm"] = lay
+    if potentials:
+        grid["phi"] = phis
+    return grid
+
+
+# ------------------------------------------------------------------ legacy Java potentials
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u8ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def legacy_dims(opts: LegacyOpts | None = None):
+    """(nx, ny, nz, check) of the Java sketch's grid."""
+    o = opts or LegacyOpts.default()
+    nx, ny, nz = C.c_int32(), C.c_int32(), C.c_int32()
+    ch = (C.c_int32 * 3)()
+    _check(load().mq_legacy_dims(C.byref(o), C.byref(nx), C.byref(ny), C.byref(nz), ch))
+    return nx.value, ny.value, nz.value, tuple(ch)
+
+
+def legacy_rasterize(basis: int, opts: LegacyOpts | None = None):
+    """Start state of one basis on the GPU: (phi float32, electrode uint8), shape (nz, ny, nx)."""
+    o = opts or LegacyOpts.default()
+    nx, ny, nz, _ = legacy_dims(o)
+    phi = np.zeros((nz, ny, nx), np.float32)
+    el = np.zeros((nz, ny, nx), np.uint8)
+    _check(load().mq_legacy_rasterize(C.byref(o), basis, _fptr(phi), _u8ptr(el)))
+    return phi, el
+
+
+def legacy_sweeps(phi: np.ndarray, electrode: np.ndarray, check, precision_pct: float, max_iter: int):
+    """The sketch's sweeps on the GPU on any grid (nz, ny, nx); returns (phi, stats dict)."""
+    nz, ny, nx = phi.shape
+    p = np.ascontiguousarray(phi, np.float32).copy()
+    e = np.ascontiguousarray(electrode, np.uint8)
+    ch = (C.c_int32 * 3)(*check)
+    st = LegacyStats()
+    _check(load().mq_legacy_sweeps(nx, ny, nz, _fptr(p), _u8ptr(e), ch, precision_pct, max_iter, C.byref(st)))
+    return p, st.as_dict()
+
+
+def legacy_potential(basis: int, opts: LegacyOpts | None = None):
+    """Rasterize + sweeps of one basis: (phi, electrode, stats dict), shape (nz, ny, nx)."""
+    o = opts or LegacyOpts.default()
+    nx, ny, nz, _ = legacy_dims(o)
+    phi = np.zeros((nz, ny, nx), np.float32)
+    el = np.zeros((nz, ny, nx), np.uint8)
+    st = LegacyStats()
+    _check(load().mq_legacy_potential(C.byref(o), basis, _fptr(phi), _u8ptr(el), C.byref(st)))
+    return phi, el, st.as_dict()
+
+
+def legacy_export(directory: str, opts: LegacyOpts | None = None) -> list:
+    """Writes eject_{RF,DC,AC}.f32 and eject_{RF,AC}_electrode.u8 to directory; returns the stats per basis."""
+    o = opts or LegacyOpts.default()
+    st = (LegacyStats * 3)()
+    _check(load().mq_legacy_export(C.byref(o), directory.encode(), st))
+    return [s.as_dict() for s in st]
+
+
+def fields_from_theirs(directory: str, potentials: bool = False) -> dict:
+    """Fields::from_theirs (channel.rs:187-202): a grid dict for Engine() from the five exported files."""
+    g = Geom.default()
+    nx, ny, nz, lay = g.dims()
+    n = nx * ny * nz
+    grid = {"rf": np.zeros(3 * n), "ac": np.zeros(3 * n), "dc": [np.zeros(3 * n) for _ in range(4)],
+            "mask": np.zeros(n, np.uint8)}
+    out = ChannelFields()
+    out.rf, out.ac = _dptr(grid["rf"]), _dptr(grid["ac"])
+    for b in range(4):
+        out.dc[b] = _dptr(grid["dc"][b])
+    phis = [np.zeros(n) for _ in range(6)] if potentials else []
+    for b, p in enumerate(phis):
+        out.phi[b] = _dptr(p)
+    out.mask = _u8ptr(grid["mask"])
+    _check(load().mq_fields_from_theirs(directory.encode(), C.byref(out)))
+    f = Fields()
+    _check(load().mq_geom_fields_params(C.byref(g), C.byref(f)))
+    for k in ("nx", "ny", "nz", "h", "axis_x", "axis_y", "z_inject", "z_return", "z_exit", "z_return_legacy",
+              "z_exit_legacy", "crash_z0_legacy", "crash_z1_legacy"):
+        grid[k] = getattr(f, k)
     grid["layers_mm"] = lay
     if potentials:
         grid["phi"] = phis

@@ -37,6 +37,7 @@
 #include "../include/mq_engine.h"
 #include "../include/mq_fieldsolve.h"
 #include "../include/mq_tandem.h"
+#include "../include/mq_legacy.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -123,6 +124,27 @@ int or_laplace_sor(int nx, int ny, int nz, const uint8_t* mask, const double* fi
 
 /* efield (channel.rs:168-184): e[3N] = -grad(phi). */
 void or_efield(const double* phi, int nx, int ny, int nz, double h, double* e);
+
+/* ---- the Java sketch's potential export (oracle_legacy.c) ---- */
+
+/* Grid size and check point node of the ejection sketch at scl nodes/mm. */
+int or_legacy_dims(float scl, int* nx, int* ny, int* nz, int check[3]);
+
+/* define_RF / define_DC / define_AC (basis MQ_LEGACY_*) of every node:
+ * phi[N] and el[N] (1 = electrode), node-major i + nx (j + ny k). */
+void or_legacy_rasterize(float scl, int basis, float* phi, uint8_t* el);
+
+/* One sequential sweep of calculatePotential (z outer, x, y inner). */
+void or_legacy_sweep(int nx, int ny, int nz, float* phi, const uint8_t* el);
+
+/* The stop rule's change |100 ((prev + 1e-10) - now) / (prev + 1e-10)|. */
+float or_legacy_dif(float prev, float now);
+
+/* calculatePotential's loop: sweeps while nit < max_iter and dif > precision;
+ * returns the sweep count, the last dif, and (trace != NULL) the check point
+ * value after every sweep. */
+uint32_t or_legacy_sweeps(int nx, int ny, int nz, float* phi, const uint8_t* el, const int check[3], float precision,
+                          uint32_t max_iter, float* dif_out, float* trace);
 
 #ifdef __cplusplus
 }

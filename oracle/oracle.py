@@ -80,6 +80,17 @@ def load() -> C.CDLL:
                                   C.POINTER(mqhip.Physics), C.c_int, C.POINTER(OrIon), u32p,
                                   C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64)]
+    f32p = C.POINTER(C.c_float)
+    u8p = C.POINTER(C.c_uint8)
+    lib.or_legacy_dims.argtypes = [C.c_float, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int)]
+    lib.or_legacy_rasterize.argtypes = [C.c_float, C.c_int, f32p, u8p]
+    lib.or_legacy_sweep.argtypes = [C.c_int, C.c_int, C.c_int, f32p, u8p]
+    lib.or_legacy_dif.argtypes = [C.c_float, C.c_float]
+    lib.or_legacy_dif.restype = C.c_float
+    lib.or_legacy_sweeps.argtypes = [C.c_int, C.c_int, C.c_int, f32p, u8p, C.POINTER(C.c_int), C.c_float,
+                                     C.c_uint32, f32p, f32p]
+    lib.or_legacy_sweeps.restype = C.c_uint32
     _lib = lib
     return lib
 
@@ -256,6 +267,55 @@ def efield(phi, h):
     e = np.zeros(p.shape + (3,))
     load().or_efield(_dp(p), nx, ny, nz, h, _dp(e))
     return e
+
+
+# ---------------------------------------------------------------- the Java sketch's potentials
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u8(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def legacy_dims(scl: float = 10.0):
+    nx, ny, nz = C.c_int(), C.c_int(), C.c_int()
+    ch = (C.c_int * 3)()
+    load().or_legacy_dims(scl, C.byref(nx), C.byref(ny), C.byref(nz), ch)
+    return nx.value, ny.value, nz.value, tuple(ch)
+
+
+def legacy_rasterize(basis: int, scl: float = 10.0):
+    """define_RF / define_DC / define_AC of every node: (phi float32, electrode uint8), shape (nz, ny, nx)."""
+    nx, ny, nz, _ = legacy_dims(scl)
+    phi = np.zeros((nz, ny, nx), np.float32)
+    el = np.zeros((nz, ny, nx), np.uint8)
+    load().or_legacy_rasterize(scl, basis, _fp(phi), _u8(el))
+    return phi, el
+
+
+def legacy_sweep(phi: np.ndarray, el: np.ndarray) -> None:
+    """One sequential sweep in place."""
+    nz, ny, nx = phi.shape
+    assert phi.dtype == np.float32 and phi.flags.c_contiguous and el.dtype == np.uint8
+    load().or_legacy_sweep(nx, ny, nz, _fp(phi), _u8(el))
+
+
+def legacy_dif(prev: float, now: float) -> float:
+    return load().or_legacy_dif(prev, now)
+
+
+def legacy_sweeps(phi: np.ndarray, el: np.ndarray, check, precision: float, max_iter: int):
+    """calculatePotential's loop on a copy of phi: (phi, sweeps, last dif, check point trace)."""
+    nz, ny, nx = phi.shape
+    p = np.ascontiguousarray(phi, np.float32).copy()
+    e = np.ascontiguousarray(el, np.uint8)
+    ch = (C.c_int * 3)(*check)
+    dif = C.c_float()
+    trace = np.zeros(max(max_iter, 1), np.float32)
+    nit = load().or_legacy_sweeps(nx, ny, nz, _fp(p), _u8(e), ch, precision, max_iter, C.byref(dif), _fp(trace))
+    return p, nit, dif.value, trace[:nit]
 
 
 # ---------------------------------------------------------------- tandem batch

@@ -80,3 +80,21 @@ def test_engine_create_without_gpu_fails_cleanly():
         assert "HIP" in str(err) or "device" in str(err)
         return
     e.close()  # a GPU is present: creation succeeded
+
+
+def test_legacy_dims_are_host_side_and_match_the_oracle(oracle_mod):
+    o = mqhip.LegacyOpts.default()
+    assert (o.scl, o.precision_pct, o.max_iter) == (10.0, np.float32(0.001), 6000)
+    for scl in (10.0, 2.0, 2.5, 4.0, 7.3):
+        assert mqhip.legacy_dims(mqhip.LegacyOpts.default(scl=scl)) == oracle_mod.legacy_dims(scl)
+    with pytest.raises(mqhip.MqError, match="scl"):
+        mqhip.legacy_dims(mqhip.LegacyOpts.default(scl=-1.0))
+
+
+def test_fields_from_theirs_reports_missing_files(tmp_path):
+    with pytest.raises(mqhip.MqError, match="eject_RF.f32"):
+        mqhip.fields_from_theirs(str(tmp_path))
+    # a short file is refused with its expected size
+    (tmp_path / "eject_RF.f32").write_bytes(b"\0" * 16)
+    with pytest.raises(mqhip.MqError, match="expected 24780800 bytes"):
+        mqhip.fields_from_theirs(str(tmp_path))

@@ -1,0 +1,183 @@
+"""The legacy Java potentials regenerated on the GPU (include/mq_legacy.h)
+against the oracle's sequential restatement of the sketch, bit for bit, and the
+whole chain against the reference's own published numbers:
+
+* rasterization: every node's value and electrode flag equal to the oracle's,
+  at the sketch's size and at two smaller scales;
+* sweeps: the wavefront sweeps equal the sequential sweeps bit for bit, with
+  the same sweep count from the stop rule, on the sketch's geometry at small
+  scales and on random grids (odd sizes, check points free, on a face, at 0,
+  the cap reached, an early stop, no sweep at all);
+* full size: the three exported files reproduce the digests of the oracle's
+  run at 160 x 160 x 242 (tests/golden/legacy_potentials.json);
+* Fields::from_theirs on those files and the legacy engine reproduce the
+  published Fig. 4 columns of docs/mass-scan.csv (tests/golden/mass-scan.csv),
+  25 m/z, 1000 ions each in the reference: java_original (the sketch's own
+  integrator on these potentials) and rust_legacy (the reference's legacy
+  engine on them, channel.rs:683-708). Pooled two-proportion z per point:
+  every |z| <= 3.54 against java_original (Bonferroni over 25 points at 1 %
+  family-wise), the sum of z^2 <= 52.6 against rust_legacy (chi-square, 25
+  degrees of freedom, 0.999), |z| <= 3.54 against both at m/z 493 (the flank
+  that separates these potentials from converged ones: 343 published, about
+  230 on converged fields), the peak at m/z 497-501. Against rust_legacy
+  alone, m/z 495 sits about 3 sigma low (620 per 1000 with 20,000 ions per
+  point, 670 published; java_original 630), which a per-point 3-sigma bound
+  against that column would flag in about a third of runs at 2,000 ions per
+  point.
+"""
+import csv
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import mqhip
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+RF, DC, AC = 0, 1, 2
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("scl", [10.0, 2.5, 4.0])
+def test_rasterization_equals_oracle(oracle_mod, scl):
+    for b in (RF, DC, AC):
+        say(f"[gpu] legacy rasterization scl={scl} basis={b}")
+        phi, el = mqhip.legacy_rasterize(b, mqhip.LegacyOpts.default(scl=scl))
+        want_phi, want_el = oracle_mod.legacy_rasterize(b, scl)
+        assert np.array_equal(el, want_el)
+        assert np.array_equal(phi.view(np.uint32), want_phi.view(np.uint32))   # bits, signed zeros included
+
+
+@pytest.mark.parametrize("scl", [2.0, 2.5, 4.0])
+def test_sketch_sweeps_equal_oracle_at_small_scales(oracle_mod, scl):
+    for b in (RF, DC, AC):
+        say(f"[gpu] legacy sweeps scl={scl} basis={b}")
+        o = mqhip.LegacyOpts.default(scl=scl)
+        phi, el, st = mqhip.legacy_potential(b, o)
+        phi0, el0 = oracle_mod.legacy_rasterize(b, scl)
+        _, _, _, check = oracle_mod.legacy_dims(scl)
+        want, nit, dif, _ = oracle_mod.legacy_sweeps(phi0, el0, check, 0.001, 6000)
+        assert st["sweeps"] == nit and st["dif_pct"] == np.float32(dif)
+        assert tuple(st["check"]) == check
+        assert np.array_equal(phi.view(np.uint32), want.view(np.uint32))
+        assert st["node_updates"] == nit * int(((el0[1:-1, 1:-1, 1:-1]) == 0).sum())
+
+
+def _random_problem(seed, shape):
+    rng = np.random.default_rng(seed)
+    el = (rng.random(shape) < 0.12).astype(np.uint8)
+    phi = np.where(el == 1, rng.uniform(-1, 1, shape), 0.0).astype(np.float32)
+    for s in (np.s_[0], np.s_[-1]):
+        phi[s] = rng.uniform(-1, 1, phi[s].shape)
+        phi[:, s] = rng.uniform(-1, 1, phi[:, s].shape)
+        phi[:, :, s] = rng.uniform(-1, 1, phi[:, :, s].shape)
+    return phi, el
+
+
+CASES = [
+    # (seed, shape (nz, ny, nx), check (i, j, k), precision, max_iter, what)
+    (1, (9, 10, 11), (5, 4, 3), 0.001, 3000, "odd sizes, free check point"),
+    (2, (12, 7, 16), (3, 3, 8), 0.5, 3000, "early stop, rerun from the start state"),
+    (3, (10, 11, 9), (4, 5, 6), 0.0, 57, "cap reached, drain"),
+    (4, (8, 8, 8), (0, 3, 3), 0.001, 41, "check point on a face"),
+    (5, (8, 9, 10), (4, 4, 4), 0.001, 0, "no sweep"),
+    (6, (3, 3, 3), (1, 1, 1), 0.001, 100, "one interior node"),
+    (7, (20, 17, 33), (16, 8, 10), 0.0001, 5000, "larger, tight threshold"),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[-1] for c in CASES])
+def test_wavefront_sweeps_equal_sequential_sweeps(oracle_mod, case):
+    seed, shape, check, prec, max_iter, what = case
+    phi0, el = _random_problem(seed, shape)
+    i, j, k = check
+    if what == "check point on a face":
+        phi0[k, j, i] = 0.0
+    elif max_iter:
+        el[k, j, i] = 0
+    say(f"[gpu] legacy wavefront vs sequential: {what} shape={shape} check={check}")
+    got, st = mqhip.legacy_sweeps(phi0, el, check, prec, max_iter)
+    want, nit, dif, _ = oracle_mod.legacy_sweeps(phi0, el, check, prec, max_iter)
+    assert st["sweeps"] == nit, (st, nit)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    if what == "check point on a face":
+        assert nit == max_iter
+    if what == "early stop, rerun from the start state":
+        assert 1 < nit < max_iter
+
+
+def test_bad_inputs():
+    phi = np.zeros((4, 4, 4), np.float32)
+    el = np.zeros((4, 4, 4), np.uint8)
+    with pytest.raises(mqhip.MqError, match="check point"):
+        mqhip.legacy_sweeps(phi, el, (4, 1, 1), 0.001, 10)
+    with pytest.raises(mqhip.MqError, match="basis"):
+        mqhip.legacy_potential(3)
+    with pytest.raises(mqhip.MqError, match="scl"):
+        mqhip.legacy_dims(mqhip.LegacyOpts.default(scl=0.0))
+
+
+@pytest.fixture(scope="module")
+def exported(tmp_path_factory):
+    d = str(tmp_path_factory.mktemp("theirs"))
+    say("[gpu] legacy export at 160 x 160 x 242 (three bases)")
+    stats = mqhip.legacy_export(d)
+    say(f"   sweeps {[s['sweeps'] for s in stats]}, ms {[round(s['ms'], 1) for s in stats]}")
+    return d, stats
+
+
+def test_full_size_export_matches_oracle_digests(exported):
+    d, stats = exported
+    ref = json.load(open(os.path.join(HERE, "golden", "legacy_potentials.json")))
+    for b, r in enumerate(ref["bases"]):
+        name = r["basis"]
+        raw = open(os.path.join(d, f"eject_{name}.f32"), "rb").read()
+        assert len(raw) == 160 * 160 * 242 * 4
+        assert stats[b]["sweeps"] == r["sweeps"], name
+        assert hashlib.sha256(raw).hexdigest() == r["sha256_phi"], name
+        if name != "DC":
+            m = open(os.path.join(d, f"eject_{name}_electrode.u8"), "rb").read()
+            assert hashlib.sha256(m).hexdigest() == r["sha256_electrode"], name
+
+
+def _z(a, na, b, nb):
+    """Pooled two-proportion z of counts a of na and b of nb (0 where both are 0)."""
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    p = (a + b) / (na + nb)
+    se = np.sqrt(p * (1 - p) * (1.0 / na + 1.0 / nb))
+    return np.where(se > 0, (a / na - b / nb) / np.where(se > 0, se, 1.0), 0.0)
+
+
+def test_from_theirs_reproduces_the_published_fig4_columns(exported):
+    d, _ = exported
+    grid = mqhip.fields_from_theirs(d)
+    rows = list(csv.DictReader(open(os.path.join(HERE, "golden", "mass-scan.csv"))))
+    mzs = [float(r["mz"]) for r in rows]
+    rust = np.array([int(r["rust_legacy"]) for r in rows])
+    java = np.array([int(r["java_original"]) for r in rows])
+    ph = mqhip.Physics.legacy(1e-3 * 133.322)
+    e = mqhip.Engine(grid)
+    try:
+        for path, n in ((mqhip.PATH_BATCHED, 2000), (mqhip.PATH_PLAIN, 1000)):
+            jobs = [mqhip.job(mz * 5, 5.0, 0.0, 100.0, 500e3, 0.5, 100e3, n, 1000 + int(mz)) for mz in mzs]
+            say(f"[gpu] legacy Fig. 4 scan on the regenerated potentials, path {path}, {n} ions per m/z")
+            t = np.array([o.transmitted for o in e.run_batch(jobs, [0.0, 5.0, 0.0, 0.0], ph, path)])
+            zr, zj = _z(t, n, rust, 1000), _z(t, n, java, 1000)
+            say(f"   transmitted {list(t)}")
+            say(f"   z vs rust_legacy {np.round(zr, 2).tolist()}, chi2 {float(np.sum(zr ** 2)):.1f}")
+            say(f"   z vs java_original {np.round(zj, 2).tolist()}, chi2 {float(np.sum(zj ** 2)):.1f}")
+            assert 497 <= mzs[int(np.argmax(t))] <= 501
+            assert np.all(np.abs(zj) <= 3.54), zj
+            assert float(np.sum(zr ** 2)) <= 52.6, zr
+            k493 = mzs.index(493.0)
+            assert abs(zr[k493]) <= 3.54 and abs(zj[k493]) <= 3.54
+    finally:
+        e.close()
