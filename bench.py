@@ -14,12 +14,14 @@ form on the same inputs, the outcome agreement between them, the roofline
 object of the batched worker kernel and the CPU baseline (the FP64 oracle on
 the host's cores over a bounded sample of the same jobs). Sub-objects: the
 differentiable form (gradient evaluations/s), the tandem multi-domain batch
-(1M ions) and the reference channel geometry solved on the GPU with the two
-numbers the reference recorded for it.
+(1M ions), the reference channel geometry solved on the GPU with the two
+numbers the reference recorded for it, and the legacy Java potentials
+regenerated on the GPU with the published Fig. 4 columns on them.
 """
 from __future__ import annotations
 
 import argparse
+import csv
 import json
 import os
 import sys
@@ -272,6 +274,62 @@ def reference_geometry_bench():
     return out
 
 
+LEGACY_BYTES_PER_UPDATE = 9  # read one other-colour value, write one, read one electrode flag (split layout)
+
+
+def legacy_potentials_bench(cpu_sweeps: int = 40):
+    """The legacy Java potentials regenerated on the GPU (160 x 160 x 242, the sketch's settings), the
+    published Fig. 4 columns on them, and the sketch's sequential sweep timed on one CPU core."""
+    import tempfile
+
+    out = {}
+    d = tempfile.mkdtemp(prefix="theirs_")
+    t0 = time.perf_counter()
+    st = mqhip.legacy_export(d)
+    out["export_wall_s"] = time.perf_counter() - t0
+    upd = sum(x["node_updates"] for x in st)
+    ms = sum(x["ms"] for x in st)
+    out["bases"] = {n: {"sweeps": x["sweeps"], "ms": x["ms"], "node_updates": x["node_updates"]}
+                    for n, x in zip(("RF", "DC", "AC"), st)}
+    out["node_updates_per_s"] = upd / (ms * 1e-3)
+    out["gbs_at_9B_per_update"] = upd * LEGACY_BYTES_PER_UPDATE / (ms * 1e-3) / 1e9
+    grid = mqhip.fields_from_theirs(d)
+    e = mqhip.Engine(grid)
+    mzs = [475.0 + 2 * i for i in range(25)]
+    jobs = [mqhip.job(mz * 5, 5.0, 0.0, 100.0, 500e3, 0.5, 100e3, 1000, 1000 + int(mz)) for mz in mzs]
+    outs = e.run_batch(jobs, [0, 5, 0, 0], mqhip.Physics.legacy(1e-3 * 133.322), mqhip.PATH_BATCHED)
+    e.close()
+    tr = np.array([o.transmitted for o in outs])
+    rows = list(csv.DictReader(open(os.path.join(ROOT, "tests", "golden", "mass-scan.csv"))))
+    zz = {}
+    for col in ("rust_legacy", "java_original"):
+        ref = np.array([int(r[col]) for r in rows], float)
+        pp = (tr + ref) / 2000.0
+        se = np.sqrt(pp * (1 - pp) * 2 / 1000.0)
+        z = np.where(se > 0, (tr - ref) / 1000.0 / np.where(se > 0, se, 1), 0.0)
+        zz[col] = {"max_abs_z": float(np.max(np.abs(z))), "chi2_25": float(np.sum(z ** 2))}
+    k = int(np.argmax(tr))
+    out["fig4_on_regenerated"] = {"transmitted_per_1000": tr.tolist(), "peak_mz": mzs[k], "peak": int(tr[k]),
+                                  "vs_published": zz}
+    # CPU baseline: the sketch's sequential sweep (the oracle's restatement) on one core, a bounded sample of
+    # sweeps at the full size
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # CPU baseline only
+    nx, ny, nz, check = oracle.legacy_dims(10.0)
+    phi, el = oracle.legacy_rasterize(0, 10.0)
+    free = int((el[1:-1, 1:-1, 1:-1] == 0).sum())
+    t0 = time.perf_counter()
+    for _ in range(cpu_sweeps):
+        oracle.legacy_sweep(phi, el)
+    cs = time.perf_counter() - t0
+    out["cpu_baseline"] = {"node_updates_per_s": cpu_sweeps * free / cs, "cores": 1, "kind": "port",
+                             "sample": f"{cpu_sweeps} sweeps of the RF basis at 160 x 160 x 242"}
+    out["speedup_vs_cpu_core"] = out["node_updates_per_s"] / out["cpu_baseline"]["node_updates_per_s"]
+    log(f"legacy potentials: sweeps {[x['sweeps'] for x in st]}, {ms:.0f} ms, "
+        f"{out['node_updates_per_s']:.3g} node-updates/s; Fig. 4 peak {tr[k]} at m/z {mzs[k]}")
+    return out
+
+
 def read_profile_value(name: str, key: str):
     """A figure from the committed rocprofv3 PMC summary of the worker kernel, if any."""
     path = os.path.join(ROOT, "profiles", f"k_batched_{name}_traffic.md")
@@ -301,6 +359,9 @@ def main():
     ap.add_argument("--grad-fd", type=int, default=1, help="check the adjoint against central differences")
     ap.add_argument("--skip-tandem", action="store_true")
     ap.add_argument("--skip-refgeom", action="store_true")
+    ap.add_argument("--skip-legacy", action="store_true")
+    ap.add_argument("--legacy-only", action="store_true",
+                    help="run only the legacy-potential regeneration sub-bench and print its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one torch elementwise copy of 1 GiB (read 1 GiB, write 1 GiB) first, as the "
                          "known-size reference for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -309,6 +370,9 @@ def main():
         sys.exit("bench.py runs on one GPU only: the integrator has no multi-GPU path in this build "
                  "(ions are independent; see the sharding design). Use --gpus 1.")
 
+    if a.legacy_only:
+        print(json.dumps({"legacy_potentials": legacy_potentials_bench()}), flush=True)
+        return
     if a.copy_calibration:
         import torch
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -403,6 +467,8 @@ def main():
         result["tandem_c5"] = tandem_bench()
     if not a.skip_refgeom:
         result["reference_geometry"] = reference_geometry_bench()
+    if not a.skip_legacy:
+        result["legacy_potentials"] = legacy_potentials_bench()
     if not a.skip_cpu:
         result["cpu_baseline"] = cpu_baseline(grid, jobs, dc, ph, a.cpu_seconds)
     print(json.dumps(result), flush=True)

@@ -192,38 +192,107 @@ __global__ void k_legacy_raster(JGeo g, int basis, float* phi, uint8_t* el)
 
 // ------------------------------------------------------------ sweeps
 
+// Red-black split storage for the sweeps: colour c = (i + j + k) & 1 keeps its
+// nodes of row (j, k) at (k ny + j) hx + (i >> 1), hx = (nx + 1) / 2. A step
+// updates one colour from the other's values only (a node's new value does not
+// read its old one), so it reads one array and writes the other, both
+// contiguous along a row: half the bytes of the interleaved layout.
 struct SweepGrid {
-    int nx, ny, nz;
-    int half;          // same-parity interior nodes per row, at most
-    uint32_t rows;     // (ny - 2) (nz - 2)
-    size_t nch;        // check point node
+    int nx, ny, nz, hx;
+    int jgroups;       // groups of LEG_R interior rows along j
+    uint32_t rows;     // row groups: jgroups (nz - 2)
+    int ch_color;      // check point: colour and split index
+    size_t ch_idx;
 };
 
-// Step tau of the wavefront: the free interior nodes with i + j + k + 2t = tau
-// and t < T. log_ch[t] receives the check point's value after sweep t.
-__global__ void __launch_bounds__(TB) k_legacy_step(SweepGrid G, float* __restrict__ phi,
-                                                     const uint8_t* __restrict__ el, int tau, int T, float* log_ch)
+__host__ __device__ inline size_t split_idx(int hx, int ny, int i, int j, int k)
 {
-    const uint32_t id = blockIdx.x * TB + threadIdx.x;
-    const uint32_t m = id % (uint32_t)G.half, r = id / (uint32_t)G.half;
-    if (r >= G.rows) return;
-    const int j = 1 + (int)(r % (uint32_t)(G.ny - 2)), k = 1 + (int)(r / (uint32_t)(G.ny - 2));
-    const int i = 1 + ((tau - 1 - j - k) & 1) + 2 * (int)m;
-    if (i > G.nx - 2) return;
-    const int s = i + j + k;
-    if (s > tau) return;
-    const int t = (tau - s) >> 1;
-    if (t >= T) return;
-    const size_t sy = (size_t)G.nx, sz = (size_t)G.nx * G.ny;
-    const size_t n = (size_t)i + sy * j + sz * k;
-    if (el[n]) return;
-    const float v = (phi[n - 1] + phi[n + 1] + phi[n - sy] + phi[n + sy] + phi[n - sz] + phi[n + sz]) / 6.0f;
-    phi[n] = v;
-    if (n == G.nch) log_ch[t] = v;
+    return ((size_t)k * ny + j) * hx + (size_t)(i >> 1);
+}
+
+__global__ void k_legacy_split(const float* __restrict__ phi, const uint8_t* __restrict__ el, int nx, int ny,
+                               int nz, int hx, float* c0, float* c1, uint8_t* e0, uint8_t* e1)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t id = (size_t)blockIdx.x * TB + threadIdx.x;
+    if (id >= n) return;
+    const int i = (int)(id % nx), j = (int)((id / nx) % ny), k = (int)(id / ((size_t)nx * ny));
+    const size_t q = split_idx(hx, ny, i, j, k);
+    if ((i + j + k) & 1) { c1[q] = phi[id]; e1[q] = el[id]; }
+    else { c0[q] = phi[id]; e0[q] = el[id]; }
+}
+
+__global__ void k_legacy_merge(const float* __restrict__ c0, const float* __restrict__ c1, int nx, int ny, int nz,
+                               int hx, float* phi)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t id = (size_t)blockIdx.x * TB + threadIdx.x;
+    if (id >= n) return;
+    const int i = (int)(id % nx), j = (int)((id / nx) % ny), k = (int)(id / ((size_t)nx * ny));
+    const size_t q = split_idx(hx, ny, i, j, k);
+    phi[id] = ((i + j + k) & 1) ? c1[q] : c0[q];
+}
+
+// Step tau of the wavefront: colour tau & 1, the free interior nodes with
+// i + j + k + 2t = tau and t < T, from the other colour's values. A thread
+// takes column m of LEG_R consecutive rows j at one k and issues all their
+// loads before any store (the arrays do not alias). Workgroups are dealt to
+// the 8 XCDs round robin (workgroup b runs on XCD b mod 8); the launch has a
+// multiple of 8 workgroups and workgroup b takes slab (b mod 8) of the z
+// range, so the neighbour rows a workgroup reads were mostly fetched into the
+// same XCD's L2 by its neighbours. log_ch[t] receives the check point's value
+// after sweep t.
+constexpr int LEG_R = 4;
+constexpr int N_XCD = 8;
+
+__global__ void __launch_bounds__(TB) k_legacy_step(SweepGrid G, const float* __restrict__ other,
+                                                     float* __restrict__ own, const uint8_t* __restrict__ el_own,
+                                                     int tau, int T, float* __restrict__ log_ch)
+{
+    const uint32_t per_xcd = gridDim.x / N_XCD;
+    const uint32_t wg = (blockIdx.x % N_XCD) * per_xcd + blockIdx.x / N_XCD;
+    const uint32_t id = wg * TB + threadIdx.x;
+    const uint32_t m = id % (uint32_t)G.hx, g = id / (uint32_t)G.hx;
+    if (g >= G.rows) return;
+    const int j0 = 1 + LEG_R * (int)(g % (uint32_t)G.jgroups), k = 1 + (int)(g / (uint32_t)G.jgroups);
+    const size_t sy = (size_t)G.hx, sz = (size_t)G.ny * G.hx;
+    float v[LEG_R];
+    bool upd[LEG_R];
+    int tt[LEG_R];
+    size_t oo[LEG_R];
+#pragma unroll
+    for (int r = 0; r < LEG_R; ++r) {
+        const int j = j0 + r;
+        const int p = (tau + j + k) & 1;  // parity of i in this colour
+        const int i = 2 * (int)m + p;
+        const int s = i + j + k;
+        const int t = (tau - s) >> 1;
+        const bool ok = j <= G.ny - 2 && i >= 1 && i <= G.nx - 2 && s <= tau && t < T;
+        const size_t row = ((size_t)k * G.ny + (ok ? j : 1)) * G.hx, o = row + (ok ? m : 1);
+        // (i-1) >> 1 = m - 1 + p, (i+1) >> 1 = m + p; the j and k neighbours share i >> 1 = m
+        const size_t a = ok ? row + m - 1 + p : o;
+        v[r] = (other[a] + other[row + (ok ? m + p : 1)] + other[o - sy] + other[o + sy] + other[o - sz] +
+                other[o + sz]) / 6.0f;
+        upd[r] = ok && el_own[o] == 0;
+        tt[r] = t;
+        oo[r] = o;
+    }
+#pragma unroll
+    for (int r = 0; r < LEG_R; ++r) {
+        if (!upd[r]) continue;
+        own[oo[r]] = v[r];
+        if ((tau & 1) == G.ch_color && oo[r] == G.ch_idx) log_ch[tt[r]] = v[r];
+    }
 }
 
 // The stop rule's change at the check point, in float.
 float dif_of(float prev, float now) { return std::fabs(100.0f * ((prev + 1e-10f) - now) / (prev + 1e-10f)); }
+
+struct PinnedLog {
+    float* p = nullptr;
+    explicit PinnedLog(size_t n) { LGCHK(hipHostMalloc((void**)&p, (n ? n : 1) * 4, hipHostMallocDefault)); }
+    ~PinnedLog() { if (p) (void)hipHostFree(p); }
+};
 
 // calculatePotential's sweeps on the device buffer phi (start state), in place.
 mq_legacy_stats run_sweeps(int nx, int ny, int nz, float* phi, const uint8_t* el, const int32_t check[3],
@@ -239,11 +308,15 @@ mq_legacy_stats run_sweeps(int nx, int ny, int nz, float* phi, const uint8_t* el
     G.nx = nx;
     G.ny = ny;
     G.nz = nz;
-    G.half = (nx - 1) / 2;
-    G.rows = (uint32_t)(ny - 2) * (uint32_t)(nz - 2);
-    G.nch = (size_t)check[0] + (size_t)nx * ((size_t)check[1] + (size_t)ny * check[2]);
-    if ((uint64_t)G.half * G.rows > 0xFFFFFFFFull - TB) throw LgErr(-1, "grid too large");
-    const unsigned blocks = (unsigned)(((uint64_t)G.half * G.rows + TB - 1) / TB);
+    G.hx = (nx + 1) / 2;
+    G.jgroups = (ny - 2 + LEG_R - 1) / LEG_R;
+    G.rows = (uint32_t)G.jgroups * (uint32_t)(nz - 2);
+    G.ch_color = (check[0] + check[1] + check[2]) & 1;
+    G.ch_idx = split_idx(G.hx, ny, check[0], check[1], check[2]);
+    if ((uint64_t)G.hx * G.rows > 0xFFFFFFFFull - TB) throw LgErr(-1, "grid too large");
+    const unsigned blocks =
+        (unsigned)((((uint64_t)G.hx * G.rows + TB - 1) / TB + N_XCD - 1) / N_XCD * N_XCD);  // multiple of 8
+    const size_t NS = (size_t)nz * ny * G.hx;  // one colour array
 
     mq_legacy_stats st{};
     st.check[0] = check[0];
@@ -252,20 +325,34 @@ mq_legacy_stats run_sweeps(int nx, int ny, int nz, float* phi, const uint8_t* el
 
     std::vector<uint8_t> h_el(N);
     LGCHK(hipMemcpyAsync(h_el.data(), el, N, hipMemcpyDeviceToHost, s));
+    const size_t nch = (size_t)check[0] + (size_t)nx * ((size_t)check[1] + (size_t)ny * check[2]);
     float c0 = 0.0f;
-    LGCHK(hipMemcpyAsync(&c0, phi + G.nch, 4, hipMemcpyDeviceToHost, s));
+    LGCHK(hipMemcpyAsync(&c0, phi + nch, 4, hipMemcpyDeviceToHost, s));
     LGCHK(hipStreamSynchronize(s));
     uint64_t nfree = 0;
     for (int k = 1; k < nz - 1; ++k)
         for (int j = 1; j < ny - 1; ++j)
             for (int i = 1; i < nx - 1; ++i) nfree += h_el[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] == 0;
-    const bool ch_fixed = h_el[G.nch] || check[0] == 0 || check[0] == nx - 1 || check[1] == 0 ||
-                          check[1] == ny - 1 || check[2] == 0 || check[2] == nz - 1;
+    const bool ch_fixed = h_el[nch] || check[0] == 0 || check[0] == nx - 1 || check[1] == 0 || check[1] == ny - 1 ||
+                          check[2] == 0 || check[2] == nz - 1;
 
     const int s_lo = 3, s_hi = (nx - 2) + (ny - 2) + (nz - 2);
     const int s_ch = check[0] + check[1] + check[2];
-    auto launch = [&](int tau, int T, float* log) {
-        k_legacy_step<<<blocks, TB, 0, s>>>(G, phi, el, tau, T, log);
+    DevBuf col(2 * NS * 4), ecol(2 * NS), log((size_t)(max_iter > 0 ? max_iter : 1) * 4);
+    float* cc[2] = {col.as<float>(), col.as<float>() + NS};
+    uint8_t* ec[2] = {ecol.as<uint8_t>(), ecol.as<uint8_t>() + NS};
+    const unsigned nblk = (unsigned)((N + TB - 1) / TB);
+    auto split = [&]() {
+        LGCHK(hipMemsetAsync(col.p, 0, 2 * NS * 4, s));
+        LGCHK(hipMemsetAsync(ecol.p, 0, 2 * NS, s));
+        k_legacy_split<<<nblk, TB, 0, s>>>(phi, el, nx, ny, nz, G.hx, cc[0], cc[1], ec[0], ec[1]);
+        LGCHK(hipGetLastError());
+    };
+    auto run_pass = [&](int T, int from, int to) {
+        for (int tau = from; tau <= to; ++tau)
+            k_legacy_step<<<blocks, TB, 0, s>>>(G, cc[(tau & 1) ^ 1], cc[tau & 1], ec[tau & 1], tau, T,
+                                                log.as<float>());
+        LGCHK(hipGetLastError());
     };
 
     // the stop rule over the check point's values after each sweep
@@ -277,11 +364,7 @@ mq_legacy_stats run_sweeps(int nx, int ny, int nz, float* phi, const uint8_t* el
         ++nit;
     };
     LGCHK(hipEventRecord(ea, s));
-    DevBuf log((size_t)(max_iter > 0 ? max_iter : 1) * 4);
-    auto run_pass = [&](int T, int from, int to) {
-        for (int tau = from; tau <= to; ++tau) launch(tau, T, log.as<float>());
-        LGCHK(hipGetLastError());
-    };
+    split();
     uint32_t T = 0;
     if (max_iter > 0 && ch_fixed) {
         // the check point never changes: its start value decides the rule
@@ -289,37 +372,57 @@ mq_legacy_stats run_sweeps(int nx, int ny, int nz, float* phi, const uint8_t* el
         T = nit;
         run_pass((int)T, s_lo, s_hi + 2 * ((int)T - 1));
     } else if (max_iter > 0) {
-        DevBuf start(N * 4);
-        LGCHK(hipMemcpyAsync(start.p, phi, N * 4, hipMemcpyDeviceToDevice, s));
-        std::vector<float> h_log(max_iter);
-        const int chunk = 512;
+        // pass one: run the wavefronts, read the check point's log one chunk behind the launches
+        PinnedLog h_log(max_iter);
+        const int chunk = 256;
         const int tau_end = s_hi + 2 * ((int)max_iter - 1);
         int tau = s_lo;
+        int pending = -1;  // sweeps of the check point whose log copy is in flight
+        hipEvent_t ev = nullptr;
+        LGCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         bool decided = false;
-        while (!decided && tau <= tau_end) {
-            const int upto = std::min(tau_end, tau + chunk - 1);
-            run_pass((int)max_iter, tau, upto);
-            tau = upto + 1;
-            // sweeps of the check point complete by now: t with s_ch + 2t <= tau - 1
-            const int have = tau - 1 >= s_ch ? std::min<int>((int)max_iter, (tau - 1 - s_ch) / 2 + 1) : 0;
-            if ((uint32_t)have > nit) {
-                LGCHK(hipMemcpyAsync(h_log.data() + nit, log.as<float>() + nit, (size_t)(have - nit) * 4,
-                                     hipMemcpyDeviceToHost, s));
-                LGCHK(hipStreamSynchronize(s));
-                while (nit < (uint32_t)have && dif > precision) feed(h_log[nit]);
-                if (!(nit < max_iter && dif > precision)) decided = true;
+        try {
+            while (!decided) {
+                int have = -1;
+                if (tau <= tau_end) {
+                    const int upto = std::min(tau_end, tau + chunk - 1);
+                    run_pass((int)max_iter, tau, upto);
+                    tau = upto + 1;
+                    // sweeps t of the check point complete: s_ch + 2t <= tau - 1
+                    have = tau - 1 >= s_ch ? std::min<int>((int)max_iter, (tau - 1 - s_ch) / 2 + 1) : 0;
+                }
+                if (pending >= 0) {
+                    LGCHK(hipEventSynchronize(ev));
+                    while (nit < (uint32_t)pending && dif > precision) feed(h_log.p[nit]);
+                    if (!(nit < max_iter && dif > precision)) decided = true;
+                    pending = -1;
+                }
+                if (decided) break;
+                if (have < 0) throw LgErr(-9, "sweep log incomplete");  // every sweep was launched and read
+                if (have > (int)nit) {
+                    LGCHK(hipMemcpyAsync(h_log.p + nit, log.as<float>() + nit, (size_t)(have - (int)nit) * 4,
+                                         hipMemcpyDeviceToHost, s));
+                    LGCHK(hipEventRecord(ev, s));
+                    pending = have;
+                }
             }
+        } catch (...) {
+            (void)hipEventDestroy(ev);
+            throw;
         }
+        (void)hipEventDestroy(ev);
         T = nit;
         if (T < max_iter) {
             // the nodes below the check point ran past T: rerun from the start state with exactly T sweeps
-            LGCHK(hipMemcpyAsync(phi, start.p, N * 4, hipMemcpyDeviceToDevice, s));
+            split();
             run_pass((int)T, s_lo, s_hi + 2 * ((int)T - 1));
         } else {
             // the cap: every node stops at max_iter sweeps by itself; finish the drain
             run_pass((int)max_iter, tau, tau_end);
         }
     }
+    k_legacy_merge<<<nblk, TB, 0, s>>>(cc[0], cc[1], nx, ny, nz, G.hx, phi);
+    LGCHK(hipGetLastError());
     LGCHK(hipEventRecord(eb, s));
     LGCHK(hipStreamSynchronize(s));
     float ms = 0.0f;
