@@ -121,8 +121,11 @@ def cpu_baseline(grid, jobs, dc, ph, seconds: float):
 def grad_bench(fd: bool):
     """Differentiable form: 65,536 collision-free ions, m/z 499, 20,000 steps, 0.05 mm grid with
     FP32-rounded basis values; one gradient evaluation = forward + backward (adjoint).
-    Operating point: the Fig-4 drive with a 0.5 V gate (no ion reaches the barrier top, so
-    the objective is smooth on the scale of the finite-difference steps)."""
+    Operating point: the Fig-4 drive with a 0.5 V gate. The adjoint is the exact derivative of
+    the discrete map, ion by ion. A handful of ions stay near r_leak on orbits whose sensitivity
+    grows over the 20,000 steps (|dJ_i/dV_rf| up to ~2e5 per volt): a step of 1e-6 relative
+    moves them far outside the linear range, so central differences are compared with the
+    adjoint over the ions they resolve, and the rest are reported as unresolved."""
     grid = synth.c2_grid()
     for k in ("rf", "ac"):
         grid[k] = grid[k].astype(np.float32).astype(np.float64)
@@ -170,20 +173,25 @@ def grad_bench(fd: bool):
             d2, p2 = central(grp, name, h / 2)
             rich, prich = (4 * d2 - d1) / 3, (4 * p2 - p1) / 3
             g = float(grad[k])
-            # per-ion agreement: an ion is well conditioned when its Richardson central
-            # difference matches its adjoint contribution within 1e-4 (or 1e-6 of the
-            # largest contribution)
-            tol = 1e-4 * np.abs(gi[k]) + 1e-6 * np.abs(gi[k]).max()
-            ok = np.abs(prich - gi[k]) <= tol
+            # An ion's central difference resolves its derivative when the estimates at h and
+            # h/2 agree to 1e-6 relative (the truncation error of a smooth ion is O(h^2); a
+            # trajectory whose sensitivity is amplified over the 20,000 steps is far outside
+            # the linear range at h). The split uses the differences only, not the adjoint.
+            scale = np.abs(p2).max()
+            res = np.abs(p1 - p2) <= 1e-6 * np.abs(p2) + 1e-12 * scale
+            adj_res, fd_res = float(gi[k][res].sum()), float(prich[res].sum())
+            top = np.argsort(np.abs(gi[k]))[::-1][:3]
             rel[label] = {"adjoint": g, "central_h": d1, "central_h2": d2, "richardson": rich,
-                          "rel_err_central": abs(d1 - g) / max(abs(g), 1e-300),
-                          "rel_err_richardson": abs(rich - g) / max(abs(g), 1e-300),
-                          "ions_agreeing_1e-4": float(ok.mean()),
-                          "adjoint_over_agreeing_ions": float(gi[k][ok].sum()),
-                          "richardson_over_agreeing_ions": float(prich[ok].sum()),
-                          "largest_disagreeing_contributions": sorted(np.abs(gi[k][~ok]).tolist())[-3:]}
-            log(f"gradient check {label}: adjoint {g:.8g} central {d1:.8g} richardson {rich:.8g}; "
-                f"{ok.mean():.5f} of ions agree per ion")
+                          "rel_err_central_all_ions": abs(d1 - g) / max(abs(g), 1e-300),
+                          "fd_resolved_ions": float(res.mean()),
+                          "adjoint_over_resolved": adj_res, "richardson_over_resolved": fd_res,
+                          "rel_err_over_resolved": abs(adj_res - fd_res) / max(abs(fd_res), 1e-300),
+                          "unresolved_share_of_adjoint": float(gi[k][~res].sum() / g) if g != 0 else 0.0,
+                          "largest_ions": [{"ion": int(i), "adjoint": float(gi[k][i] * n),
+                                            "central_h": float(p1[i] * n), "central_h2": float(p2[i] * n),
+                                            "resolved": bool(res[i])} for i in top]}
+            log(f"gradient check {label}: adjoint {g:.8g} central {d1:.8g}; over the {res.mean():.5f} of ions "
+                f"that central differences resolve: adjoint {adj_res:.10g} richardson {fd_res:.10g}")
         out["fd_check_h_rel_1e-6"] = rel
     eng.close()
     return out
@@ -420,15 +428,16 @@ def main():
                           "ion_steps_per_launch": steps / max(launches, 1),
                           "avg_launch_ms": kms / max(launches, 1)}
 
-    # achievable random-gather bandwidth of this config's cell table (304-B records, worker occupancy):
-    # the whole table, a 128 MB window (Infinity Cache sized) and a 2.5 MB window (L2 sized)
+    # achievable random-gather bandwidth of this config's cell table (304-B records at a 320-B stride,
+    # worker occupancy): the whole table, a 128 MB window (Infinity Cache sized) and a 2.5 MB window
+    # (L2 sized)
     ncell = (grid["nx"] - 1) * (grid["ny"] - 1) * (grid["nz"] - 1)
-    gath = {"table": eng.gather_probe(0) / 1e9, "window_128MB": eng.gather_probe(128 * 2**20 // 304) / 1e9,
+    gath = {"table": eng.gather_probe(0) / 1e9, "window_128MB": eng.gather_probe(128 * 2**20 // 320) / 1e9,
             "window_2.5MB": eng.gather_probe(8192) / 1e9}
     traffic = result["roofline"]["traffic"]
     r = result["roofline"]
     r["achievable_gather_gbs"] = gath
-    r["table_bytes"] = ncell * 304
+    r["table_bytes"] = ncell * 320
     if traffic:
         r["traffic_gbs"] = traffic / (r["avg_launch_ms"] * 1e-3) / 1e9
         r["traffic_frac_of_achievable_table_gather"] = r["traffic_gbs"] / gath["table"]

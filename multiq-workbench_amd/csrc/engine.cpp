@@ -137,7 +137,7 @@ struct mq_engine {
         }
         if (want_cells && !cells_valid) {
             const size_t nc = (size_t)(nx - 1) * (ny - 1) * (nz - 1);
-            cells.ensure(nc * CELL_F4 * sizeof(float4));
+            cells.ensure(nc * CELL_STRIDE_F4 * sizeof(float4));
             launch_build_cells(cells.as<float4>(), rf.as<double>(), dcc.as<double>(), ac.as<double>(),
                                mask.as<uint8_t>(), nx, ny, nz, stream);
             HIPCHK(hipGetLastError());

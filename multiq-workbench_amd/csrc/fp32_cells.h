@@ -15,7 +15,7 @@ __device__ __forceinline__ float log_f(float x) { return __builtin_amdgcn_logf(x
 
 __device__ __forceinline__ void load_cell(const float4* __restrict__ cells, int32_t c, float4 R[CELL_F4])
 {
-    const float4* src = cells + (size_t)c * CELL_F4;
+    const float4* src = cells + (size_t)c * CELL_STRIDE_F4;
 #pragma unroll
     for (int q = 0; q < CELL_F4; ++q) R[q] = src[q];
 }

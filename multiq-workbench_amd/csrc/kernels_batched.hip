@@ -6,10 +6,11 @@
 // (:97-124), and the live-list compaction tc_filter / tc_size_args (:341-374).
 //
 // Design for CDNA4:
-//  * Fields: one 304-byte record per grid cell holding the trilinear polynomial
-//    coefficients of the RF, combined-DC and AC bases (72 FP32) and the
-//    conductor bits of its 8 corners. A cell change costs 19 contiguous 16-byte
-//    loads; the record stays in VGPRs while the ion stays in the cell.
+//  * Fields: one 304-byte record per grid cell (at a 320-byte stride) holding
+//    the trilinear polynomial coefficients of the RF, combined-DC and AC bases
+//    (72 FP32) and the conductor bits of its 8 corners. A cell change costs 19
+//    contiguous 16-byte loads; the record stays in VGPRs while the ion stays in
+//    the cell.
 //  * Precision: FP32 field values, interpolation, kick and collision sampling;
 //    FP64 position, drift, time (step counter) and RF/AC phase.
 //  * Scheduling: each wavefront keeps 64 ions in flight; every `period` steps
@@ -67,9 +68,11 @@ __global__ void k_build_cells(float4* __restrict__ cells, const double* __restri
     }
     out[72] = __uint_as_float(bits);
     out[73] = out[74] = out[75] = 0.0f;
-    float4* dst = cells + (size_t)c * CELL_F4;
+    float4* dst = cells + (size_t)c * CELL_STRIDE_F4;
 #pragma unroll
     for (int q = 0; q < CELL_F4; ++q) dst[q] = make_float4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+#pragma unroll
+    for (int q = CELL_F4; q < CELL_STRIDE_F4; ++q) dst[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 __global__ void k_fill_hist(uint32_t* hist, uint32_t n)

@@ -50,8 +50,16 @@ constexpr int ACC = 8;
 // FP32 cell record of the batched path: trilinear polynomial coefficients
 // c0..c7 (1, tx, ty, tz, tx ty, ty tz, tx tz, tx ty tz) x 3 components for the
 // RF, combined-DC and AC bases, then the conductor mask of the 8 corners
-// (bit dx + 2 dy + 4 dz), padded to 19 float4 = 304 B.
+// (bit dx + 2 dy + 4 dz): 19 float4 = 304 B loaded per cell change. Records
+// sit at a stride of 20 float4 = 320 B, so each starts on a 64-byte boundary
+// and spans exactly three 128-byte lines (a 304-byte stride leaves 1 record in 4
+// straddling a fourth line).
+#ifndef MQ_CELL_STRIDE_F4
+#define MQ_CELL_STRIDE_F4 20
+#endif
 constexpr int CELL_F4 = 19;
+constexpr int CELL_STRIDE_F4 = MQ_CELL_STRIDE_F4;
+static_assert(CELL_STRIDE_F4 >= CELL_F4, "cell record stride below its size");
 
 // ---- tandem batch (kernels_tandem.hip)
 // One grid of the tandem: FP64 bases and mask (plain path), geometry.

@@ -182,7 +182,7 @@ struct mq_tandem {
         }
         if (want_cells && !T.cells_valid) {
             const size_t nc = (size_t)(G.meta.nx - 1) * (G.meta.ny - 1) * (G.meta.nz - 1);
-            T.cells.ensure(nc * CELL_F4 * sizeof(float4));
+            T.cells.ensure(nc * CELL_STRIDE_F4 * sizeof(float4));
             // the cube has no excitation: its AC slot is never weighted (a_ac = 0)
             const double* acp = G.has_ac ? G.ac.as<double>() : G.rf.as<double>();
             launch_build_cells(T.cells.as<float4>(), G.rf.as<double>(), T.dcc.as<double>(), acp,

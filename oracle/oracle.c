@@ -702,7 +702,8 @@ typedef struct {
     const mq_job* job;
     const mq_physics* ph;
     const mq_objective* obj;
-    uint32_t n;
+    uint32_t n;      /* number of output slots */
+    const uint32_t* ions;  /* slot -> ion index, or NULL for slot == ion */
     double* Ji;      /* n */
     double* gi;      /* 5 * n */
     double* gsi;     /* 6 * n */
@@ -711,7 +712,7 @@ typedef struct {
 
 static double sigm(double u) { return 1.0 / (1.0 + exp(-u)); }
 
-static void grad_one(const or_gwork* W, uint32_t ion)
+static void grad_one(const or_gwork* W, uint32_t ion, uint32_t slot)
 {
     const or_ctx* C = W->C;
     const or_jp* J = W->J;
@@ -775,11 +776,11 @@ static void grad_one(const or_gwork* W, uint32_t ion)
             gJ[1] = k * ry;
         }
     }
-    W->Ji[ion] = Ji;
+    W->Ji[slot] = Ji;
     for (int k = 0; k < NT; ++k) {
         double g = gJ[0] * dx[k][0] + gJ[1] * dx[k][1] + gJ[2] * dx[k][2];
-        if (k < 5) W->gi[(size_t)k * W->n + ion] = g;
-        else W->gsi[(size_t)(k - 5) * W->n + ion] = g;
+        if (k < 5) W->gi[(size_t)k * W->n + slot] = g;
+        else if (W->gsi) W->gsi[(size_t)(k - 5) * W->n + slot] = g;
     }
 }
 
@@ -789,7 +790,7 @@ static void* grad_worker(void* arg)
     for (;;) {
         size_t a = atomic_fetch_add(&W->next, 1);
         if (a >= W->n) break;
-        grad_one(W, (uint32_t)a);
+        grad_one(W, W->ions ? W->ions[a] : (uint32_t)a, (uint32_t)a);
     }
     return NULL;
 }
@@ -805,7 +806,7 @@ int or_grad_reference(const mq_fields* f, const mq_job* job, const double dc[4],
     job_params(job, ph, &J);
     uint32_t n = (uint32_t)job->n;
     or_gwork W;
-    W.C = &C; W.J = &J; W.job = job; W.ph = ph; W.obj = obj; W.n = n;
+    W.C = &C; W.J = &J; W.job = job; W.ph = ph; W.obj = obj; W.n = n; W.ions = NULL;
     W.Ji = (double*)calloc(n, sizeof(double));
     W.gi = (double*)calloc((size_t)5 * n, sizeof(double));
     W.gsi = (double*)calloc((size_t)6 * n, sizeof(double));
@@ -828,6 +829,35 @@ int or_grad_reference(const mq_fields* f, const mq_job* job, const double dc[4],
     free(W.Ji);
     free(W.gi);
     free(W.gsi);
+    free(dcc);
+    return 0;
+}
+
+/* Per-ion form of or_grad_reference for a list of ion indices of the job:
+ * Ji[s] = objective of ion ions[s], gi[k * nions + s] = its d/dtheta_k (not
+ * divided by job->n). Used to examine single trajectories of a large batch. */
+int or_grad_ions(const mq_fields* f, const mq_job* job, const double dc[4], const mq_physics* ph,
+                 const mq_objective* obj, int nthreads, const uint32_t* ions, uint32_t nions, double* Ji, double* gi)
+{
+    for (uint32_t s = 0; s < nions; ++s)
+        if (ions[s] >= (uint32_t)job->n) return -2;
+    double* dcc = combine_dc(f, dc);
+    if (!dcc) return -1;
+    or_ctx C;
+    ctx_init(&C, f, dcc, ph);
+    or_jp J;
+    job_params(job, ph, &J);
+    or_gwork W;
+    W.C = &C; W.J = &J; W.job = job; W.ph = ph; W.obj = obj; W.n = nions; W.ions = ions;
+    W.Ji = Ji;
+    W.gi = gi;
+    W.gsi = NULL;
+    atomic_init(&W.next, 0);
+    if (nthreads < 1) nthreads = 1;
+    pthread_t* th = (pthread_t*)malloc((size_t)nthreads * sizeof(pthread_t));
+    for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, grad_worker, &W);
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th);
     free(dcc);
     return 0;
 }

@@ -89,6 +89,12 @@ void or_summarize(const or_ion* ions, const uint32_t* job, size_t n, size_t njob
 int or_grad_reference(const mq_fields* f, const mq_job* job, const double dc[4], const mq_physics* ph,
                       const mq_objective* obj, int nthreads, double* J, double grad[5], double* grad_state);
 
+/* The same for the listed ions of the job only: Ji[s] and gi[k * nions + s] =
+ * objective of ion ions[s] and its derivative by parameter k (per ion, not
+ * divided by job->n). Returns -2 if an index is out of range. */
+int or_grad_ions(const mq_fields* f, const mq_job* job, const double dc[4], const mq_physics* ph,
+                 const mq_objective* obj, int nthreads, const uint32_t* ions, uint32_t nions, double* Ji, double* gi);
+
 /* Trilinear sample of one node-major FP64 basis at grid coordinates g. */
 void or_sample(const double* F, int nx, int ny, int nz, const double g[3], int nearest, double out[3]);
 

@@ -216,6 +216,27 @@ def grad_reference(grid: dict, job, dc, ph, obj, nthreads: int = 8):
     return J.value, np.array(list(g)), gs.reshape(6, n)
 
 
+def grad_ions(grid: dict, job, dc, ph, obj, ions, nthreads: int = 8):
+    """Dual-number gradient of single ions of a job: (Ji (m,), gi (5, m)) for the m listed
+    ion indices, per ion (not divided by job.n)."""
+    f, keep = mqhip.make_fields(grid)
+    ions = np.ascontiguousarray(ions, np.uint32)
+    m = len(ions)
+    Ji = np.zeros(m)
+    gi = np.zeros(5 * m)
+    lib = load()
+    lib.or_grad_ions.argtypes = [C.POINTER(mqhip.Fields), C.POINTER(mqhip.Job), C.POINTER(C.c_double),
+                                 C.POINTER(mqhip.Physics), C.POINTER(mqhip.Objective), C.c_int,
+                                 C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    rc = lib.or_grad_ions(C.byref(f), C.byref(job), (C.c_double * 4)(*dc), C.byref(ph), C.byref(obj), nthreads,
+                          ions.ctypes.data_as(C.POINTER(C.c_uint32)), m, Ji.ctypes.data_as(C.POINTER(C.c_double)),
+                          gi.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise RuntimeError(f"oracle per-ion gradient failed ({rc})")
+    del keep
+    return Ji, gi.reshape(5, m)
+
+
 # ---------------------------------------------------------------- field pipeline
 
 def _dp(a):

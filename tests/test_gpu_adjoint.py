@@ -103,3 +103,39 @@ def test_adjoint_errors(c1):
         e.backward()
     with pytest.raises(mqhip.MqError, match="collision-free"):
         e.forward(job, [0, 5, 0, 0], ph.copy(pressure_pa=0.1), obj)
+
+
+def test_adjoint_per_ion_at_full_size_matches_dual_numbers(oracle_mod):
+    """The differentiable form at its full size (65,536 collision-free ions, 20,000 steps,
+    128x128x512 grid, FP32-rounded bases): per-ion adjoint contributions of the 16 ions with
+    the largest |dJ_i/dV_rf| (orbits whose sensitivity grows to ~1e5 per volt, where central
+    differences at 1e-6 relative are far outside the linear range) and of 48 random ions
+    against the oracle's FP64 dual-number derivative of the same ions."""
+    grid = synth.c2_grid()
+    for k in ("rf", "ac"):
+        grid[k] = grid[k].astype(np.float32).astype(np.float64)
+    grid["dc"] = [d.astype(np.float32).astype(np.float64) if d is not None else None for d in grid["dc"]]
+    ph = mqhip.Physics.correct(0.0).copy(dt=1e-7, injection=mqhip.INJ_GAUSSIAN, inj_sigma_mm=0.1, legacy_force=1)
+    n = 65536
+    job = mqhip.job(499.0, 1.0, 0.0, 100.0, 500e3, 0.5, 100e3, n, 4499, dc_scale=0.5)
+    obj = mqhip.Objective(20000, 0, 18.6e-3, 2e-5, 1.0e-3, 0.3)
+    dc = [0.0, 1.0, 0.0, 0.0]
+    e = mqhip.Engine(grid)
+    try:
+        print("[gpu] full-size adjoint: forward + backward of 65536 ions x 20000 steps", flush=True)
+        e.forward(job, dc, ph, obj)
+        Ji = e.forward_per_ion()
+        e.backward(1.0)
+        gi = e.backward_per_ion(n) * n
+    finally:
+        e.close()
+    rng = np.random.default_rng(7)
+    top = np.argsort(np.abs(gi[0]))[::-1][:16]
+    pick = np.unique(np.concatenate([top, rng.choice(n, 48, replace=False)])).astype(np.uint32)
+    Jr, gr = oracle_mod.grad_ions(grid, job, dc, ph, obj, pick, 8)
+    print(f"[gpu]   largest |dJ_i/dV_rf| {np.abs(gi[0][top[0]]):.6g} (ion {top[0]})", flush=True)
+    assert np.abs(gi[0][top[0]]) > 1e3   # the chaotic ions are in the check
+    assert np.allclose(Ji[pick], Jr, rtol=1e-11, atol=1e-14)
+    for k in range(5):
+        sc = np.abs(gr[k]).max()
+        assert np.allclose(gi[k][pick], gr[k], rtol=1e-8, atol=1e-12 * sc), k

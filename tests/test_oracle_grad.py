@@ -123,3 +123,26 @@ def test_coupled_linear_fields_central_differences(oracle_mod):
             setattr(j2.drive if grp else j2, name, base + sgn * h)
             vals.append(oracle_mod.grad_reference(g, j2, [0, 1, 0, 0], ph, obj, 8)[0])
         assert (vals[0] - vals[1]) / (2 * h) == pytest.approx(grad[k], rel=1e-5), name
+
+
+def test_per_ion_subset_matches_batch(oracle_mod):
+    """grad_ions on a shuffled subset returns, slot by slot, the per-ion terms whose mean is
+    grad_reference: the same ions in a different order and with duplicates give the same
+    values, and the full list sums to n times the batch gradient."""
+    r0, ax = 2.765e-3, 3.2e-3
+    g = make_grid(32, 32, 64, 0.2e-3, rf=lambda X, Y, Z: (2 * (Y - ax) / r0**2, 2 * (X - ax) / r0**2, 0 * X),
+                  ac=lambda X, Y, Z: (0 * X - 1 / r0, 0 * X, 0 * X), dc1=lambda X, Y, Z: (0 * X, 0 * X, -(Z - 6e-3) * 100),
+                  axis=(ax, ax), z_inject=5e-3)
+    ph = mqhip.Physics.correct(0.0).copy(dt=1e-7, injection=mqhip.INJ_GAUSSIAN, inj_sigma_mm=0.1, legacy_force=1)
+    n = 24
+    job = mqhip.job(499.0, 1.0, 0.0, 100.0, 500e3, 0.5, 100e3, n, 1499)
+    obj = mqhip.Objective(400, 0, 6.0e-3, 2e-4, 1.0e-3, 0.3)
+    J, grad, _ = oracle_mod.grad_reference(g, job, [0, 1, 0, 0], ph, obj, 4)
+    Ji, gi = oracle_mod.grad_ions(g, job, [0, 1, 0, 0], ph, obj, np.arange(n), 3)
+    assert Ji.mean() == pytest.approx(J, rel=1e-13)
+    assert np.allclose(gi.sum(axis=1) / n, grad, rtol=1e-12, atol=1e-15 * np.abs(grad).max())
+    sub = np.array([17, 3, 3, 23, 0])
+    Js, gs = oracle_mod.grad_ions(g, job, [0, 1, 0, 0], ph, obj, sub, 2)
+    assert np.array_equal(Js, Ji[sub]) and np.array_equal(gs, gi[:, sub])
+    with pytest.raises(RuntimeError):
+        oracle_mod.grad_ions(g, job, [0, 1, 0, 0], ph, obj, np.array([n]), 1)
