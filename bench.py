@@ -90,13 +90,14 @@ def cpu_baseline(grid, jobs, dc, ph, seconds: float):
     except AttributeError:
         ncpu = os.cpu_count() or 1
     threads = max(1, min(16, ncpu))
-    # calibrate on one job, then size the sample to about `seconds`
-    idx = np.linspace(0, len(jobs) - 1, len(jobs)).astype(int)
+    # calibrate on `threads` evenly spaced jobs (one job alone leaves most threads idle behind
+    # its longest-lived ions), then size the sample to about `seconds`
+    cal = [jobs[i] for i in np.linspace(0, len(jobs) - 1, min(len(jobs), threads)).astype(int)]
     t0 = time.perf_counter()
-    r = oracle.run_batch(grid, [jobs[idx[len(idx) // 2]]], dc, ph, threads)
+    r = oracle.run_batch(grid, cal, dc, ph, threads)
     t1 = time.perf_counter() - t0
     rate0 = r["steps"].sum() / max(t1, 1e-9)
-    per_job = r["steps"].sum()
+    per_job = r["steps"].sum() / len(cal)
     njobs = int(max(1, min(len(jobs), seconds * rate0 / max(per_job, 1))))
     pick = [jobs[i] for i in np.linspace(0, len(jobs) - 1, njobs).astype(int)]
     log(f"cpu baseline: oracle on {njobs} jobs x {pick[0].n} ions with {threads} threads")
@@ -122,10 +123,11 @@ def grad_bench(fd: bool):
     """Differentiable form: 65,536 collision-free ions, m/z 499, 20,000 steps, 0.05 mm grid with
     FP32-rounded basis values; one gradient evaluation = forward + backward (adjoint).
     Operating point: the Fig-4 drive with a 0.5 V gate. The adjoint is the exact derivative of
-    the discrete map, ion by ion. A handful of ions stay near r_leak on orbits whose sensitivity
-    grows over the 20,000 steps (|dJ_i/dV_rf| up to ~2e5 per volt): a step of 1e-6 relative
-    moves them far outside the linear range, so central differences are compared with the
-    adjoint over the ions they resolve, and the rest are reported as unresolved."""
+    the discrete map, ion by ion. A few ions end near r_leak on orbits whose sensitivity grows
+    over the 20,000 steps (|dJ_i/dV_rf| up to ~2e5 per volt): a step of 1e-6 relative moves them
+    far outside the linear range, so central differences are compared with the adjoint over all
+    ions and over the ions the step keeps in its linear range (fd_check), and again over all
+    ions on a 2,000-step horizon."""
     grid = synth.c2_grid()
     for k in ("rf", "ac"):
         grid[k] = grid[k].astype(np.float32).astype(np.float64)
@@ -150,51 +152,72 @@ def grad_bench(fd: bool):
                      "100 V RF @ 500 kHz, 0.5 V AC @ 100 kHz, 0.5 V gate; objective mean sigmoid((z-18.6 mm)/0.02 mm)"
                      " - 0.3 sigmoid((r-1 mm)/0.02 mm); sqrt(N) checkpoints"}
     if fd:
-        rel = {}
-        n = 65536
-        gi = eng.backward_per_ion(n)   # per-ion adjoint contributions (dJ_i/dtheta / n)
-        names = [("drive", "v_rf", "V_rf"), ("drive", "f_rf", "f_rf"), ("drive", "v_ac", "V_exc"),
-                 ("drive", "f_ac", "f_exc"), (None, "dc_scale", "V_dc")]
-
-        def central(grp, name, h):
-            base = getattr(job.drive if grp else job, name)
-            vals, per = [], []
-            for sgn in (+1, -1):
-                j2 = mqhip.Job()
-                j2.sp, j2.drive, j2.n, j2.seed, j2.dc_scale = job.sp, job.drive, job.n, job.seed, job.dc_scale
-                setattr(j2.drive if grp else j2, name, base + sgn * h)
-                vals.append(eng.forward(j2, dc, ph, obj))
-                per.append(eng.forward_per_ion() / n)
-            return (vals[0] - vals[1]) / (2 * h), (per[0] - per[1]) / (2 * h)
-
-        for k, (grp, name, label) in enumerate(names):
-            h = 1e-6 * abs(getattr(job.drive if grp else job, name))
-            d1, p1 = central(grp, name, h)
-            d2, p2 = central(grp, name, h / 2)
-            rich, prich = (4 * d2 - d1) / 3, (4 * p2 - p1) / 3
-            g = float(grad[k])
-            # An ion's central difference resolves its derivative when the estimates at h and
-            # h/2 agree to 1e-6 relative (the truncation error of a smooth ion is O(h^2); a
-            # trajectory whose sensitivity is amplified over the 20,000 steps is far outside
-            # the linear range at h). The split uses the differences only, not the adjoint.
-            scale = np.abs(p2).max()
-            res = np.abs(p1 - p2) <= 1e-6 * np.abs(p2) + 1e-12 * scale
-            adj_res, fd_res = float(gi[k][res].sum()), float(prich[res].sum())
-            top = np.argsort(np.abs(gi[k]))[::-1][:3]
-            rel[label] = {"adjoint": g, "central_h": d1, "central_h2": d2, "richardson": rich,
-                          "rel_err_central_all_ions": abs(d1 - g) / max(abs(g), 1e-300),
-                          "fd_resolved_ions": float(res.mean()),
-                          "adjoint_over_resolved": adj_res, "richardson_over_resolved": fd_res,
-                          "rel_err_over_resolved": abs(adj_res - fd_res) / max(abs(fd_res), 1e-300),
-                          "unresolved_share_of_adjoint": float(gi[k][~res].sum() / g) if g != 0 else 0.0,
-                          "largest_ions": [{"ion": int(i), "adjoint": float(gi[k][i] * n),
-                                            "central_h": float(p1[i] * n), "central_h2": float(p2[i] * n),
-                                            "resolved": bool(res[i])} for i in top]}
-            log(f"gradient check {label}: adjoint {g:.8g} central {d1:.8g}; over the {res.mean():.5f} of ions "
-                f"that central differences resolve: adjoint {adj_res:.10g} richardson {fd_res:.10g}")
-        out["fd_check_h_rel_1e-6"] = rel
+        axis = (grid["axis_x"], grid["axis_y"])
+        out["fd_check_h_rel_1e-6"] = fd_check(eng, job, dc, ph, obj, grad, axis)
+        # the same operating point over 2,000 steps, where every trajectory stays in the
+        # linear range of the step: all ions, no selection
+        obj_s = mqhip.Objective(2000, 0, 18.6e-3, 2e-5, 1.0e-3, 0.3)
+        eng.forward(job, dc, ph, obj_s)
+        grad_s = eng.backward(1.0)
+        out["fd_check_2000_steps_h_rel_1e-6"] = fd_check(eng, job, dc, ph, obj_s, grad_s, axis)
     eng.close()
     return out
+
+
+def fd_check(eng, job, dc, ph, obj, grad, axis):
+    """Adjoint vs central differences (h = 1e-6 relative) and their Richardson extrapolation
+    (h, h/2), over all ions and over the ions that the step keeps in its linear range: an ion
+    whose final z or radius moves by more than 1 % of the objective's length scale between
+    theta - h and theta + h is outside it (the step then samples the sigmoid's curvature, or
+    a trajectory whose sensitivity grew over the run, rather than its slope). The split uses
+    forward runs only."""
+    n = int(job.n)
+    gi = eng.backward_per_ion(n)   # per-ion adjoint contributions (dJ_i/dtheta / n)
+    names = [("drive", "v_rf", "V_rf"), ("drive", "f_rf", "f_rf"), ("drive", "v_ac", "V_exc"),
+             ("drive", "f_ac", "f_exc"), (None, "dc_scale", "V_dc")]
+
+    def central(grp, name, h):
+        base = getattr(job.drive if grp else job, name)
+        vals, per, fin = [], [], []
+        for sgn in (+1, -1):
+            j2 = mqhip.Job()
+            j2.sp, j2.drive, j2.n, j2.seed, j2.dc_scale = job.sp, job.drive, job.n, job.seed, job.dc_scale
+            setattr(j2.drive if grp else j2, name, base + sgn * h)
+            vals.append(eng.forward(j2, dc, ph, obj))
+            per.append(eng.forward_per_ion() / n)
+            fin.append(eng.forward_final())
+        return (vals[0] - vals[1]) / (2 * h), (per[0] - per[1]) / (2 * h), fin
+
+    def moved(fin):
+        (xp, yp, zp), (xm, ym, zm) = fin
+        rp = np.hypot(xp - axis[0], yp - axis[1])
+        rm = np.hypot(xm - axis[0], ym - axis[1])
+        return np.maximum(np.abs(zp - zm), np.abs(rp - rm))
+
+    rel = {}
+    for k, (grp, name, label) in enumerate(names):
+        h = 1e-6 * abs(getattr(job.drive if grp else job, name))
+        d1, p1, f1 = central(grp, name, h)
+        d2, p2, _ = central(grp, name, h / 2)
+        rich, prich = (4 * d2 - d1) / 3, (4 * p2 - p1) / 3
+        g = float(grad[k])
+        lin = moved(f1) <= 0.01 * obj.scale
+        adj_l, fd_l = float(gi[k][lin].sum()), float(prich[lin].sum())
+        mag_l = float(np.abs(prich[lin]).sum())
+        top = np.argsort(np.abs(gi[k]))[::-1][:3]
+        rel[label] = {"adjoint": g, "central_h": d1, "central_h2": d2, "richardson": rich,
+                      "rel_err_richardson_all_ions": abs(rich - g) / max(abs(g), 1e-300),
+                      "linear_range_ions": float(lin.mean()),
+                      "adjoint_over_linear": adj_l, "richardson_over_linear": fd_l,
+                      "rel_err_over_linear": abs(adj_l - fd_l) / max(abs(fd_l), 1e-300),
+                      "err_over_linear_per_abs_sum": abs(adj_l - fd_l) / max(mag_l, 1e-300),
+                      "outside_share_of_adjoint": float(gi[k][~lin].sum() / g) if g != 0 else 0.0,
+                      "largest_ions": [{"ion": int(i), "adjoint": float(gi[k][i] * n),
+                                        "central_h": float(p1[i] * n), "central_h2": float(p2[i] * n),
+                                        "linear": bool(lin[i])} for i in top]}
+        log(f"gradient check {label}: adjoint {g:.8g} richardson {rich:.8g}; over the {lin.mean():.5f} of ions "
+            f"in the step's linear range: adjoint {adj_l:.10g} richardson {fd_l:.10g}")
+    return rel
 
 
 def tandem_bench(steps: int = 2):

@@ -248,6 +248,12 @@ int mq_backward(mq_engine* eng, double dJ_up, double grad[5], double* grad_state
 int mq_forward_per_ion(mq_engine* eng, double* J_ion);
 int mq_backward_per_ion(mq_engine* eng, double* grad_ion);
 
+/* Final position of each ion after the last mq_forward (n values each, metres).
+ * No counterpart in the reference: the differentiable form is new. Lets a
+ * caller tell whether a finite-difference step moved an ion by more than the
+ * objective's length scale, i.e. left the range where differences are linear. */
+int mq_forward_final(mq_engine* eng, double* x, double* y, double* z);
+
 /* ------------------------------------------------------------ scan API */
 
 /* One point of an adaptive scan: AC frequency [kHz], transmitted and crashed

@@ -889,6 +889,17 @@ int mq_forward_per_ion(mq_engine* eng, double* J_ion)
     })
 }
 
+int mq_forward_final(mq_engine* eng, double* x, double* y, double* z)
+{
+    MQ_GUARD({
+        if (!eng || !x || !y || !z) throw Err(-1, "NULL argument");
+        if (!eng->adj_valid) throw Err(-1, "needs a preceding mq_forward on this engine");
+        double* out[3] = {x, y, z};
+        for (int d = 0; d < 3; ++d)
+            HIPCHK(hipMemcpy(out[d], eng->adj_buf.fin[d], (size_t)eng->adj_n * sizeof(double), hipMemcpyDeviceToHost));
+    })
+}
+
 int mq_backward_per_ion(mq_engine* eng, double* grad_ion)
 {
     MQ_GUARD({

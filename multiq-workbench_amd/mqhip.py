@@ -243,6 +243,7 @@ SIGNATURES = {
     "mq_backward": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mq_forward_per_ion": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "mq_backward_per_ion": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "mq_forward_final": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mq_cached_batch": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(Job), C.c_size_t, C.POINTER(C.c_double),
                                   C.POINTER(Physics), C.c_int32, C.c_char_p, C.POINTER(Outcome),
                                   C.POINTER(C.c_size_t)]),
@@ -449,6 +450,13 @@ class Engine:
     def forward_per_ion(self) -> np.ndarray:
         out = np.zeros(self.stats()["n_ions"])
         _check(load().mq_forward_per_ion(self._h, _dptr(out)))
+        return out
+
+    def forward_final(self) -> np.ndarray:
+        """Final positions (3, n) [m] of the last forward."""
+        n = self.stats()["n_ions"]
+        out = np.zeros((3, n))
+        _check(load().mq_forward_final(self._h, _dptr(out[0]), _dptr(out[1]), _dptr(out[2])))
         return out
 
     def backward_per_ion(self, n: int) -> np.ndarray:
