@@ -426,9 +426,8 @@ struct mq_engine {
     // captured once as a hipGraph and replayed.
     void enqueue_block(const IonState& S, const BatchP& B, uint32_t n, const SortGeom& G, int& cur, uint32_t& chunk,
                        uint32_t cap, uint32_t period, uint32_t grid, uint32_t block, int iters, bool timed,
-                       std::vector<hipEvent_t>* evs)
+                       std::vector<hipEvent_t>* evs, bool resort)
     {
-        const bool resort = tune.resort != 0;
         uint32_t* c0 = ctr.as<uint32_t>();
         uint32_t* c1 = c0 + CTR_BLOCK;
         for (int it = 0; it < iters; ++it) {
@@ -482,7 +481,7 @@ struct mq_engine {
         const uint32_t block = (uint32_t)tune.block;
         const uint32_t grid = std::max<uint32_t>(1, (uint32_t)((uint64_t)cus * tune.waves_per_cu * 64 / block));
         uint32_t chunk = ((uint32_t)tune.chunk + 3u) & ~3u;
-        const uint32_t cap = std::max(chunk, ((uint32_t)tune.chunk_cap + 3u) & ~3u);
+        uint32_t cap = std::max(chunk, ((uint32_t)tune.chunk_cap + 3u) & ~3u);
         const uint32_t period = std::max(4u, ((uint32_t)tune.refill_period + 3u) & ~3u);
         const int iters = std::max(1, tune.sync_every);
         kms = 0.0f;
@@ -495,7 +494,7 @@ struct mq_engine {
             uint32_t chunk_cap = chunk;
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             try {
-                enqueue_block(S, B, n, G, cur_cap, chunk_cap, cap, period, grid, block, iters, false, nullptr);
+                enqueue_block(S, B, n, G, cur_cap, chunk_cap, cap, period, grid, block, iters, false, nullptr, resort);
             } catch (...) {
                 (void)hipStreamEndCapture(stream, &g);
                 if (g) (void)hipGraphDestroy(g);
@@ -533,8 +532,16 @@ struct mq_engine {
             evs.assign(2 * iters, nullptr);
             for (auto& e : evs) HIPCHK(hipEventCreate(&e));
         }
+        // Tail: once no more ions are live than the launch has lanes, a re-sort no
+        // longer puts several ions of one region into a wavefront, and the launches
+        // are bound by the step latency of the longest-lived ions. The remaining
+        // launches then skip the re-sort and double the chunk up to the reference's
+        // 200000 (tiled.rs:26-27, 214), so the tail takes a few launches, not one per
+        // re-sort interval.
+        const uint64_t lanes = (uint64_t)grid * block;
+        bool tail = false;
         for (;;) {
-            enqueue_block(S, B, n, G, cur, chunk, cap, period, grid, block, iters, true, &evs);
+            enqueue_block(S, B, n, G, cur, chunk, cap, period, grid, block, iters, true, &evs, resort && !tail);
             HIPCHK(hipGetLastError());
             stats.launches += (uint32_t)iters;
             uint32_t alive = 0;
@@ -547,6 +554,10 @@ struct mq_engine {
                 kms += ms;
             }
             if (alive == 0) break;
+            if (resort && !tail && alive <= lanes) {
+                tail = true;
+                cap = std::max(cap, 200000u);
+            }
         }
     }
 };

@@ -437,7 +437,23 @@ __global__ void k_bin(IonState S, BatchP B, const uint32_t* __restrict__ list, c
     const int ck = min((int)fmin(fmax(S.z[id] * B.inv_h, 0.0), (double)(B.nz - 1)), B.nz - 2);
     const uint32_t bin = (((uint32_t)ck >> G.sz) * G.nby + ((uint32_t)cj >> G.sy)) * G.nbx + ((uint32_t)ci >> G.sx);
     bin_of[i] = bin;
-    rank_of[i] = atomicAdd(&hist[bin], 1u);
+    // Lanes of a wavefront that share a bin take their ranks with one atomic:
+    // peel the bin of the lowest pending lane until every lane has its rank.
+    // (Freshly injected ions all start in a few bins; one atomic per ion
+    // serialised on those counters.)
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t todo = __ballot(1);
+    for (;;) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t lb = (uint32_t)__shfl((int)bin, leader);
+        const uint64_t same = __ballot(bin == lb) & todo;
+        uint32_t base = 0;
+        if (lane == (uint32_t)leader) base = atomicAdd(&hist[lb], (uint32_t)__popcll(same));
+        base = (uint32_t)__shfl((int)base, leader);
+        if ((same >> lane) & 1ull) rank_of[i] = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+        if (todo == 0) break;
+    }
 }
 
 // Exclusive scan of hist in place: blocks of 1024 entries (256 threads x 4).
