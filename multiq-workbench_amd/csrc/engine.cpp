@@ -58,6 +58,11 @@ struct mq_engine {
     BatchP adj_B{};
     DevBuf adj_mem, adj_cols, adj_part, adj_out;
     AdjBuf adj_buf{};
+    AdjGrids adj_G{};
+    // float copies of the node tables for the differentiable form, used when every
+    // value is exactly a float: -1 not checked yet, 0 not exact, 1 copies valid
+    int rfac32_state = -1, dcc32_state = -1;
+    DevBuf rf32, ac32, dcc32, exact_flag;
     std::vector<hipEvent_t> evs;
     mq_run_stats stats{};
 
@@ -106,7 +111,8 @@ struct mq_engine {
     {
         for (DevBuf* b : {&rf, &ac, &dcb[0], &dcb[1], &dcb[2], &dcb[3], &mask, &dcc, &cells, &jobs, &acc, &list[0],
                           &list[1], &list[2], &ctr, &bin_of, &rank_of, &hist, &bsum, &sx, &sy, &sz, &sphi, &svx, &svy, &svz, &sfx, &sfy, &sfz, &ssteps,
-                          &sncoll, &scd, &sjob, &sstatus, &srng, &adj_mem, &adj_cols, &adj_part, &adj_out})
+                          &sncoll, &scd, &sjob, &sstatus, &srng, &adj_mem, &adj_cols, &adj_part, &adj_out, &rf32, &ac32,
+                          &dcc32, &exact_flag})
             b->release();
         for (auto e : evs) (void)hipEventDestroy(e);
         evs.clear();
@@ -133,6 +139,7 @@ struct mq_engine {
             launch_combine_dc(dcc.as<double>(), d, v, (uint32_t)(3 * nodes()), stream);
             HIPCHK(hipGetLastError());
             dc_valid = true;
+            dcc32_state = -1;
             cells_valid = false;
         }
         if (want_cells && !cells_valid) {
@@ -354,7 +361,7 @@ struct mq_engine {
         O.z_exit = o.z_exit; O.scale = o.scale; O.r_leak = o.r_leak; O.w_leak = o.w_leak;
         O.axis_x = meta.axis_x; O.axis_y = meta.axis_y; O.nsteps = o.nsteps; O.ckpt = K;
         BatchP B = batch_params(ph, 1);
-        AdjGrids G{rf.as<double>(), dcc.as<double>(), ac.as<double>()};
+        const AdjGrids G = adj_grids();
         stats = mq_run_stats{};
         stats.n_ions = n;
         HIPCHK(hipEventRecord(ev0, stream));
@@ -370,7 +377,7 @@ struct mq_engine {
         stats.launches = 1;
         stats.ion_steps = (uint64_t)n * o.nsteps;
         stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
-        adj_buf = A; adj_obj = O; adj_B = B; adj_n = n; adj_valid = true;
+        adj_buf = A; adj_obj = O; adj_B = B; adj_G = G; adj_n = n; adj_valid = true;
         return sum[0] / (double)n;
     }
 
@@ -380,9 +387,8 @@ struct mq_engine {
         if (!adj_valid) throw Err(-1, "mq_backward needs a preceding mq_forward on this engine");
         const uint32_t n = adj_n;
         IonState S = state_buffers(n);
-        AdjGrids G{rf.as<double>(), dcc.as<double>(), ac.as<double>()};
         HIPCHK(hipEventRecord(ev0, stream));
-        launch_adj_backward(S, jobs.as<JobP>(), adj_B, G, adj_obj, adj_buf, n, dJ / (double)n, stream);
+        launch_adj_backward(S, jobs.as<JobP>(), adj_B, adj_G, adj_obj, adj_buf, n, dJ / (double)n, stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ev1, stream));
         double g[5];
@@ -459,6 +465,50 @@ struct mq_engine {
             cur ^= 1;
             if (!resort) chunk = std::min(cap, chunk * 2u);
         }
+    }
+
+    // True when every value of the n doubles at a is exactly a float.
+    bool exact_f32(const double* a, uint64_t n)
+    {
+        exact_flag.ensure(4);
+        launch_fill_u32(exact_flag.as<uint32_t>(), 1u, 1, stream);
+        launch_exact_f32(a, n, exact_flag.as<uint32_t>(), stream);
+        uint32_t f = 0;
+        HIPCHK(hipMemcpyAsync(&f, exact_flag.p, 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return f != 0;
+    }
+
+    // Node tables of the differentiable form for the current combined DC: the
+    // float copies when RF, AC and the combined DC are all exactly floats (the
+    // kernels convert back to double exactly, so the results are the same bit for
+    // bit), else the double tables.
+    AdjGrids adj_grids()
+    {
+        const uint64_t n3 = 3 * nodes();
+        if (rfac32_state < 0) {
+            rfac32_state = exact_f32(rf.as<double>(), n3) && exact_f32(ac.as<double>(), n3) ? 1 : 0;
+            if (rfac32_state) {
+                rf32.ensure(n3 * sizeof(float));
+                ac32.ensure(n3 * sizeof(float));
+                launch_to_f32(rf.as<double>(), n3, rf32.as<float>(), stream);
+                launch_to_f32(ac.as<double>(), n3, ac32.as<float>(), stream);
+            }
+        }
+        if (rfac32_state == 1 && dcc32_state < 0) {
+            dcc32_state = exact_f32(dcc.as<double>(), n3) ? 1 : 0;
+            if (dcc32_state) {
+                dcc32.ensure(n3 * sizeof(float));
+                launch_to_f32(dcc.as<double>(), n3, dcc32.as<float>(), stream);
+            }
+        }
+        AdjGrids G{rf.as<double>(), dcc.as<double>(), ac.as<double>(), nullptr, nullptr, nullptr, 0};
+        if (rfac32_state == 1 && dcc32_state == 1) {
+            G.rf32 = rf32.as<float>(); G.dc32 = dcc32.as<float>(); G.ac32 = ac32.as<float>();
+            G.f32 = 1;
+        }
+        HIPCHK(hipGetLastError());
+        return G;
     }
 
     void run_batched(const IonState& S, const BatchP& B, uint32_t n, float& kms)

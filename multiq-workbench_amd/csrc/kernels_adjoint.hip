@@ -57,16 +57,20 @@ __device__ __forceinline__ Cell3 cell_of(const BatchP& B, double x, double y, do
 }
 
 // Value (same lerp chain as the forward step) and the Jacobian d out[d] / d x_e.
-__device__ __forceinline__ void sample_jac(const double* __restrict__ F, const Cell3& c, uint32_t sy, uint32_t sz,
+// T: the stored node type. float when every value of the grid is exactly a
+// float (the conversion to double is then exact and the arithmetic, all in
+// double, gives bit for bit what the double tables give, from half the bytes).
+template <typename T>
+__device__ __forceinline__ void sample_jac(const T* __restrict__ F, const Cell3& c, uint32_t sy, uint32_t sz,
                                            double out[3], double jac[3][3])
 {
     const uint32_t b = c.b;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double c000 = F[3 * b + d], c100 = F[3 * (b + 1) + d];
-        const double c010 = F[3 * (b + sy) + d], c110 = F[3 * (b + 1 + sy) + d];
-        const double c001 = F[3 * (b + sz) + d], c101 = F[3 * (b + 1 + sz) + d];
-        const double c011 = F[3 * (b + sy + sz) + d], c111 = F[3 * (b + 1 + sy + sz) + d];
+        const double c000 = (double)F[3 * b + d], c100 = (double)F[3 * (b + 1) + d];
+        const double c010 = (double)F[3 * (b + sy) + d], c110 = (double)F[3 * (b + 1 + sy) + d];
+        const double c001 = (double)F[3 * (b + sz) + d], c101 = (double)F[3 * (b + 1 + sz) + d];
+        const double c011 = (double)F[3 * (b + sy + sz) + d], c111 = (double)F[3 * (b + 1 + sy + sz) + d];
         const double c00 = lerp_d(c000, c100, c.tx), c10 = lerp_d(c010, c110, c.tx);
         const double c01 = lerp_d(c001, c101, c.tx), c11 = lerp_d(c011, c111, c.tx);
         const double lo = lerp_d(c00, c10, c.ty), hi = lerp_d(c01, c11, c.ty);
@@ -80,22 +84,24 @@ __device__ __forceinline__ void sample_jac(const double* __restrict__ F, const C
     }
 }
 
-__device__ __forceinline__ void sample_val(const double* __restrict__ F, const Cell3& c, uint32_t sy, uint32_t sz,
+template <typename T>
+__device__ __forceinline__ void sample_val(const T* __restrict__ F, const Cell3& c, uint32_t sy, uint32_t sz,
                                            double out[3])
 {
     const uint32_t b = c.b;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double c00 = lerp_d(F[3 * b + d], F[3 * (b + 1) + d], c.tx);
-        const double c10 = lerp_d(F[3 * (b + sy) + d], F[3 * (b + 1 + sy) + d], c.tx);
-        const double c01 = lerp_d(F[3 * (b + sz) + d], F[3 * (b + 1 + sz) + d], c.tx);
-        const double c11 = lerp_d(F[3 * (b + sy + sz) + d], F[3 * (b + 1 + sy + sz) + d], c.tx);
+        const double c00 = lerp_d((double)F[3 * b + d], (double)F[3 * (b + 1) + d], c.tx);
+        const double c10 = lerp_d((double)F[3 * (b + sy) + d], (double)F[3 * (b + 1 + sy) + d], c.tx);
+        const double c01 = lerp_d((double)F[3 * (b + sz) + d], (double)F[3 * (b + 1 + sz) + d], c.tx);
+        const double c11 = lerp_d((double)F[3 * (b + sy + sz) + d], (double)F[3 * (b + 1 + sy + sz) + d], c.tx);
         out[d] = lerp_d(lerp_d(c00, c10, c.ty), lerp_d(c01, c11, c.ty), c.tz);
     }
 }
 
 // One forward step (identical arithmetic to k_plain without collisions).
-__device__ __forceinline__ void step_fwd(const BatchP& B, const JobP& J, const AdjGrids& G, double phi, uint32_t n,
+template <typename T>
+__device__ __forceinline__ void step_fwd(const BatchP& B, const JobP& J, const AdjGridsT<T>& G, double phi, uint32_t n,
                                          double& x, double& y, double& z, double& vx, double& vy, double& vz)
 {
     const uint32_t sy = (uint32_t)B.nx, sz = (uint32_t)B.nx * (uint32_t)B.ny;
@@ -138,7 +144,8 @@ __device__ __forceinline__ double objective(const AdjObj& O, double x, double y,
 }
 
 // Forward: N steps per ion, (x, v) stored every K steps (slot s holds step s*K).
-__global__ __launch_bounds__(256) void k_adj_forward(IonState S, const JobP* __restrict__ jobs, BatchP B, AdjGrids G,
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_forward(IonState S, const JobP* __restrict__ jobs, BatchP B, AdjGridsT<T> G,
                                                     AdjObj O, AdjBuf A, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -160,7 +167,8 @@ __global__ __launch_bounds__(256) void k_adj_forward(IonState S, const JobP* __r
 
 // Backward: per ion, segments last to first; per-ion partial gradients and the
 // gradient with respect to the initial state.
-__global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __restrict__ jobs, BatchP B, AdjGrids G,
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __restrict__ jobs, BatchP B, AdjGridsT<T> G,
                                                      AdjObj O, AdjBuf A, uint32_t n, double scale)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -255,13 +263,40 @@ static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b -
 void launch_adj_forward(const IonState& S, const JobP* jobs, const BatchP& B, const AdjGrids& G, const AdjObj& O,
                         const AdjBuf& A, uint32_t n, hipStream_t st)
 {
-    if (n) k_adj_forward<<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, G, O, A, n);
+    if (!n) return;
+    if (G.f32) k_adj_forward<float><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, G.f32s(), O, A, n);
+    else k_adj_forward<double><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, G.f64s(), O, A, n);
 }
 
 void launch_adj_backward(const IonState& S, const JobP* jobs, const BatchP& B, const AdjGrids& G, const AdjObj& O,
                          const AdjBuf& A, uint32_t n, double scale, hipStream_t st)
 {
-    if (n) k_adj_backward<<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, G, O, A, n, scale);
+    if (!n) return;
+    if (G.f32) k_adj_backward<float><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, G.f32s(), O, A, n, scale);
+    else k_adj_backward<double><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, G.f64s(), O, A, n, scale);
+}
+
+// flag = 0 when some value of a is not exactly a float (flag starts at 1).
+__global__ void k_exact_f32(const double* __restrict__ a, uint64_t n, uint32_t* __restrict__ flag)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (double)(float)a[i] != a[i]) *flag = 0u;
+}
+
+__global__ void k_to_f32(const double* __restrict__ a, uint64_t n, float* __restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)a[i];
+}
+
+void launch_exact_f32(const double* a, uint64_t n, uint32_t* flag, hipStream_t st)
+{
+    if (n) k_exact_f32<<<(uint32_t)((n + 255) / 256), 256, 0, st>>>(a, n, flag);
+}
+
+void launch_to_f32(const double* a, uint64_t n, float* out, hipStream_t st)
+{
+    if (n) k_to_f32<<<(uint32_t)((n + 255) / 256), 256, 0, st>>>(a, n, out);
 }
 
 void launch_colsum(const double* const* cols_dev, uint32_t ncols, uint32_t n, double* part, double* out, hipStream_t st)

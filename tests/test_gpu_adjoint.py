@@ -40,6 +40,29 @@ def test_adjoint_matches_dual_numbers(c1, oracle_mod):
     assert np.allclose(gstate, gsr, rtol=1e-8, atol=1e-10 * sscale)
 
 
+def test_float_tables_match_dual_numbers(oracle_mod):
+    """A grid whose basis values are all exactly floats runs the differentiable form on float
+    copies of the node tables (half the bytes per gather); the arithmetic is the same FP64, so
+    the bars of the double-table test above hold unchanged."""
+    g = synth.c1_grid()
+    g["rf"] = g["rf"].astype(np.float32).astype(np.float64)
+    g["ac"] = g["ac"].astype(np.float32).astype(np.float64)
+    g["dc"] = [d.astype(np.float32).astype(np.float64) if d is not None else None for d in g["dc"]]
+    e = mqhip.Engine(g)
+    try:
+        job, ph, obj = setting()
+        job.dc_scale *= 5.0   # the gate voltage on the job; the combined DC (1 x basis) stays exactly float
+        print("[gpu] adjoint on float node tables vs dual numbers", flush=True)
+        J = e.forward(job, [0, 1, 0, 0], ph, obj)
+        grad, gstate = e.backward(1.0, state=True)
+    finally:
+        e.close()
+    Jr, gr, gsr = oracle_mod.grad_reference(g, job, [0, 1, 0, 0], ph, obj, 8)
+    assert J == pytest.approx(Jr, rel=1e-12)
+    assert np.allclose(grad, gr, rtol=1e-9, atol=1e-12 * np.abs(gr).max())
+    assert np.allclose(gstate, gsr, rtol=1e-8, atol=1e-10 * np.abs(gsr).max())
+
+
 def test_adjoint_matches_central_differences(c1):
     """Central differences with h = 1e-6 relative and their Richardson extrapolation (h, h/2).
     Operating point with a 0.5 V gate: no ion reaches the barrier top, so the objective is
