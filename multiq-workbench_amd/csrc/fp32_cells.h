@@ -20,6 +20,30 @@ __device__ __forceinline__ void load_cell(const float4* __restrict__ cells, int3
     for (int q = 0; q < CELL_F4; ++q) R[q] = src[q];
 }
 
+// Rank of live-list entry i within its re-sort bin: hist[bin] counts the bin's
+// entries and rank_of[i] is this entry's place among them. Lanes of a wavefront
+// that share a bin take their ranks with one atomic: the bin of the lowest
+// pending lane is peeled until every lane has its rank (freshly injected ions
+// all start in a few bins, and one atomic per ion serialised on those counters).
+// All active lanes of the wavefront must call this together.
+__device__ __forceinline__ void bin_rank(uint32_t* __restrict__ hist, uint32_t bin, uint32_t i,
+                                         uint32_t* __restrict__ rank_of)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t todo = __ballot(1);
+    for (;;) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t lb = (uint32_t)__shfl((int)bin, leader);
+        const uint64_t same = __ballot(bin == lb) & todo;
+        uint32_t base = 0;
+        if (lane == (uint32_t)leader) base = atomicAdd(&hist[lb], (uint32_t)__popcll(same));
+        base = (uint32_t)__shfl((int)base, leader);
+        if ((same >> lane) & 1ull) rank_of[i] = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+        if (todo == 0) break;
+    }
+}
+
 __device__ __forceinline__ float rget(const float4 R[CELL_F4], int k)
 {
     const float4 v = R[k >> 2];

@@ -49,6 +49,12 @@ struct SortGeom {
 void launch_sort(const IonState& S, const BatchP& B, const uint32_t* list, const uint32_t* ctr, uint32_t* out,
                  uint32_t* bin_of, uint32_t* rank_of, uint32_t* hist, uint32_t* bsum, const SortGeom& G,
                  uint32_t nmax, hipStream_t st);
+// Pieces of launch_sort for callers with their own binning kernel: clear the
+// histogram; then, after bin_of / rank_of / hist are filled, turn counts into
+// offsets and place every entry.
+void launch_fill_hist(uint32_t* hist, uint32_t nb, hipStream_t st);
+void launch_sort_place(const uint32_t* list, const uint32_t* ctr, uint32_t* out, const uint32_t* bin_of,
+                       const uint32_t* rank_of, uint32_t* hist, uint32_t* bsum, uint32_t nb, uint32_t nmax, hipStream_t st);
 void launch_prepare(const WorkerArgs& a, hipStream_t st);
 void launch_gather_probe(const float4* cells, uint64_t window, uint32_t iters, float* out, uint32_t grid,
                          uint32_t block, hipStream_t st);

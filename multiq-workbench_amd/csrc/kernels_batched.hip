@@ -437,23 +437,7 @@ __global__ void k_bin(IonState S, BatchP B, const uint32_t* __restrict__ list, c
     const int ck = min((int)fmin(fmax(S.z[id] * B.inv_h, 0.0), (double)(B.nz - 1)), B.nz - 2);
     const uint32_t bin = (((uint32_t)ck >> G.sz) * G.nby + ((uint32_t)cj >> G.sy)) * G.nbx + ((uint32_t)ci >> G.sx);
     bin_of[i] = bin;
-    // Lanes of a wavefront that share a bin take their ranks with one atomic:
-    // peel the bin of the lowest pending lane until every lane has its rank.
-    // (Freshly injected ions all start in a few bins; one atomic per ion
-    // serialised on those counters.)
-    const uint32_t lane = threadIdx.x & 63;
-    uint64_t todo = __ballot(1);
-    for (;;) {
-        const int leader = __ffsll((unsigned long long)todo) - 1;
-        const uint32_t lb = (uint32_t)__shfl((int)bin, leader);
-        const uint64_t same = __ballot(bin == lb) & todo;
-        uint32_t base = 0;
-        if (lane == (uint32_t)leader) base = atomicAdd(&hist[lb], (uint32_t)__popcll(same));
-        base = (uint32_t)__shfl((int)base, leader);
-        if ((same >> lane) & 1ull) rank_of[i] = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
-        todo &= ~same;
-        if (todo == 0) break;
-    }
+    bin_rank(hist, bin, i, rank_of);
 }
 
 // Exclusive scan of hist in place: blocks of 1024 entries (256 threads x 4).
@@ -520,18 +504,29 @@ __global__ void k_scatter(const uint32_t* __restrict__ list, const uint32_t* __r
 
 static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
-void launch_sort(const IonState& S, const BatchP& B, const uint32_t* list, const uint32_t* ctr, uint32_t* out,
-                 uint32_t* bin_of, uint32_t* rank_of, uint32_t* hist, uint32_t* bsum, const SortGeom& G,
-                 uint32_t nmax, hipStream_t st)
+void launch_fill_hist(uint32_t* hist, uint32_t nb, hipStream_t st)
 {
-    const uint32_t nb = G.nbins;
     k_fill_hist<<<nblk(nb, 256), 256, 0, st>>>(hist, nb);
-    if (nmax) k_bin<<<nblk(nmax, 256), 256, 0, st>>>(S, B, list, ctr, bin_of, rank_of, hist, G);
+}
+
+void launch_sort_place(const uint32_t* list, const uint32_t* ctr, uint32_t* out, const uint32_t* bin_of,
+                       const uint32_t* rank_of, uint32_t* hist, uint32_t* bsum, uint32_t nb, uint32_t nmax, hipStream_t st)
+{
     const uint32_t nblocks = nblk(nb, 1024);
     k_scan_blocks<<<nblocks, 256, 0, st>>>(hist, bsum, nb);
     k_scan_sums<<<1, 1024, 0, st>>>(bsum, nblocks);
     k_scan_add<<<nblk(nb, 256), 256, 0, st>>>(hist, bsum, nb);
     if (nmax) k_scatter<<<nblk(nmax, 256), 256, 0, st>>>(list, ctr, bin_of, rank_of, hist, out);
+}
+
+void launch_sort(const IonState& S, const BatchP& B, const uint32_t* list, const uint32_t* ctr, uint32_t* out,
+                 uint32_t* bin_of, uint32_t* rank_of, uint32_t* hist, uint32_t* bsum, const SortGeom& G,
+                 uint32_t nmax, hipStream_t st)
+{
+    const uint32_t nb = G.nbins;
+    launch_fill_hist(hist, nb, st);
+    if (nmax) k_bin<<<nblk(nmax, 256), 256, 0, st>>>(S, B, list, ctr, bin_of, rank_of, hist, G);
+    launch_sort_place(list, ctr, out, bin_of, rank_of, hist, bsum, nb, nmax, st);
 }
 
 void launch_build_cells(float4* cells, const double* rf, const double* dc, const double* ac, const uint8_t* mask,

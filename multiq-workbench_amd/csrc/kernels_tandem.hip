@@ -237,6 +237,7 @@ struct TBatchArgs {
     uint32_t* ctr_in;
     uint32_t* ctr_out;
     uint32_t chunk, period;
+    uint32_t nqueues;  // 1, or MAX_QUEUES contiguous segments of a re-sorted list (one per XCD)
 };
 
 __device__ __forceinline__ int32_t tab_of(const TandemP& T, uint32_t reg)
@@ -303,6 +304,8 @@ __global__ __launch_bounds__(256) void k_tandem_batched(TBatchArgs A)
     const Tables C = {A.cells[0], A.cells[1], A.cells[2], A.cells[3]};
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t count = A.ctr_in[CTR_COUNT];
+    const uint32_t nq = A.nqueues;
+    uint32_t q = nq > 1 ? blockIdx.x % nq : 0, tried = 0;
     const double dt = T.dt;
     TLane L;
     float4 R[CELL_F4];
@@ -310,19 +313,25 @@ __global__ __launch_bounds__(256) void k_tandem_batched(TBatchArgs A)
     L.cell = -1;
     L.tab = -1;
     for (;;) {
+        // refill idle lanes: with several queues the (re-sorted) list is cut into
+        // contiguous segments and the wavefronts of one XCD start on their own
+        // segment, then steal, as in k_batched
         const uint64_t idle = __ballot(!have);
         if (idle && !drained) {
             const uint32_t nidle = (uint32_t)__popcll(idle);
             const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1;
+            const uint32_t s0 = (uint32_t)((uint64_t)count * q / nq), s1 = (uint32_t)((uint64_t)count * (q + 1) / nq);
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&A.ctr_in[0], nidle);
+            if (lane == leader) base = atomicAdd(&A.ctr_in[q * CTR_Q], nidle);
             base = __shfl(base, (int)leader);
-            if (base >= count) {
-                drained = true;
+            const uint32_t avail = s1 - s0;
+            if (base >= avail) {
+                if (++tried >= nq) drained = true;
+                else q = (q + 1 == nq) ? 0 : q + 1;
             } else if (!have) {
                 const uint32_t slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (slot < count) {
-                    const uint32_t id = A.list_in[slot];
+                if (slot < avail) {
+                    const uint32_t id = A.list_in[s0 + slot];
                     const uint32_t jk = S.job[id];
                     const JobP& J = A.jobs[jk];
                     L.id = id;
@@ -475,14 +484,45 @@ void launch_tandem_plain(const IonState& S, const JobP* jobs, const TandemP& T, 
 void launch_tandem_batched(const IonState& S, const JobP* jobs, const TandemP& T, const TGrid& cube,
                            const TGrid& port, const float4* const cells[4], unsigned long long* ctr,
                            const uint32_t* list_in, uint32_t* list_out, uint32_t* ctr_in, uint32_t* ctr_out,
-                           uint32_t chunk, uint32_t period, uint32_t grid, uint32_t block, hipStream_t st)
+                           uint32_t chunk, uint32_t period, uint32_t nqueues, uint32_t grid, uint32_t block,
+                           hipStream_t st)
 {
     TBatchArgs a;
     a.S = S; a.jobs = jobs; a.T = T; a.cube = cube; a.port = port;
     for (int k = 0; k < 4; ++k) a.cells[k] = cells[k];
     a.ctr = ctr; a.list_in = list_in; a.list_out = list_out; a.ctr_in = ctr_in; a.ctr_out = ctr_out;
-    a.chunk = chunk; a.period = period;
+    a.chunk = chunk; a.period = period; a.nqueues = nqueues;
     k_tandem_batched<<<grid, block, 0, st>>>(a);
+}
+
+// Re-sort key of a live tandem ion: its cell table (splitter, combiner, driven
+// port, idle port: ions of different ports that share a table and a cell read
+// the same record) and the 4 x 4 x 4-cell bin of its cell in that table's grid.
+__global__ void k_tandem_bin(IonState S, TandemP T, TGrid cube, TGrid port, const uint32_t* __restrict__ list,
+                             const uint32_t* __restrict__ ctr, uint32_t* __restrict__ bin_of,
+                             uint32_t* __restrict__ rank_of, uint32_t* __restrict__ hist, uint32_t nbc, uint32_t nbp)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ctr[CTR_COUNT]) return;
+    const uint32_t id = list[i];
+    const uint32_t reg = S.region[id];
+    const int32_t t = tab_of(T, reg);
+    const TGrid& g = t < 2 ? cube : port;
+    const int ci = min((int)fmin(fmax(S.x[id] * g.inv_h, 0.0), (double)(g.nx - 1)), g.nx - 2);
+    const int cj = min((int)fmin(fmax(S.y[id] * g.inv_h, 0.0), (double)(g.ny - 1)), g.ny - 2);
+    const int ck = min((int)fmin(fmax(S.z[id] * g.inv_h, 0.0), (double)(g.nz - 1)), g.nz - 2);
+    const uint32_t nbx = (uint32_t)((g.nx - 2) >> 2) + 1, nby = (uint32_t)((g.ny - 2) >> 2) + 1;
+    const uint32_t off = t == 0 ? 0u : t == 1 ? nbc : t == 2 ? 2u * nbc : 2u * nbc + nbp;
+    const uint32_t bin = off + (((uint32_t)ck >> 2) * nby + ((uint32_t)cj >> 2)) * nbx + ((uint32_t)ci >> 2);
+    bin_of[i] = bin;
+    bin_rank(hist, bin, i, rank_of);
+}
+
+void launch_tandem_bin(const IonState& S, const TandemP& T, const TGrid& cube, const TGrid& port, const uint32_t* list,
+                       const uint32_t* ctr, uint32_t* bin_of, uint32_t* rank_of, uint32_t* hist, uint32_t nbc,
+                       uint32_t nbp, uint32_t nmax, hipStream_t st)
+{
+    if (nmax) k_tandem_bin<<<(nmax + 255) / 256, 256, 0, st>>>(S, T, cube, port, list, ctr, bin_of, rank_of, hist, nbc, nbp);
 }
 
 void launch_tandem_totals(const IonState& S, uint32_t n, unsigned long long* tot, hipStream_t st)
