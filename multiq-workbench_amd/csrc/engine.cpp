@@ -59,10 +59,10 @@ struct mq_engine {
     DevBuf adj_mem, adj_cols, adj_part, adj_out;
     AdjBuf adj_buf{};
     AdjGrids adj_G{};
-    // float copies of the node tables for the differentiable form, used when every
-    // value is exactly a float: -1 not checked yet, 0 not exact, 1 copies valid
+    // float cell records for the differentiable form, used when every value is
+    // exactly a float: -1 not checked yet, 0 not exact, 1 exact (records built)
     int rfac32_state = -1, dcc32_state = -1;
-    DevBuf rf32, ac32, dcc32, exact_flag;
+    DevBuf adj_rec, exact_flag;
     std::vector<hipEvent_t> evs;
     mq_run_stats stats{};
 
@@ -111,8 +111,8 @@ struct mq_engine {
     {
         for (DevBuf* b : {&rf, &ac, &dcb[0], &dcb[1], &dcb[2], &dcb[3], &mask, &dcc, &cells, &jobs, &acc, &list[0],
                           &list[1], &list[2], &ctr, &bin_of, &rank_of, &hist, &bsum, &sx, &sy, &sz, &sphi, &svx, &svy, &svz, &sfx, &sfy, &sfz, &ssteps,
-                          &sncoll, &scd, &sjob, &sstatus, &srng, &adj_mem, &adj_cols, &adj_part, &adj_out, &rf32, &ac32,
-                          &dcc32, &exact_flag})
+                          &sncoll, &scd, &sjob, &sstatus, &srng, &adj_mem, &adj_cols, &adj_part, &adj_out, &adj_rec,
+                          &exact_flag})
             b->release();
         for (auto e : evs) (void)hipEventDestroy(e);
         evs.clear();
@@ -479,34 +479,25 @@ struct mq_engine {
         return f != 0;
     }
 
-    // Node tables of the differentiable form for the current combined DC: the
-    // float copies when RF, AC and the combined DC are all exactly floats (the
-    // kernels convert back to double exactly, so the results are the same bit for
-    // bit), else the double tables.
+    // Tables of the differentiable form for the current combined DC: per-cell
+    // float records of the corner values when RF, AC and the combined DC are all
+    // exactly floats (the kernels convert back to double exactly, so the results
+    // are the same bit for bit), else the double node tables.
     AdjGrids adj_grids()
     {
         const uint64_t n3 = 3 * nodes();
-        if (rfac32_state < 0) {
-            rfac32_state = exact_f32(rf.as<double>(), n3) && exact_f32(ac.as<double>(), n3) ? 1 : 0;
-            if (rfac32_state) {
-                rf32.ensure(n3 * sizeof(float));
-                ac32.ensure(n3 * sizeof(float));
-                launch_to_f32(rf.as<double>(), n3, rf32.as<float>(), stream);
-                launch_to_f32(ac.as<double>(), n3, ac32.as<float>(), stream);
-            }
-        }
+        if (rfac32_state < 0) rfac32_state = exact_f32(rf.as<double>(), n3) && exact_f32(ac.as<double>(), n3) ? 1 : 0;
         if (rfac32_state == 1 && dcc32_state < 0) {
             dcc32_state = exact_f32(dcc.as<double>(), n3) ? 1 : 0;
             if (dcc32_state) {
-                dcc32.ensure(n3 * sizeof(float));
-                launch_to_f32(dcc.as<double>(), n3, dcc32.as<float>(), stream);
+                const uint64_t nc = (uint64_t)(nx - 1) * (ny - 1) * (nz - 1);
+                adj_rec.ensure(nc * ADJ_REC_F4 * sizeof(float4));
+                launch_build_adj_recs(adj_rec.as<float4>(), rf.as<double>(), dcc.as<double>(), ac.as<double>(), nx, ny,
+                                      nz, stream);
             }
         }
-        AdjGrids G{rf.as<double>(), dcc.as<double>(), ac.as<double>(), nullptr, nullptr, nullptr, 0};
-        if (rfac32_state == 1 && dcc32_state == 1) {
-            G.rf32 = rf32.as<float>(); G.dc32 = dcc32.as<float>(); G.ac32 = ac32.as<float>();
-            G.f32 = 1;
-        }
+        AdjGrids G{rf.as<double>(), dcc.as<double>(), ac.as<double>(), nullptr};
+        if (rfac32_state == 1 && dcc32_state == 1) G.rec = adj_rec.as<float4>();
         HIPCHK(hipGetLastError());
         return G;
     }

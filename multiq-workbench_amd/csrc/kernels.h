@@ -60,18 +60,13 @@ void launch_gather_probe(const float4* cells, uint64_t window, uint32_t iters, f
                          uint32_t block, hipStream_t st);
 void launch_batched(const WorkerArgs& a, uint32_t grid, uint32_t block, hipStream_t st);
 // Differentiable form (kernels_adjoint.hip).
-template <typename T>
-struct AdjGridsT {
-    const T *rf, *dc, *ac;
-};
-// The node tables of the differentiable form: double, or float copies when
-// every value of all three is exactly a float (f32 != 0).
+// The node tables of the differentiable form (double), and per-cell float
+// records of the corner values when every value of all three tables is exactly
+// a float (rec != nullptr): ADJ_REC_F4 float4 per cell, 72 floats.
+constexpr int ADJ_REC_F4 = 18;
 struct AdjGrids {
     const double *rf, *dc, *ac;
-    const float *rf32, *dc32, *ac32;
-    int f32;
-    AdjGridsT<double> f64s() const { return {rf, dc, ac}; }
-    AdjGridsT<float> f32s() const { return {rf32, dc32, ac32}; }
+    const float4* rec;
 };
 struct AdjObj {
     double z_exit, scale, r_leak, w_leak, axis_x, axis_y;
@@ -91,6 +86,7 @@ void launch_adj_backward(const IonState& S, const JobP* jobs, const BatchP& B, c
                          const AdjBuf& A, uint32_t n, double scale, hipStream_t st);
 void launch_colsum(const double* const* cols_dev, uint32_t ncols, uint32_t n, double* part, double* out, hipStream_t st);
 void launch_exact_f32(const double* a, uint64_t n, uint32_t* flag, hipStream_t st);
-void launch_to_f32(const double* a, uint64_t n, float* out, hipStream_t st);
+void launch_build_adj_recs(float4* rec, const double* rf, const double* dc, const double* ac, int nx, int ny, int nz,
+                           hipStream_t st);
 
 }  // namespace mq

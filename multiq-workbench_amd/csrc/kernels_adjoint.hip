@@ -26,7 +26,8 @@
 namespace mq {
 
 struct Cell3 {
-    uint32_t b;
+    uint32_t b;     // node index of the lower corner
+    uint32_t cell;  // cell index i + (nx - 1) (j + (ny - 1) k)
     double tx, ty, tz;
     double dx, dy, dz;  // d(t)/d(x): inv_h inside the grid, 0 where the coordinate is clamped
 };
@@ -53,63 +54,137 @@ __device__ __forceinline__ Cell3 cell_of(const BatchP& B, double x, double y, do
     locate_d(y, B.ny, B.inv_h, B.nearest, j, c.ty, c.dy);
     locate_d(z, B.nz, B.inv_h, B.nearest, k, c.tz, c.dz);
     c.b = (uint32_t)i + (uint32_t)B.nx * ((uint32_t)j + (uint32_t)B.ny * (uint32_t)k);
+    c.cell = (uint32_t)i + (uint32_t)(B.nx - 1) * ((uint32_t)j + (uint32_t)(B.ny - 1) * (uint32_t)k);
     return c;
 }
 
-// Value (same lerp chain as the forward step) and the Jacobian d out[d] / d x_e.
-// T: the stored node type. float when every value of the grid is exactly a
-// float (the conversion to double is then exact and the arithmetic, all in
-// double, gives bit for bit what the double tables give, from half the bytes).
-template <typename T>
-__device__ __forceinline__ void sample_jac(const T* __restrict__ F, const Cell3& c, uint32_t sy, uint32_t sz,
-                                           double out[3], double jac[3][3])
+// Trilinear value of one component from its 8 corner values (corner index
+// dx + 2 dy + 4 dz), in the lerp order of the forward step, and the derivatives
+// by x, y, z.
+__device__ __forceinline__ double tri_val(const double f[8], const Cell3& c)
 {
-    const uint32_t b = c.b;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double c000 = (double)F[3 * b + d], c100 = (double)F[3 * (b + 1) + d];
-        const double c010 = (double)F[3 * (b + sy) + d], c110 = (double)F[3 * (b + 1 + sy) + d];
-        const double c001 = (double)F[3 * (b + sz) + d], c101 = (double)F[3 * (b + 1 + sz) + d];
-        const double c011 = (double)F[3 * (b + sy + sz) + d], c111 = (double)F[3 * (b + 1 + sy + sz) + d];
+    const double c00 = lerp_d(f[0], f[1], c.tx), c10 = lerp_d(f[2], f[3], c.tx);
+    const double c01 = lerp_d(f[4], f[5], c.tx), c11 = lerp_d(f[6], f[7], c.tx);
+    return lerp_d(lerp_d(c00, c10, c.ty), lerp_d(c01, c11, c.ty), c.tz);
+}
+
+__device__ __forceinline__ double tri_jac(const double f[8], const Cell3& c, double jac[3])
+{
+    const double c000 = f[0], c100 = f[1], c010 = f[2], c110 = f[3];
+    const double c001 = f[4], c101 = f[5], c011 = f[6], c111 = f[7];
+    {
         const double c00 = lerp_d(c000, c100, c.tx), c10 = lerp_d(c010, c110, c.tx);
         const double c01 = lerp_d(c001, c101, c.tx), c11 = lerp_d(c011, c111, c.tx);
         const double lo = lerp_d(c00, c10, c.ty), hi = lerp_d(c01, c11, c.ty);
-        out[d] = lerp_d(lo, hi, c.tz);
         const double dtx = lerp_d(lerp_d(c100 - c000, c110 - c010, c.ty), lerp_d(c101 - c001, c111 - c011, c.ty), c.tz);
         const double dty = lerp_d(c10 - c00, c11 - c01, c.tz);
         const double dtz = hi - lo;
-        jac[d][0] = dtx * c.dx;
-        jac[d][1] = dty * c.dy;
-        jac[d][2] = dtz * c.dz;
+        jac[0] = dtx * c.dx;
+        jac[1] = dty * c.dy;
+        jac[2] = dtz * c.dz;
+        return lerp_d(lo, hi, c.tz);
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void sample_val(const T* __restrict__ F, const Cell3& c, uint32_t sy, uint32_t sz,
-                                           double out[3])
+// The fields of the three bases at a cell: from the double node tables (any
+// grid), or from per-cell float records of the 8 corner node values (grids
+// whose values are all exactly floats: each value converts back to the same
+// double, so the arithmetic and results are those of the node tables, from 18
+// contiguous 16-byte loads instead of 72 scattered ones).
+struct NodeGrids {
+    const double *rf, *dc, *ac;
+    uint32_t sy, sz;
+};
+struct CellRecs {
+    const float4* rec;   // ADJ_REC_F4 per cell: [basis][corner][component]
+};
+
+__device__ __forceinline__ void corners(const double* __restrict__ F, uint32_t b, uint32_t sy, uint32_t sz, int d,
+                                        double f[8])
 {
-    const uint32_t b = c.b;
+    f[0] = F[3 * b + d]; f[1] = F[3 * (b + 1) + d];
+    f[2] = F[3 * (b + sy) + d]; f[3] = F[3 * (b + 1 + sy) + d];
+    f[4] = F[3 * (b + sz) + d]; f[5] = F[3 * (b + 1 + sz) + d];
+    f[6] = F[3 * (b + sy + sz) + d]; f[7] = F[3 * (b + 1 + sy + sz) + d];
+}
+
+__device__ __forceinline__ void load_rec(const CellRecs& G, const Cell3& c, float v[72])
+{
+    const float4* p = G.rec + (size_t)c.cell * ADJ_REC_F4;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double c00 = lerp_d((double)F[3 * b + d], (double)F[3 * (b + 1) + d], c.tx);
-        const double c10 = lerp_d((double)F[3 * (b + sy) + d], (double)F[3 * (b + 1 + sy) + d], c.tx);
-        const double c01 = lerp_d((double)F[3 * (b + sz) + d], (double)F[3 * (b + 1 + sz) + d], c.tx);
-        const double c11 = lerp_d((double)F[3 * (b + sy + sz) + d], (double)F[3 * (b + 1 + sy + sz) + d], c.tx);
-        out[d] = lerp_d(lerp_d(c00, c10, c.ty), lerp_d(c01, c11, c.ty), c.tz);
+    for (int q = 0; q < ADJ_REC_F4; ++q) {
+        const float4 t = p[q];
+        v[4 * q] = t.x; v[4 * q + 1] = t.y; v[4 * q + 2] = t.z; v[4 * q + 3] = t.w;
     }
+}
+
+__device__ __forceinline__ void sample3(const NodeGrids& G, const Cell3& c, double e[3][3])
+{
+    const double* F[3] = {G.rf, G.dc, G.ac};
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double f[8];
+            corners(F[q], c.b, G.sy, G.sz, d, f);
+            e[q][d] = tri_val(f, c);
+        }
+}
+
+__device__ __forceinline__ void sample3(const CellRecs& G, const Cell3& c, double e[3][3])
+{
+    float v[72];
+    load_rec(G, c, v);
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double f[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) f[k] = (double)v[24 * q + 3 * k + d];
+            e[q][d] = tri_val(f, c);
+        }
+}
+
+__device__ __forceinline__ void sample3_jac(const NodeGrids& G, const Cell3& c, double e[3][3], double j[3][3][3])
+{
+    const double* F[3] = {G.rf, G.dc, G.ac};
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double f[8];
+            corners(F[q], c.b, G.sy, G.sz, d, f);
+            e[q][d] = tri_jac(f, c, j[q][d]);
+        }
+}
+
+__device__ __forceinline__ void sample3_jac(const CellRecs& G, const Cell3& c, double e[3][3], double j[3][3][3])
+{
+    float v[72];
+    load_rec(G, c, v);
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double f[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) f[k] = (double)v[24 * q + 3 * k + d];
+            e[q][d] = tri_jac(f, c, j[q][d]);
+        }
 }
 
 // One forward step (identical arithmetic to k_plain without collisions).
-template <typename T>
-__device__ __forceinline__ void step_fwd(const BatchP& B, const JobP& J, const AdjGridsT<T>& G, double phi, uint32_t n,
+template <class G_>
+__device__ __forceinline__ void step_fwd(const BatchP& B, const JobP& J, const G_& G, double phi, uint32_t n,
                                          double& x, double& y, double& z, double& vx, double& vy, double& vz)
 {
-    const uint32_t sy = (uint32_t)B.nx, sz = (uint32_t)B.nx * (uint32_t)B.ny;
     const Cell3 c = cell_of(B, x, y, z);
-    double er[3], ed[3], ea[3];
-    sample_val(G.rf, c, sy, sz, er);
-    sample_val(G.dc, c, sy, sz, ed);
-    sample_val(G.ac, c, sy, sz, ea);
+    double e3[3][3];
+    sample3(G, c, e3);
+    const double* er = e3[0];
+    const double* ed = e3[1];
+    const double* ea = e3[2];
     const double a_rf = J.v_rf * sin2pi(frac_d(J.w_rf * (double)n));
     const double a_ac = J.v_ac * sin2pi(frac_d(fma(J.w_ac, (double)n, phi)));
     double E;
@@ -144,8 +219,8 @@ __device__ __forceinline__ double objective(const AdjObj& O, double x, double y,
 }
 
 // Forward: N steps per ion, (x, v) stored every K steps (slot s holds step s*K).
-template <typename T>
-__global__ __launch_bounds__(256) void k_adj_forward(IonState S, const JobP* __restrict__ jobs, BatchP B, AdjGridsT<T> G,
+template <class G_>
+__global__ __launch_bounds__(256) void k_adj_forward(IonState S, const JobP* __restrict__ jobs, BatchP B, G_ G,
                                                     AdjObj O, AdjBuf A, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -167,15 +242,14 @@ __global__ __launch_bounds__(256) void k_adj_forward(IonState S, const JobP* __r
 
 // Backward: per ion, segments last to first; per-ion partial gradients and the
 // gradient with respect to the initial state.
-template <typename T>
-__global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __restrict__ jobs, BatchP B, AdjGridsT<T> G,
+template <class G_>
+__global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __restrict__ jobs, BatchP B, G_ G,
                                                      AdjObj O, AdjBuf A, uint32_t n, double scale)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const JobP J = jobs[S.job[i]];
     const double phi = S.phi[i];
-    const uint32_t sy = (uint32_t)B.nx, sz = (uint32_t)B.nx * (uint32_t)B.ny;
     double g[3];
     (void)objective(O, A.fin[0][i], A.fin[1][i], A.fin[2][i], g);
     double lx = scale * g[0], ly = scale * g[1], lz = scale * g[2], lvx = 0.0, lvy = 0.0, lvz = 0.0;
@@ -196,10 +270,14 @@ __global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __
         for (int s = (int)s1 - 1; s >= (int)s0; --s) {
             const size_t o = (size_t)((uint32_t)s - s0) * n + i;
             const Cell3 c = cell_of(B, A.seg[0][o], A.seg[1][o], A.seg[2][o]);
-            double er[3], ed[3], ea[3], jr[3][3], jd[3][3], ja[3][3];
-            sample_jac(G.rf, c, sy, sz, er, jr);
-            sample_jac(G.dc, c, sy, sz, ed, jd);
-            sample_jac(G.ac, c, sy, sz, ea, ja);
+            double e3[3][3], j3[3][3][3];
+            sample3_jac(G, c, e3, j3);
+            const double* er = e3[0];
+            const double* ed = e3[1];
+            const double* ea = e3[2];
+            const double (&jr)[3][3] = j3[0];
+            const double (&jd)[3][3] = j3[1];
+            const double (&ja)[3][3] = j3[2];
             double srf, crf, sac, cac;
             sincos2pi(frac_d(J.w_rf * (double)s), srf, crf);
             sincos2pi(frac_d(fma(J.w_ac, (double)s, phi)), sac, cac);
@@ -260,20 +338,25 @@ __global__ void k_colsum_final(const double* __restrict__ part, uint32_t nblocks
 
 static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
+static NodeGrids node_grids(const BatchP& B, const AdjGrids& G)
+{
+    return NodeGrids{G.rf, G.dc, G.ac, (uint32_t)B.nx, (uint32_t)B.nx * (uint32_t)B.ny};
+}
+
 void launch_adj_forward(const IonState& S, const JobP* jobs, const BatchP& B, const AdjGrids& G, const AdjObj& O,
                         const AdjBuf& A, uint32_t n, hipStream_t st)
 {
     if (!n) return;
-    if (G.f32) k_adj_forward<float><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, G.f32s(), O, A, n);
-    else k_adj_forward<double><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, G.f64s(), O, A, n);
+    if (G.rec) k_adj_forward<CellRecs><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, CellRecs{G.rec}, O, A, n);
+    else k_adj_forward<NodeGrids><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, node_grids(B, G), O, A, n);
 }
 
 void launch_adj_backward(const IonState& S, const JobP* jobs, const BatchP& B, const AdjGrids& G, const AdjObj& O,
                          const AdjBuf& A, uint32_t n, double scale, hipStream_t st)
 {
     if (!n) return;
-    if (G.f32) k_adj_backward<float><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, G.f32s(), O, A, n, scale);
-    else k_adj_backward<double><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, G.f64s(), O, A, n, scale);
+    if (G.rec) k_adj_backward<CellRecs><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, CellRecs{G.rec}, O, A, n, scale);
+    else k_adj_backward<NodeGrids><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, node_grids(B, G), O, A, n, scale);
 }
 
 // flag = 0 when some value of a is not exactly a float (flag starts at 1).
@@ -283,10 +366,29 @@ __global__ void k_exact_f32(const double* __restrict__ a, uint64_t n, uint32_t* 
     if (i < n && (double)(float)a[i] != a[i]) *flag = 0u;
 }
 
-__global__ void k_to_f32(const double* __restrict__ a, uint64_t n, float* __restrict__ out)
+// Per-cell float records of the corner node values of the three bases
+// (float index 24 basis + 3 corner + component, corner = dx + 2 dy + 4 dz).
+__global__ void k_build_adj_recs(float4* __restrict__ rec, const double* __restrict__ rf, const double* __restrict__ dc,
+                                 const double* __restrict__ ac, int nx, int ny, int nz)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (float)a[i];
+    const uint32_t cx = (uint32_t)(nx - 1), cy = (uint32_t)(ny - 1), cz = (uint32_t)(nz - 1);
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (uint64_t)cx * cy * cz) return;
+    const uint32_t i = (uint32_t)(c % cx), j = (uint32_t)((c / cx) % cy), k = (uint32_t)(c / ((uint64_t)cx * cy));
+    const uint32_t sy = (uint32_t)nx, sz = (uint32_t)nx * (uint32_t)ny;
+    const uint32_t b = i + sy * (j + (uint32_t)ny * k);
+    const uint32_t node[8] = {b, b + 1, b + sy, b + 1 + sy, b + sz, b + 1 + sz, b + sy + sz, b + 1 + sy + sz};
+    const double* F[3] = {rf, dc, ac};
+    float v[72];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v[24 * q + 3 * kk + d] = (float)F[q][3 * node[kk] + d];
+    float4* dst = rec + c * ADJ_REC_F4;
+#pragma unroll
+    for (int q = 0; q < ADJ_REC_F4; ++q) dst[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
 }
 
 void launch_exact_f32(const double* a, uint64_t n, uint32_t* flag, hipStream_t st)
@@ -294,9 +396,11 @@ void launch_exact_f32(const double* a, uint64_t n, uint32_t* flag, hipStream_t s
     if (n) k_exact_f32<<<(uint32_t)((n + 255) / 256), 256, 0, st>>>(a, n, flag);
 }
 
-void launch_to_f32(const double* a, uint64_t n, float* out, hipStream_t st)
+void launch_build_adj_recs(float4* rec, const double* rf, const double* dc, const double* ac, int nx, int ny, int nz,
+                           hipStream_t st)
 {
-    if (n) k_to_f32<<<(uint32_t)((n + 255) / 256), 256, 0, st>>>(a, n, out);
+    const uint64_t nc = (uint64_t)(nx - 1) * (ny - 1) * (nz - 1);
+    if (nc) k_build_adj_recs<<<(uint32_t)((nc + 127) / 128), 128, 0, st>>>(rec, rf, dc, ac, nx, ny, nz);
 }
 
 void launch_colsum(const double* const* cols_dev, uint32_t ncols, uint32_t n, double* part, double* out, hipStream_t st)
