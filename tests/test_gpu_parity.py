@@ -168,6 +168,31 @@ def test_batched_collision_batching_is_exact(c1):
         e.set_tune(**t0)
 
 
+def test_batched_tail_switch_keeps_results(c1):
+    """A batch larger than a launch's lanes: re-sorted launches with one queue per XCD until the
+    live count fits the lanes, then launches without re-sort and with doubling chunks. Per-ion
+    results equal those of launches that never re-sort."""
+    g, e = c1
+    ph = physics("half_thermal")
+    jobs = jobs_for("half_thermal", 80000)   # 120,000 ions
+    t0 = e.get_tune()
+    say("[gpu] batched path: re-sorted launches, then the tail switch, vs no re-sort")
+    try:
+        # 256 CUs x 3 waves x 64 lanes = 49,152 lanes: the first launches carry more ions than lanes
+        e.set_tune(block=64, waves_per_cu=3, refill_period=8, chunk=64, chunk_cap=64, resort=1, sync_every=1, graph=0)
+        sorted_run = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+        n_sorted = e.stats()["launches"]
+        e.set_tune(block=256, waves_per_cu=8, refill_period=8, chunk=20000, chunk_cap=200000, resort=0,
+                   sync_every=2, graph=0)
+        plain_launches = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+    finally:
+        e.set_tune(**t0)
+    say(f"[gpu]   {n_sorted} launches with re-sort and tail")
+    for k in KEYS + ("steps", "ncoll", "status"):
+        assert np.array_equal(sorted_run[k], plain_launches[k]), k
+    assert sorted_run["steps"].max() > 64 * 8   # the run outlasted several re-sort intervals
+
+
 def test_plain_independent_of_batch(c1):
     g, e = c1
     ph = physics("legacy")
