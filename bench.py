@@ -464,6 +464,9 @@ def main():
     if traffic:
         r["traffic_gbs"] = traffic / (r["avg_launch_ms"] * 1e-3) / 1e9
         r["traffic_frac_of_achievable_table_gather"] = r["traffic_gbs"] / gath["table"]
+        # the same random whole-record reads over a window the Infinity Cache holds: what the
+        # memory system delivers to this access pattern once the re-sort has made it local
+        r["traffic_frac_of_achievable_window_128MB"] = r["traffic_gbs"] / gath["window_128MB"]
     r["valu_busy"] = read_profile_value(a.config, "valu_busy")
     log(f"gather probe: {gath}")
 
