@@ -38,10 +38,12 @@ __device__ __forceinline__ double sel9(const double (&a)[9], uint32_t k)
 // enter 326-356 and leave 357-377). `crash` tests the conductor mask of a
 // grid at a position inside that grid's box. Returns the new status
 // (0 alive, 1 lost, 2 back, 3 detected).
+// mxp, myp: the mouth centre of the ion's port when it is in a port (T.mx[p],
+// T.my[p]; the caller keeps them per region), ignored in the cube.
 template <class V, class Crash>
 __device__ __forceinline__ int tandem_cross(const TandemP& T, const TGrid& cube, const TGrid& port, uint32_t& reg,
                                             uint32_t& last, double& x, double& y, double& z, V& vz, uint32_t job,
-                                            unsigned long long* __restrict__ ctr, Crash crash)
+                                            unsigned long long* __restrict__ ctr, Crash crash, double mxp, double myp)
 {
     if (reg == REG_SPLIT || reg == REG_COMBINE) {
         if (x < 0.0 || y < 0.0 || z < 0.0 || x > cube.lim0 || y > cube.lim1 || z > cube.lim2 || crash(false))
@@ -80,7 +82,7 @@ __device__ __forceinline__ int tandem_cross(const TandemP& T, const TGrid& cube,
     }
     const uint32_t p = reg - 1u;
     unsigned long long* c = ctr + (size_t)p * TC_PORT;
-    const double mx = sel9(T.mx, p), my = sel9(T.my, p);
+    const double mx = mxp, my = myp;
     if (x < 0.0 || y < 0.0 || z < 0.0 || x > port.lim0 || y > port.lim1 || z > port.lim2 || crash(true)) {
         atomicAdd(&c[TC_LOST], 1ull);
         return 1;
@@ -201,7 +203,9 @@ __global__ __launch_bounds__(256) void k_tandem_plain(TPlainArgs A)
                 return H.mask[(uint32_t)rint(x * H.inv_h) +
                               hy * ((uint32_t)rint(y * H.inv_h) + (uint32_t)H.ny * (uint32_t)rint(z * H.inv_h))] != 0;
             };
-            status = tandem_cross(T, A.cube, A.port, reg, last, x, y, z, vz, jk, A.ctr, crash);
+            const bool in_port = reg != REG_SPLIT && reg != REG_COMBINE;
+            status = tandem_cross(T, A.cube, A.port, reg, last, x, y, z, vz, jk, A.ctr, crash,
+                                  in_port ? sel9(T.mx, reg - 1u) : 0.0, in_port ? sel9(T.my, reg - 1u) : 0.0);
             if (status == 0 && st >= T.max_steps) status = 5;
             if (status) break;
         }
@@ -223,7 +227,20 @@ struct TLane {
     float tx, ty, tz, fx, fy, fz, a_rf, a_ac;
     uint32_t cw1, cw2, cw3;
     int32_t cell, tab;
+    // constants of the current region, set on entry: cell table, mouth centre and
+    // excitation (cycles per step; 0 in the cube and idle ports)
+    int32_t rtab;
+    double mx, my, wac;
 };
+
+__device__ __forceinline__ void set_region(const TandemP& T, TLane& L)
+{
+    const bool in_port = L.reg != REG_SPLIT && L.reg != REG_COMBINE;
+    L.wac = w_ac_of(T, L.reg);
+    L.rtab = L.reg == REG_SPLIT ? 0 : L.reg == REG_COMBINE ? 1 : (L.wac > 0.0 ? 2 : 3);
+    L.mx = in_port ? sel9(T.mx, L.reg - 1u) : 0.0;
+    L.my = in_port ? sel9(T.my, L.reg - 1u) : 0.0;
+}
 
 struct TBatchArgs {
     IonState S;
@@ -272,7 +289,7 @@ struct Tables {
 __device__ __forceinline__ void tfetch(const TandemP& T, const TGrid& cube, const TGrid& port, const Tables& C,
                                        TLane& L, float4 R[CELL_F4])
 {
-    const int32_t t = tab_of(T, L.reg);
+    const int32_t t = L.rtab;
     const int32_t c = tlocate(cube, port, T.nearest, L);
     if (c != L.cell || t != L.tab) {
         L.cell = c;
@@ -287,7 +304,7 @@ __device__ __forceinline__ void tphases(const TandemP& T, TLane& L)
     pr = pr - floor(pr);
     L.a_rf = (float)T.v_rf * sin_rev((float)pr);
     L.a_ac = 0.0f;
-    const double wa = w_ac_of(T, L.reg);
+    const double wa = L.wac;
     if (wa > 0.0) {
         double pa = wa * (double)L.st;
         pa = pa - floor(pa);
@@ -346,6 +363,7 @@ __global__ __launch_bounds__(256) void k_tandem_batched(TBatchArgs A)
                     L.k0 = J.key0; L.k1 = J.key1;
                     L.visit = 0;
                     L.cw1 = L.cw2 = L.cw3 = 0;
+                    set_region(T, L);
                     tfetch(T, cube, port, C, L, R);
                     tphases(T, L);
                     have = true;
@@ -411,8 +429,11 @@ __global__ __launch_bounds__(256) void k_tandem_batched(TBatchArgs A)
                 const bool on_conductor = ((__float_as_uint(R[18].x) >> corner) & 1u) != 0u;
                 auto crash = [on_conductor](bool) { return on_conductor; };
                 const uint32_t reg0 = L.reg;
-                status = tandem_cross(T, cube, port, L.reg, L.last, L.x, L.y, L.z, L.vz, L.job, A.ctr, crash);
-                if (status == 0 && L.reg != reg0) tfetch(T, cube, port, C, L, R);
+                status = tandem_cross(T, cube, port, L.reg, L.last, L.x, L.y, L.z, L.vz, L.job, A.ctr, crash, L.mx, L.my);
+                if (status == 0 && L.reg != reg0) {
+                    set_region(T, L);
+                    tfetch(T, cube, port, C, L, R);
+                }
                 if (status == 0) tphases(T, L);
                 if (status == 0 && L.st >= T.max_steps) status = 5;
                 if (status != 0 || L.visit >= A.chunk) stop = true;
