@@ -118,7 +118,28 @@ __device__ __forceinline__ void load_rec(const CellRecs& G, const Cell3& c, floa
     }
 }
 
-__device__ __forceinline__ void sample3(const NodeGrids& G, const Cell3& c, double e[3][3])
+// Per-lane cache of the grid data of one cell. With float records the lane
+// keeps its cell's record in registers and reloads it only when the ion has
+// moved to another cell (the loads are masked to those lanes), which takes the
+// per-step record traffic through the vector L1 down to the cell changes. The
+// node tables are read at each use.
+template <class G_> struct Cache;
+template <> struct Cache<CellRecs> {
+    uint32_t cell = 0xFFFFFFFFu;
+    float v[72];
+};
+template <> struct Cache<NodeGrids> {};
+
+__device__ __forceinline__ void refresh(const CellRecs& G, const Cell3& c, Cache<CellRecs>& k)
+{
+    if (c.cell != k.cell) {
+        k.cell = c.cell;
+        load_rec(G, c, k.v);
+    }
+}
+__device__ __forceinline__ void refresh(const NodeGrids&, const Cell3&, Cache<NodeGrids>&) {}
+
+__device__ __forceinline__ void sample3(const NodeGrids& G, const Cell3& c, const Cache<NodeGrids>&, double e[3][3])
 {
     const double* F[3] = {G.rf, G.dc, G.ac};
 #pragma unroll
@@ -131,22 +152,21 @@ __device__ __forceinline__ void sample3(const NodeGrids& G, const Cell3& c, doub
         }
 }
 
-__device__ __forceinline__ void sample3(const CellRecs& G, const Cell3& c, double e[3][3])
+__device__ __forceinline__ void sample3(const CellRecs&, const Cell3& c, const Cache<CellRecs>& k, double e[3][3])
 {
-    float v[72];
-    load_rec(G, c, v);
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             double f[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) f[k] = (double)v[24 * q + 3 * k + d];
+            for (int kk = 0; kk < 8; ++kk) f[kk] = (double)k.v[24 * q + 3 * kk + d];
             e[q][d] = tri_val(f, c);
         }
 }
 
-__device__ __forceinline__ void sample3_jac(const NodeGrids& G, const Cell3& c, double e[3][3], double j[3][3][3])
+__device__ __forceinline__ void sample3_jac(const NodeGrids& G, const Cell3& c, const Cache<NodeGrids>&, double e[3][3],
+                                            double j[3][3][3])
 {
     const double* F[3] = {G.rf, G.dc, G.ac};
 #pragma unroll
@@ -159,34 +179,37 @@ __device__ __forceinline__ void sample3_jac(const NodeGrids& G, const Cell3& c, 
         }
 }
 
-__device__ __forceinline__ void sample3_jac(const CellRecs& G, const Cell3& c, double e[3][3], double j[3][3][3])
+__device__ __forceinline__ void sample3_jac(const CellRecs&, const Cell3& c, const Cache<CellRecs>& k, double e[3][3],
+                                            double j[3][3][3])
 {
-    float v[72];
-    load_rec(G, c, v);
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             double f[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) f[k] = (double)v[24 * q + 3 * k + d];
+            for (int kk = 0; kk < 8; ++kk) f[kk] = (double)k.v[24 * q + 3 * kk + d];
             e[q][d] = tri_jac(f, c, j[q][d]);
         }
 }
 
-// One forward step (identical arithmetic to k_plain without collisions).
+// One forward step (identical arithmetic to k_plain without collisions). The
+// drive amplitudes do not depend on the cell, so they are computed while the
+// record loads are in flight.
 template <class G_>
-__device__ __forceinline__ void step_fwd(const BatchP& B, const JobP& J, const G_& G, double phi, uint32_t n,
-                                         double& x, double& y, double& z, double& vx, double& vy, double& vz)
+__device__ __forceinline__ void step_fwd(const BatchP& B, const JobP& J, const G_& G, Cache<G_>& K, double phi,
+                                         uint32_t n, double& x, double& y, double& z, double& vx, double& vy,
+                                         double& vz)
 {
     const Cell3 c = cell_of(B, x, y, z);
+    refresh(G, c, K);
+    const double a_rf = J.v_rf * sin2pi(frac_d(J.w_rf * (double)n));
+    const double a_ac = J.v_ac * sin2pi(frac_d(fma(J.w_ac, (double)n, phi)));
     double e3[3][3];
-    sample3(G, c, e3);
+    sample3(G, c, K, e3);
     const double* er = e3[0];
     const double* ed = e3[1];
     const double* ea = e3[2];
-    const double a_rf = J.v_rf * sin2pi(frac_d(J.w_rf * (double)n));
-    const double a_ac = J.v_ac * sin2pi(frac_d(fma(J.w_ac, (double)n, phi)));
     double E;
     E = (a_rf * er[0] + J.s_dc * ed[0]) + a_ac * ea[0]; vx = fma(J.kq, E, vx); x = fma(vx, B.dt, x);
     E = (a_rf * er[1] + J.s_dc * ed[1]) + a_ac * ea[1]; vy = fma(J.kq, E, vy); y = fma(vy, B.dt, y);
@@ -228,20 +251,70 @@ __global__ __launch_bounds__(256) void k_adj_forward(IonState S, const JobP* __r
     const JobP J = jobs[S.job[i]];
     double x = S.x[i], y = S.y[i], z = S.z[i], vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
     const double phi = S.phi[i];
+    Cache<G_> K;
+    // The job parameters must have arrived before the loop: loads still in
+    // flight at the loop head make the wait before their first use in each
+    // step also wait for that step's record loads.
+    asm volatile("" ::"v"(J.kq), "v"(J.v_rf), "v"(J.w_rf), "v"(J.v_ac), "v"(J.w_ac), "v"(J.s_dc), "v"(phi));
     for (uint32_t s = 0; s < O.nsteps; ++s) {
         if (s % O.ckpt == 0) {
             const size_t o = (size_t)(s / O.ckpt) * n + i;
             A.ck[0][o] = x; A.ck[1][o] = y; A.ck[2][o] = z; A.ck[3][o] = vx; A.ck[4][o] = vy; A.ck[5][o] = vz;
         }
-        step_fwd(B, J, G, phi, s, x, y, z, vx, vy, vz);
+        step_fwd(B, J, G, K, phi, s, x, y, z, vx, vy, vz);
     }
     double g[3];
     A.Ji[i] = objective(O, x, y, z, g);
     A.fin[0][i] = x; A.fin[1][i] = y; A.fin[2][i] = z;
 }
 
+// Adjoint variables and per-ion parameter gradients of the backward walk.
+struct Adj {
+    double lx, ly, lz, lvx, lvy, lvz;
+    double gvrf, gfrf, gvac, gfac, gdc;
+};
+
+// One backward step s, from the fields e3 and their derivatives j3 at the
+// recomputed position x_s.
+__device__ __forceinline__ void walk_step(const BatchP& B, const JobP& J, double phi, double two_pi_dt, uint32_t s,
+                                          const double (&e3)[3][3], const double (&j3)[3][3][3], Adj& a)
+{
+    const double* er = e3[0];
+    const double* ed = e3[1];
+    const double* ea = e3[2];
+    const double (&jr)[3][3] = j3[0];
+    const double (&jd)[3][3] = j3[1];
+    const double (&ja)[3][3] = j3[2];
+    double srf, crf, sac, cac;
+    sincos2pi(frac_d(J.w_rf * (double)s), srf, crf);
+    sincos2pi(frac_d(fma(J.w_ac, (double)s, phi)), sac, cac);
+    const double a_rf = J.v_rf * srf, a_ac = J.v_ac * sac;
+    const double mx = a.lvx + B.dt * a.lx, my = a.lvy + B.dt * a.ly, mz = a.lvz + B.dt * a.lz;
+    const double Gx = J.kq * mx, Gy = J.kq * my, Gz = J.kq * mz;
+    // lambda_x += (dE/dx)^T G
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const double t = Gx * ((a_rf * jr[0][e] + J.s_dc * jd[0][e]) + a_ac * ja[0][e]) +
+                         Gy * ((a_rf * jr[1][e] + J.s_dc * jd[1][e]) + a_ac * ja[1][e]) +
+                         Gz * ((a_rf * jr[2][e] + J.s_dc * jd[2][e]) + a_ac * ja[2][e]);
+        if (e == 0) a.lx += t; else if (e == 1) a.ly += t; else a.lz += t;
+    }
+    a.lvx = mx; a.lvy = my; a.lvz = mz;
+    const double gr = Gx * er[0] + Gy * er[1] + Gz * er[2];
+    const double ga = Gx * ea[0] + Gy * ea[1] + Gz * ea[2];
+    a.gvrf += srf * gr;
+    a.gfrf += J.v_rf * crf * two_pi_dt * (double)s * gr;
+    a.gvac += sac * ga;
+    a.gfac += J.v_ac * cac * two_pi_dt * (double)s * ga;
+    a.gdc += Gx * ed[0] + Gy * ed[1] + Gz * ed[2];
+}
+
 // Backward: per ion, segments last to first; per-ion partial gradients and the
-// gradient with respect to the initial state.
+// gradient with respect to the initial state. In the walk down a segment the
+// recomputed positions are known ahead: step s - 1's position is loaded one
+// step early, and its cell record (for lanes whose cell changes) is loaded as
+// soon as step s has taken its fields from the cached record, so the loads are
+// in flight during the rest of step s.
 template <class G_>
 __global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __restrict__ jobs, BatchP B, G_ G,
                                                      AdjObj O, AdjBuf A, uint32_t n, double scale)
@@ -252,10 +325,11 @@ __global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __
     const double phi = S.phi[i];
     double g[3];
     (void)objective(O, A.fin[0][i], A.fin[1][i], A.fin[2][i], g);
-    double lx = scale * g[0], ly = scale * g[1], lz = scale * g[2], lvx = 0.0, lvy = 0.0, lvz = 0.0;
-    double gvrf = 0.0, gfrf = 0.0, gvac = 0.0, gfac = 0.0, gdc = 0.0;
+    Adj a{scale * g[0], scale * g[1], scale * g[2], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const double two_pi_dt = 6.283185307179586 * B.dt;
     const uint32_t nseg = (O.nsteps + O.ckpt - 1) / O.ckpt;
+    Cache<G_> K;  // record cache, kept across segments (records are keyed by the cell index)
+    asm volatile("" ::"v"(J.kq), "v"(J.v_rf), "v"(J.w_rf), "v"(J.v_ac), "v"(J.w_ac), "v"(J.s_dc), "v"(phi));  // as in k_adj_forward
     for (int seg = (int)nseg - 1; seg >= 0; --seg) {
         const uint32_t s0 = (uint32_t)seg * O.ckpt;
         const uint32_t s1 = min(O.nsteps, s0 + O.ckpt);
@@ -265,45 +339,29 @@ __global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __
         for (uint32_t s = s0; s < s1; ++s) {
             const size_t o = (size_t)(s - s0) * n + i;
             A.seg[0][o] = x; A.seg[1][o] = y; A.seg[2][o] = z;
-            step_fwd(B, J, G, phi, s, x, y, z, vx, vy, vz);
+            step_fwd(B, J, G, K, phi, s, x, y, z, vx, vy, vz);
         }
+        // walk it backwards (the look-ahead index is clamped to s0: a redundant read in bounds)
+        const auto pos = [&](int s, double& px, double& py, double& pz) {
+            const size_t o = (size_t)((uint32_t)max(s, (int)s0) - s0) * n + i;
+            px = A.seg[0][o]; py = A.seg[1][o]; pz = A.seg[2][o];
+        };
+        double px, py, pz;
+        pos((int)s1 - 1, px, py, pz);
+        Cell3 c = cell_of(B, px, py, pz);
+        refresh(G, c, K);
+        pos((int)s1 - 2, px, py, pz);
         for (int s = (int)s1 - 1; s >= (int)s0; --s) {
-            const size_t o = (size_t)((uint32_t)s - s0) * n + i;
-            const Cell3 c = cell_of(B, A.seg[0][o], A.seg[1][o], A.seg[2][o]);
             double e3[3][3], j3[3][3][3];
-            sample3_jac(G, c, e3, j3);
-            const double* er = e3[0];
-            const double* ed = e3[1];
-            const double* ea = e3[2];
-            const double (&jr)[3][3] = j3[0];
-            const double (&jd)[3][3] = j3[1];
-            const double (&ja)[3][3] = j3[2];
-            double srf, crf, sac, cac;
-            sincos2pi(frac_d(J.w_rf * (double)s), srf, crf);
-            sincos2pi(frac_d(fma(J.w_ac, (double)s, phi)), sac, cac);
-            const double a_rf = J.v_rf * srf, a_ac = J.v_ac * sac;
-            const double mx = lvx + B.dt * lx, my = lvy + B.dt * ly, mz = lvz + B.dt * lz;
-            const double Gx = J.kq * mx, Gy = J.kq * my, Gz = J.kq * mz;
-            // lambda_x += (dE/dx)^T G
-#pragma unroll
-            for (int e = 0; e < 3; ++e) {
-                const double t = Gx * ((a_rf * jr[0][e] + J.s_dc * jd[0][e]) + a_ac * ja[0][e]) +
-                                 Gy * ((a_rf * jr[1][e] + J.s_dc * jd[1][e]) + a_ac * ja[1][e]) +
-                                 Gz * ((a_rf * jr[2][e] + J.s_dc * jd[2][e]) + a_ac * ja[2][e]);
-                if (e == 0) lx += t; else if (e == 1) ly += t; else lz += t;
-            }
-            lvx = mx; lvy = my; lvz = mz;
-            const double gr = Gx * er[0] + Gy * er[1] + Gz * er[2];
-            const double ga = Gx * ea[0] + Gy * ea[1] + Gz * ea[2];
-            gvrf += srf * gr;
-            gfrf += J.v_rf * crf * two_pi_dt * (double)s * gr;
-            gvac += sac * ga;
-            gfac += J.v_ac * cac * two_pi_dt * (double)s * ga;
-            gdc += Gx * ed[0] + Gy * ed[1] + Gz * ed[2];
+            sample3_jac(G, c, K, e3, j3);
+            c = cell_of(B, px, py, pz);  // step s - 1
+            refresh(G, c, K);
+            pos(s - 2, px, py, pz);
+            walk_step(B, J, phi, two_pi_dt, (uint32_t)s, e3, j3, a);
         }
     }
-    A.gp[0][i] = gvrf; A.gp[1][i] = gfrf; A.gp[2][i] = gvac; A.gp[3][i] = gfac; A.gp[4][i] = gdc;
-    A.gs[0][i] = lx; A.gs[1][i] = ly; A.gs[2][i] = lz; A.gs[3][i] = lvx; A.gs[4][i] = lvy; A.gs[5][i] = lvz;
+    A.gp[0][i] = a.gvrf; A.gp[1][i] = a.gfrf; A.gp[2][i] = a.gvac; A.gp[3][i] = a.gfac; A.gp[4][i] = a.gdc;
+    A.gs[0][i] = a.lx; A.gs[1][i] = a.ly; A.gs[2][i] = a.lz; A.gs[3][i] = a.lvx; A.gs[4][i] = a.lvy; A.gs[5][i] = a.lvz;
 }
 
 // Deterministic sum of `cols` columns of length n (fixed order: 1024-element
