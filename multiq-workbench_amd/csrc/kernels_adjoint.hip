@@ -274,23 +274,27 @@ struct Adj {
     double gvrf, gfrf, gvac, gfac, gdc;
 };
 
-// One backward step s, from the fields e3 and their derivatives j3 at the
-// recomputed position x_s.
-__device__ __forceinline__ void walk_step(const BatchP& B, const JobP& J, double phi, double two_pi_dt, uint32_t s,
-                                          const double (&e3)[3][3], const double (&j3)[3][3][3], Adj& a)
+// One backward step s at the cell c of the recomputed position x_s, whose
+// record the caller has just refreshed: the drive phases and the adjoint
+// momentum come first, while the record loads are in flight.
+template <class G_>
+__device__ __forceinline__ void walk_step(const BatchP& B, const JobP& J, const G_& G, const Cache<G_>& K,
+                                          const Cell3& c, double phi, double two_pi_dt, uint32_t s, Adj& a)
 {
-    const double* er = e3[0];
-    const double* ed = e3[1];
-    const double* ea = e3[2];
-    const double (&jr)[3][3] = j3[0];
-    const double (&jd)[3][3] = j3[1];
-    const double (&ja)[3][3] = j3[2];
     double srf, crf, sac, cac;
     sincos2pi(frac_d(J.w_rf * (double)s), srf, crf);
     sincos2pi(frac_d(fma(J.w_ac, (double)s, phi)), sac, cac);
     const double a_rf = J.v_rf * srf, a_ac = J.v_ac * sac;
     const double mx = a.lvx + B.dt * a.lx, my = a.lvy + B.dt * a.ly, mz = a.lvz + B.dt * a.lz;
     const double Gx = J.kq * mx, Gy = J.kq * my, Gz = J.kq * mz;
+    double e3[3][3], j3[3][3][3];
+    sample3_jac(G, c, K, e3, j3);
+    const double* er = e3[0];
+    const double* ed = e3[1];
+    const double* ea = e3[2];
+    const double (&jr)[3][3] = j3[0];
+    const double (&jd)[3][3] = j3[1];
+    const double (&ja)[3][3] = j3[2];
     // lambda_x += (dE/dx)^T G
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
@@ -311,10 +315,7 @@ __device__ __forceinline__ void walk_step(const BatchP& B, const JobP& J, double
 
 // Backward: per ion, segments last to first; per-ion partial gradients and the
 // gradient with respect to the initial state. In the walk down a segment the
-// recomputed positions are known ahead: step s - 1's position is loaded one
-// step early, and its cell record (for lanes whose cell changes) is loaded as
-// soon as step s has taken its fields from the cached record, so the loads are
-// in flight during the rest of step s.
+// recomputed position of step s - 1 is loaded during step s.
 template <class G_>
 __global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __restrict__ jobs, BatchP B, G_ G,
                                                      AdjObj O, AdjBuf A, uint32_t n, double scale)
@@ -348,16 +349,11 @@ __global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __
         };
         double px, py, pz;
         pos((int)s1 - 1, px, py, pz);
-        Cell3 c = cell_of(B, px, py, pz);
-        refresh(G, c, K);
-        pos((int)s1 - 2, px, py, pz);
         for (int s = (int)s1 - 1; s >= (int)s0; --s) {
-            double e3[3][3], j3[3][3][3];
-            sample3_jac(G, c, K, e3, j3);
-            c = cell_of(B, px, py, pz);  // step s - 1
+            const Cell3 c = cell_of(B, px, py, pz);
             refresh(G, c, K);
-            pos(s - 2, px, py, pz);
-            walk_step(B, J, phi, two_pi_dt, (uint32_t)s, e3, j3, a);
+            pos(s - 1, px, py, pz);
+            walk_step(B, J, G, K, c, phi, two_pi_dt, (uint32_t)s, a);
         }
     }
     A.gp[0][i] = a.gvrf; A.gp[1][i] = a.gfrf; A.gp[2][i] = a.gvac; A.gp[3][i] = a.gfac; A.gp[4][i] = a.gdc;

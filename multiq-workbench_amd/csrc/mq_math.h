@@ -119,27 +119,29 @@ MQ_HD double cos_q(double r2)
     return pc;
 }
 
-// sin(2 pi p) and cos(2 pi p).
+// sin(2 pi p) and cos(2 pi p). Quadrant k: (s, c) = (sn, cs), (cs, -sn),
+// (-sn, -cs), (-cs, sn). Both polynomials are evaluated and the quadrant is
+// applied by selects and sign flips, so lanes of a wavefront in different
+// quadrants take no divergent branches.
 MQ_HD void sincos2pi(double p, double& s, double& c)
 {
     double r; int k;
     quarter(p, r, k);
-    double r2 = r * r;
-    double sn = sin_q(r, r2), cs = cos_q(r2);
-    switch (k) {
-    case 0: s = sn; c = cs; break;
-    case 1: s = cs; c = -sn; break;
-    case 2: s = -sn; c = -cs; break;
-    default: s = -cs; c = sn; break;
-    }
+    const double r2 = r * r;
+    const double sn = sin_q(r, r2), cs = cos_q(r2);
+    const bool odd = (k & 1) != 0;
+    const double s0 = odd ? cs : sn, c0 = odd ? sn : cs;
+    s = (k & 2) ? -s0 : s0;
+    c = ((k + 1) & 2) ? -c0 : c0;
 }
 
 MQ_HD double sin2pi(double p)
 {
     double r; int k;
     quarter(p, r, k);
-    double r2 = r * r;
-    double v = (k & 1) ? cos_q(r2) : sin_q(r, r2);
+    const double r2 = r * r;
+    const double sn = sin_q(r, r2), cs = cos_q(r2);
+    const double v = (k & 1) ? cs : sn;
     return (k & 2) ? -v : v;
 }
 
