@@ -63,6 +63,36 @@ def test_float_tables_match_dual_numbers(oracle_mod):
     assert np.allclose(gstate, gsr, rtol=1e-8, atol=1e-10 * np.abs(gsr).max())
 
 
+@pytest.fixture(scope="module")
+def c1_float():
+    """The C1 grid rounded to floats: the differentiable form then runs on per-cell float records."""
+    g = synth.c1_grid()
+    g["rf"] = g["rf"].astype(np.float32).astype(np.float64)
+    g["ac"] = g["ac"].astype(np.float32).astype(np.float64)
+    g["dc"] = [d.astype(np.float32).astype(np.float64) if d is not None else None for d in g["dc"]]
+    e = mqhip.Engine(g)
+    yield g, e
+    e.close()
+
+
+@pytest.mark.parametrize("n,K", [(300, 0), (300, 7), (64, 1), (1, 0), (777, 700)])
+def test_float_tables_ragged_blocks_match_dual_numbers(c1_float, oracle_mod, n, K):
+    """The float-record kernels with ion counts that leave partial wavefronts and blocks,
+    and with checkpoint intervals that give many, odd, even and single segments (700 steps:
+    K = 7 gives 100 segments, K = 0 picks 27 -> 26, K = 1 gives 700, K = 700 gives one)."""
+    g, e = c1_float
+    job, ph, obj = setting(n=n, nsteps=700)
+    job.dc_scale *= 5.0
+    obj = mqhip.Objective(700, K, obj.z_exit, obj.scale, obj.r_leak, obj.w_leak)
+    print(f"[gpu] float-record adjoint, n={n}, ckpt={K}", flush=True)
+    J = e.forward(job, [0, 1, 0, 0], ph, obj)
+    grad, gstate = e.backward(1.0, state=True)
+    Jr, gr, gsr = oracle_mod.grad_reference(g, job, [0, 1, 0, 0], ph, obj, 8)
+    assert J == pytest.approx(Jr, rel=1e-12)
+    assert np.allclose(grad, gr, rtol=1e-9, atol=1e-12 * np.abs(gr).max())
+    assert np.allclose(gstate, gsr, rtol=1e-8, atol=1e-10 * np.abs(gsr).max())
+
+
 def test_adjoint_matches_central_differences(c1):
     """Central differences with h = 1e-6 relative and their Richardson extrapolation (h, h/2).
     Operating point with a 0.5 V gate: no ion reaches the barrier top, so the objective is
