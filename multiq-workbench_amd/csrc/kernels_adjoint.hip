@@ -203,8 +203,8 @@ __device__ __forceinline__ void step_fwd(const BatchP& B, const JobP& J, const G
 {
     const Cell3 c = cell_of(B, x, y, z);
     refresh(G, c, K);
-    const double a_rf = J.v_rf * sin2pi(frac_d(J.w_rf * (double)n));
-    const double a_ac = J.v_ac * sin2pi(frac_d(fma(J.w_ac, (double)n, phi)));
+    const double a_rf = J.v_rf * sin2pi_sel(frac_d(J.w_rf * (double)n));
+    const double a_ac = J.v_ac * sin2pi_sel(frac_d(fma(J.w_ac, (double)n, phi)));
     double e3[3][3];
     sample3(G, c, K, e3);
     const double* er = e3[0];
@@ -282,8 +282,8 @@ __device__ __forceinline__ void walk_step(const BatchP& B, const JobP& J, const 
                                           const Cell3& c, double phi, double two_pi_dt, uint32_t s, Adj& a)
 {
     double srf, crf, sac, cac;
-    sincos2pi(frac_d(J.w_rf * (double)s), srf, crf);
-    sincos2pi(frac_d(fma(J.w_ac, (double)s, phi)), sac, cac);
+    sincos2pi_sel(frac_d(J.w_rf * (double)s), srf, crf);
+    sincos2pi_sel(frac_d(fma(J.w_ac, (double)s, phi)), sac, cac);
     const double a_rf = J.v_rf * srf, a_ac = J.v_ac * sac;
     const double mx = a.lvx + B.dt * a.lx, my = a.lvy + B.dt * a.ly, mz = a.lvz + B.dt * a.lz;
     const double Gx = J.kq * mx, Gy = J.kq * my, Gz = J.kq * mz;

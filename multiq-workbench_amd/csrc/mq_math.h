@@ -119,11 +119,36 @@ MQ_HD double cos_q(double r2)
     return pc;
 }
 
-// sin(2 pi p) and cos(2 pi p). Quadrant k: (s, c) = (sn, cs), (cs, -sn),
-// (-sn, -cs), (-cs, sn). Both polynomials are evaluated and the quadrant is
-// applied by selects and sign flips, so lanes of a wavefront in different
-// quadrants take no divergent branches.
+// sin(2 pi p) and cos(2 pi p).
 MQ_HD void sincos2pi(double p, double& s, double& c)
+{
+    double r; int k;
+    quarter(p, r, k);
+    double r2 = r * r;
+    double sn = sin_q(r, r2), cs = cos_q(r2);
+    switch (k) {
+    case 0: s = sn; c = cs; break;
+    case 1: s = cs; c = -sn; break;
+    case 2: s = -sn; c = -cs; break;
+    default: s = -cs; c = sn; break;
+    }
+}
+
+MQ_HD double sin2pi(double p)
+{
+    double r; int k;
+    quarter(p, r, k);
+    double r2 = r * r;
+    double v = (k & 1) ? cos_q(r2) : sin_q(r, r2);
+    return (k & 2) ? -v : v;
+}
+
+// The same values (bit for bit) with both polynomials evaluated and the
+// quadrant k applied by selects and sign flips: (s, c) = (sn, cs), (cs, -sn),
+// (-sn, -cs), (-cs, sn). Lanes of a wavefront in different quadrants then take
+// no divergent branches; where the quadrant is mostly uniform across a
+// wavefront the forms above, which evaluate one polynomial, are cheaper.
+MQ_HD void sincos2pi_sel(double p, double& s, double& c)
 {
     double r; int k;
     quarter(p, r, k);
@@ -135,7 +160,7 @@ MQ_HD void sincos2pi(double p, double& s, double& c)
     c = ((k + 1) & 2) ? -c0 : c0;
 }
 
-MQ_HD double sin2pi(double p)
+MQ_HD double sin2pi_sel(double p)
 {
     double r; int k;
     quarter(p, r, k);
