@@ -233,7 +233,8 @@ typedef struct {
 /* Forward op: injects job->n ions (as mq_run_batch would) and returns the
  * objective in *J, keeping the checkpoints on the engine for mq_backward.
  * Parameters theta = (drive.v_rf, drive.f_rf, drive.v_ac, drive.f_ac,
- * dc_scale). The reference has no gradients (SURVEY: Optimize is a sweep). */
+ * dc_scale). No counterpart in the reference, which has no gradients: its
+ * Optimize search is a grid of sweeps (src-tauri/src/research.rs:133-163). */
 int mq_forward(mq_engine* eng, const mq_job* job, const double dc[4], const mq_physics* ph,
                const mq_objective* obj, double* J);
 
