@@ -430,9 +430,13 @@ struct mq_engine {
     // device memory, so launches past the end of the work are empty) and the host
     // checks the live count once per block; with `graph` on, the block is
     // captured once as a hipGraph and replayed.
+    // bins_known: bin_of holds the bins of list[cur] (the last launch wrote them);
+    // pass_bins: each launch writes the bins of the ions it carries, so that the
+    // next re-sort skips the gather of their positions (off in graph mode, where
+    // one captured block must serve every replay).
     void enqueue_block(const IonState& S, const BatchP& B, uint32_t n, const SortGeom& G, int& cur, uint32_t& chunk,
                        uint32_t cap, uint32_t period, uint32_t grid, uint32_t block, int iters, bool timed,
-                       std::vector<hipEvent_t>* evs, bool resort)
+                       std::vector<hipEvent_t>* evs, bool resort, bool& bins_known, bool pass_bins)
     {
         uint32_t* c0 = ctr.as<uint32_t>();
         uint32_t* c1 = c0 + CTR_BLOCK;
@@ -442,10 +446,10 @@ struct mq_engine {
             const uint32_t* lin = list[cur].as<uint32_t>();
             if (resort) {
                 launch_sort(S, B, lin, cin, list[2].as<uint32_t>(), bin_of.as<uint32_t>(), rank_of.as<uint32_t>(),
-                            hist.as<uint32_t>(), bsum.as<uint32_t>(), G, n, stream);
+                            hist.as<uint32_t>(), bsum.as<uint32_t>(), G, n, bins_known, stream);
                 lin = list[2].as<uint32_t>();
             }
-            WorkerArgs a;
+            WorkerArgs a{};
             a.S = S;
             a.jobs = jobs.as<JobP>();
             a.B = B;
@@ -458,6 +462,9 @@ struct mq_engine {
             a.period = period;
             a.nqueues = resort ? MAX_QUEUES : 1u;
             a.coll_batch = (uint32_t)std::max(1, tune.coll_batch);
+            a.bin_out = resort && pass_bins ? bin_of.as<uint32_t>() : nullptr;
+            a.sg = G;
+            bins_known = a.bin_out != nullptr;
             launch_prepare(a, stream);
             if (timed) HIPCHK(hipEventRecord((*evs)[2 * it], stream));
             launch_batched(a, grid, block, stream);
@@ -533,9 +540,11 @@ struct mq_engine {
             hipGraphExec_t ge = nullptr;
             int cur_cap = cur;
             uint32_t chunk_cap = chunk;
+            bool known = false;
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             try {
-                enqueue_block(S, B, n, G, cur_cap, chunk_cap, cap, period, grid, block, iters, false, nullptr, resort);
+                enqueue_block(S, B, n, G, cur_cap, chunk_cap, cap, period, grid, block, iters, false, nullptr, resort,
+                              known, false);
             } catch (...) {
                 (void)hipStreamEndCapture(stream, &g);
                 if (g) (void)hipGraphDestroy(g);
@@ -581,8 +590,10 @@ struct mq_engine {
         // re-sort interval.
         const uint64_t lanes = (uint64_t)grid * block;
         bool tail = false;
+        bool bins_known = false;  // list[0] is the injected list: no bins yet
         for (;;) {
-            enqueue_block(S, B, n, G, cur, chunk, cap, period, grid, block, iters, true, &evs, resort && !tail);
+            enqueue_block(S, B, n, G, cur, chunk, cap, period, grid, block, iters, true, &evs, resort && !tail,
+                          bins_known, true);
             HIPCHK(hipGetLastError());
             stats.launches += (uint32_t)iters;
             uint32_t alive = 0;

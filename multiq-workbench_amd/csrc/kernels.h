@@ -27,6 +27,11 @@ constexpr uint32_t CTR_Q = 32;
 constexpr uint32_t CTR_COUNT = MAX_QUEUES * CTR_Q;
 constexpr uint32_t CTR_BLOCK = CTR_COUNT + 32;
 
+// Coarse cell bins of the re-sort: bin = ((k >> sz) nby + (j >> sy)) nbx + (i >> sx).
+struct SortGeom {
+    uint32_t sx, sy, sz, nbx, nby, nbz, nbins;
+};
+
 struct WorkerArgs {
     IonState S;
     const JobP* jobs;
@@ -40,15 +45,15 @@ struct WorkerArgs {
     uint32_t period;    // steps between refills (multiple of 4)
     uint32_t nqueues;   // 1, or MAX_QUEUES contiguous segments (one per XCD)
     uint32_t coll_batch;  // pending collisions per wavefront that trigger the collision code
+    uint32_t* bin_out;    // optional: re-sort bin of each carried ion, at its list_out position
+    SortGeom sg;          // bins of bin_out
 };
 
-// Coarse cell bins of the re-sort: bin = ((k >> sz) nby + (j >> sy)) nbx + (i >> sx).
-struct SortGeom {
-    uint32_t sx, sy, sz, nbx, nby, nbz, nbins;
-};
+// bins_known: bin_of already holds the bin of each list entry (written by the
+// worker launch that produced the list), so only the ranks are computed.
 void launch_sort(const IonState& S, const BatchP& B, const uint32_t* list, const uint32_t* ctr, uint32_t* out,
                  uint32_t* bin_of, uint32_t* rank_of, uint32_t* hist, uint32_t* bsum, const SortGeom& G,
-                 uint32_t nmax, hipStream_t st);
+                 uint32_t nmax, bool bins_known, hipStream_t st);
 // Pieces of launch_sort for callers with their own binning kernel: clear the
 // histogram; then, after bin_of / rank_of / hist are filled, turn counts into
 // offsets and place every entry.

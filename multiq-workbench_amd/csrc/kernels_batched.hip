@@ -100,6 +100,15 @@ __global__ void k_prepare(uint32_t* ctr_in, uint32_t* ctr_out)
     if (t == 0) ctr_out[CTR_COUNT] = 0;
 }
 
+// Re-sort bin of a position (coarse cell bins, z-major, x fastest).
+__device__ __forceinline__ uint32_t sort_bin(const BatchP& B, const SortGeom& G, double x, double y, double z)
+{
+    const int ci = min((int)fmin(fmax(x * B.inv_h, 0.0), (double)(B.nx - 1)), B.nx - 2);
+    const int cj = min((int)fmin(fmax(y * B.inv_h, 0.0), (double)(B.ny - 1)), B.ny - 2);
+    const int ck = min((int)fmin(fmax(z * B.inv_h, 0.0), (double)(B.nz - 1)), B.nz - 2);
+    return (((uint32_t)ck >> G.sz) * G.nby + ((uint32_t)cj >> G.sy)) * G.nbx + ((uint32_t)ci >> G.sx);
+}
+
 // ------------------------------------------------------------- the worker
 struct Lane {
     double x, y, z, phi, w_rf, w_ac;
@@ -373,7 +382,11 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(&A.ctr_out[CTR_COUNT], (uint32_t)__popcll(cm));
             base = __shfl(base, (int)leader);
-            if (carry) A.list_out[base + (uint32_t)__popcll(cm & ((1ull << lane) - 1ull))] = L.id;
+            if (carry) {
+                const uint32_t pos = base + (uint32_t)__popcll(cm & ((1ull << lane) - 1ull));
+                A.list_out[pos] = L.id;
+                if (A.bin_out) A.bin_out[pos] = sort_bin(A.B, A.sg, L.x, L.y, L.z);
+            }
         }
         if (have && stop) {
             const uint32_t id = L.id;
@@ -432,12 +445,19 @@ __global__ void k_bin(IonState S, BatchP B, const uint32_t* __restrict__ list, c
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ctr[CTR_COUNT]) return;
     const uint32_t id = list[i];
-    const int ci = min((int)fmin(fmax(S.x[id] * B.inv_h, 0.0), (double)(B.nx - 1)), B.nx - 2);
-    const int cj = min((int)fmin(fmax(S.y[id] * B.inv_h, 0.0), (double)(B.ny - 1)), B.ny - 2);
-    const int ck = min((int)fmin(fmax(S.z[id] * B.inv_h, 0.0), (double)(B.nz - 1)), B.nz - 2);
-    const uint32_t bin = (((uint32_t)ck >> G.sz) * G.nby + ((uint32_t)cj >> G.sy)) * G.nbx + ((uint32_t)ci >> G.sx);
+    const uint32_t bin = sort_bin(B, G, S.x[id], S.y[id], S.z[id]);
     bin_of[i] = bin;
     bin_rank(hist, bin, i, rank_of);
+}
+
+// k_bin for a list whose bins the worker has already written (bin_of): the
+// positions are not gathered again.
+__global__ void k_rank(const uint32_t* __restrict__ ctr, const uint32_t* __restrict__ bin_of,
+                       uint32_t* __restrict__ rank_of, uint32_t* __restrict__ hist)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ctr[CTR_COUNT]) return;
+    bin_rank(hist, bin_of[i], i, rank_of);
 }
 
 // Exclusive scan of hist in place: blocks of 1024 entries (256 threads x 4).
@@ -521,11 +541,12 @@ void launch_sort_place(const uint32_t* list, const uint32_t* ctr, uint32_t* out,
 
 void launch_sort(const IonState& S, const BatchP& B, const uint32_t* list, const uint32_t* ctr, uint32_t* out,
                  uint32_t* bin_of, uint32_t* rank_of, uint32_t* hist, uint32_t* bsum, const SortGeom& G,
-                 uint32_t nmax, hipStream_t st)
+                 uint32_t nmax, bool bins_known, hipStream_t st)
 {
     const uint32_t nb = G.nbins;
     launch_fill_hist(hist, nb, st);
-    if (nmax) k_bin<<<nblk(nmax, 256), 256, 0, st>>>(S, B, list, ctr, bin_of, rank_of, hist, G);
+    if (nmax && bins_known) k_rank<<<nblk(nmax, 256), 256, 0, st>>>(ctr, bin_of, rank_of, hist);
+    else if (nmax) k_bin<<<nblk(nmax, 256), 256, 0, st>>>(S, B, list, ctr, bin_of, rank_of, hist, G);
     launch_sort_place(list, ctr, out, bin_of, rank_of, hist, bsum, nb, nmax, st);
 }
 

@@ -276,18 +276,53 @@ __global__ __launch_bounds__(256) void k_plain(IonState S, const JobP* __restric
 
 // ------------------------------------------------------------ summarize
 // summarize (channel.rs:496-506): per-job counts, sum of steps and collisions.
+// Per-job counters. The ions of a job are contiguous, so the lanes of a
+// wavefront nearly always share one job: each group of lanes with the same job
+// (peeled off by ballot) is reduced in the wavefront and added by one lane,
+// instead of four atomics per ion on the same few counters. The sums are
+// integers, so the order of the additions does not matter.
 __global__ void k_summarize(const uint8_t* __restrict__ status, const uint32_t* __restrict__ steps,
                             const uint32_t* __restrict__ ncoll, const uint32_t* __restrict__ job,
                             unsigned long long* __restrict__ acc, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    unsigned long long* a = acc + (size_t)job[i] * ACC;
-    uint32_t st = status[i];
-    atomicAdd(&a[0], 1ull);
-    atomicAdd(&a[st >= 1 && st <= 5 ? st : 5], 1ull);
-    atomicAdd(&a[6], (unsigned long long)steps[i]);
-    atomicAdd(&a[7], (unsigned long long)ncoll[i]);
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool act = i < n;
+    uint32_t j = 0, st = 0;
+    unsigned long long sv = 0, cv = 0;
+    if (act) {
+        j = job[i];
+        st = status[i];
+        st = st >= 1 && st <= 5 ? st : 5;
+        sv = steps[i];
+        cv = ncoll[i];
+    }
+    uint64_t todo = __ballot(act);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t lj = (uint32_t)__shfl((int)j, leader);
+        const uint64_t same = __ballot(act && j == lj) & todo;
+        const bool in = (same >> lane) & 1ull;
+        unsigned long long s6 = in ? sv : 0ull, s7 = in ? cv : 0ull;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            s6 += __shfl_xor(s6, o);
+            s7 += __shfl_xor(s7, o);
+        }
+        uint32_t cls[6];
+#pragma unroll
+        for (uint32_t c = 1; c <= 5; ++c) cls[c] = (uint32_t)__popcll(__ballot(in && st == c));
+        if (lane == (uint32_t)leader) {
+            unsigned long long* a = acc + (size_t)lj * ACC;
+            atomicAdd(&a[0], (unsigned long long)__popcll(same));
+#pragma unroll
+            for (uint32_t c = 1; c <= 5; ++c)
+                if (cls[c]) atomicAdd(&a[c], (unsigned long long)cls[c]);
+            atomicAdd(&a[6], s6);
+            atomicAdd(&a[7], s7);
+        }
+        todo &= ~same;
+    }
 }
 
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, uint32_t n)
