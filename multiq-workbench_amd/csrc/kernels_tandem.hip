@@ -470,19 +470,41 @@ __global__ void k_tandem_init(IonState S, uint32_t n)
     S.status[i] = 0;
 }
 
-__global__ void k_tandem_totals(IonState S, uint32_t n, unsigned long long* __restrict__ tot)
+// End totals: each 256-thread block counts its ions per class (ballots) and
+// sums their steps (wavefront sums), then adds the block's eight figures once,
+// instead of three atomics per ion on the same eight counters.
+__global__ __launch_bounds__(256) void k_tandem_totals(IonState S, uint32_t n, unsigned long long* __restrict__ tot)
 {
+    __shared__ unsigned long long part[4][TT_N];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t st = S.status[i], reg = S.region[i];
-    atomicAdd(&tot[0], 1ull);
-    int k;
-    if (st == 1) k = 4;
-    else if (st == 2) k = 5;
-    else if (st == 3) k = 6;
-    else k = reg == REG_SPLIT ? 1 : reg == REG_COMBINE ? 3 : 2;
-    atomicAdd(&tot[k], 1ull);
-    atomicAdd(&tot[7], (unsigned long long)S.steps[i]);
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const bool act = i < n;
+    int k = 0;
+    unsigned long long sv = 0;
+    if (act) {
+        const uint32_t st = S.status[i], reg = S.region[i];
+        if (st == 1) k = 4;
+        else if (st == 2) k = 5;
+        else if (st == 3) k = 6;
+        else k = reg == REG_SPLIT ? 1 : reg == REG_COMBINE ? 3 : 2;
+        sv = S.steps[i];
+    }
+    unsigned long long c[TT_N];
+    c[0] = (unsigned long long)__popcll(__ballot(act));
+#pragma unroll
+    for (int q = 1; q <= 6; ++q) c[q] = (unsigned long long)__popcll(__ballot(act && k == q));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sv += __shfl_xor(sv, o);
+    c[7] = sv;
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < TT_N; ++q) part[w][q] = c[q];
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)TT_N) {
+        const uint32_t q = threadIdx.x;
+        const unsigned long long v = part[0][q] + part[1][q] + part[2][q] + part[3][q];
+        if (v) atomicAdd(&tot[q], v);
+    }
 }
 
 // ------------------------------------------------------------- launchers
