@@ -287,9 +287,14 @@ int mq_adaptive_fine_points(const double* f_khz, const double* t, size_t nf, uin
                             double f_step, double* out, size_t max_out, size_t* n_out);
 
 /* The cache key of one job (scan.rs:9-11) into buf; returns 0, < 0 on error,
- * or the required length if len is too small. */
-int mq_scan_key(const char* geom_key, const mq_job* job, const double dc[4], const mq_physics* ph, char* buf,
-                size_t len);
+ * or the required length if len is too small. path is the execution path the
+ * job runs on (the reference's key() reads it from the Engine). For Philox
+ * streams on the batched path the key is the reference's; ph->rng other than
+ * MQ_RNG_PHILOX appends ", rng: <n>" and a path other than MQ_PATH_BATCHED
+ * appends ", path: <n>" inside the Physics braces, so a cached result is only
+ * returned to a request for the same streams and the same path. */
+int mq_scan_key(const char* geom_key, const mq_job* job, const double dc[4], const mq_physics* ph, int32_t path,
+                char* buf, size_t len);
 
 /* An f64 in Rust Debug ({:?}, display = 0) or Display ({}, display = 1)
  * notation, as used in the cache file; same return convention as mq_scan_key. */

@@ -274,8 +274,13 @@ struct mq_engine {
         HIPCHK(hipMemcpyAsync(acc_h.data(), acc.p, njobs * ACC * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                               stream));
         HIPCHK(hipStreamSynchronize(stream));
-        uint64_t steps = 0;
-        for (size_t k = 0; k < njobs; ++k) steps += acc_h[k * ACC + 6];
+        uint64_t steps = 0, unfinished = 0;
+        for (size_t k = 0; k < njobs; ++k) {
+            steps += acc_h[k * ACC + 6];
+            unfinished += acc_h[k * ACC + ACC_UNFINISHED];
+        }
+        if (unfinished)
+            throw Err(-5, "internal error: " + std::to_string(unfinished) + " ions never finished (status 0)");
         stats.ion_steps = steps;
         stats.kernel_ms = kms;
         stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
@@ -464,7 +469,8 @@ struct mq_engine {
             a.ctr_out = cout;
             a.chunk = chunk;
             a.period = period;
-            a.nqueues = resort ? MAX_QUEUES : 1u;
+            // one queue per XCD, each the home of at least one workgroup
+            a.nqueues = resort ? std::min<uint32_t>(MAX_QUEUES, grid) : 1u;
             a.coll_batch = (uint32_t)std::max(1, tune.coll_batch);
             a.bin_out = resort && pass_bins ? bin_of.as<uint32_t>() : nullptr;
             a.sg = G;
@@ -836,7 +842,7 @@ int mq_summarize(const uint8_t* status, const uint32_t* ncoll, const uint32_t* s
             unsigned long long* p = &a[(size_t)job[i] * ACC];
             p[0] += 1;
             uint8_t s = status[i];
-            p[(s >= 1 && s <= 5) ? s : 5] += 1;
+            p[(s >= 1 && s <= 5) ? s : 5] += 1;  // summarize counts any other status as a timeout
             p[6] += steps[i];
             p[7] += ncoll[i];
         }

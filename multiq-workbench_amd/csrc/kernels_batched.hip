@@ -177,8 +177,9 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
         // ---- refill idle lanes from the live list (one atomic per wavefront). With
         // several queues the list is cut into contiguous (spatially sorted) segments
         // and the wavefronts of one XCD start on their own segment, then steal.
-        const uint64_t idle = __ballot(!have);
-        if (idle && !drained) {
+        while (!drained) {
+            const uint64_t idle = __ballot(!have);
+            if (!idle) break;
             const uint32_t nidle = (uint32_t)__popcll(idle);
             const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1;
             const uint32_t s0 = (uint32_t)((uint64_t)count * q / nq), s1 = (uint32_t)((uint64_t)count * (q + 1) / nq);
@@ -187,9 +188,15 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
             base = __shfl(base, (int)leader);
             const uint32_t avail = s1 - s0;
             if (base >= avail) {
+                // queue q is used up for this launch: move on. A wavefront with no
+                // ion left claims from the next queue at once; one that still
+                // steps ions tries it at its next refill.
                 if (++tried >= nq) drained = true;
                 else q = (q + 1 == nq) ? 0 : q + 1;
-            } else if (!have) {
+                if (__any(have)) break;
+                continue;
+            }
+            if (!have) {
                 const uint32_t slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (slot < avail) {
                     const uint32_t id = A.list_in[s0 + slot];
@@ -218,6 +225,7 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                     have = true;
                 }
             }
+            break;
         }
         if (!__any(have)) break;
         // ---- advance every held ion by up to `period` steps. A lane whose collision

@@ -293,7 +293,7 @@ __global__ void k_summarize(const uint8_t* __restrict__ status, const uint32_t* 
     if (act) {
         j = job[i];
         st = status[i];
-        st = st >= 1 && st <= 5 ? st : 5;
+        st = st >= 1 && st <= 5 ? st : (uint32_t)ACC_UNFINISHED;
         sv = steps[i];
         cv = ncoll[i];
     }
@@ -312,12 +312,14 @@ __global__ void k_summarize(const uint8_t* __restrict__ status, const uint32_t* 
         uint32_t cls[6];
 #pragma unroll
         for (uint32_t c = 1; c <= 5; ++c) cls[c] = (uint32_t)__popcll(__ballot(in && st == c));
+        const uint32_t unfinished = (uint32_t)__popcll(__ballot(in && st == (uint32_t)ACC_UNFINISHED));
         if (lane == (uint32_t)leader) {
             unsigned long long* a = acc + (size_t)lj * ACC;
             atomicAdd(&a[0], (unsigned long long)__popcll(same));
 #pragma unroll
             for (uint32_t c = 1; c <= 5; ++c)
                 if (cls[c]) atomicAdd(&a[c], (unsigned long long)cls[c]);
+            if (unfinished) atomicAdd(&a[ACC_UNFINISHED], (unsigned long long)unfinished);
             atomicAdd(&a[6], s6);
             atomicAdd(&a[7], s7);
         }

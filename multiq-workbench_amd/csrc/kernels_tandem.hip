@@ -333,8 +333,9 @@ __global__ __launch_bounds__(256) void k_tandem_batched(TBatchArgs A)
         // refill idle lanes: with several queues the (re-sorted) list is cut into
         // contiguous segments and the wavefronts of one XCD start on their own
         // segment, then steal, as in k_batched
-        const uint64_t idle = __ballot(!have);
-        if (idle && !drained) {
+        while (!drained) {
+            const uint64_t idle = __ballot(!have);
+            if (!idle) break;
             const uint32_t nidle = (uint32_t)__popcll(idle);
             const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1;
             const uint32_t s0 = (uint32_t)((uint64_t)count * q / nq), s1 = (uint32_t)((uint64_t)count * (q + 1) / nq);
@@ -343,9 +344,15 @@ __global__ __launch_bounds__(256) void k_tandem_batched(TBatchArgs A)
             base = __shfl(base, (int)leader);
             const uint32_t avail = s1 - s0;
             if (base >= avail) {
+                // queue q is used up for this launch: move on. A wavefront with no
+                // ion left claims from the next queue at once; one that still
+                // steps ions tries it at its next refill.
                 if (++tried >= nq) drained = true;
                 else q = (q + 1 == nq) ? 0 : q + 1;
-            } else if (!have) {
+                if (__any(have)) break;
+                continue;
+            }
+            if (!have) {
                 const uint32_t slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (slot < avail) {
                     const uint32_t id = A.list_in[s0 + slot];
@@ -369,6 +376,7 @@ __global__ __launch_bounds__(256) void k_tandem_batched(TBatchArgs A)
                     have = true;
                 }
             }
+            break;
         }
         if (!__any(have)) break;
         bool stop = false;
@@ -486,7 +494,8 @@ __global__ __launch_bounds__(256) void k_tandem_totals(IonState S, uint32_t n, u
         if (st == 1) k = 4;
         else if (st == 2) k = 5;
         else if (st == 3) k = 6;
-        else k = reg == REG_SPLIT ? 1 : reg == REG_COMBINE ? 3 : 2;
+        else if (st == 5) k = reg == REG_SPLIT ? 1 : reg == REG_COMBINE ? 3 : 2;
+        else k = TT_N;  // never finished: reported by the host as an error
         sv = S.steps[i];
     }
     unsigned long long c[TT_N];
@@ -496,6 +505,8 @@ __global__ __launch_bounds__(256) void k_tandem_totals(IonState S, uint32_t n, u
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) sv += __shfl_xor(sv, o);
     c[7] = sv;
+    const unsigned long long unfinished = (unsigned long long)__popcll(__ballot(act && k == TT_N));
+    if (lane == 0 && unfinished) atomicAdd(&tot[TT_N], unfinished);
     if (lane == 0)
 #pragma unroll
         for (int q = 0; q < TT_N; ++q) part[w][q] = c[q];

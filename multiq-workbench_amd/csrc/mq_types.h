@@ -44,8 +44,10 @@ struct IonState {
 };
 
 // Per-job accumulators (u64): [0] n, [1..5] crashed, returned, transmitted,
-// left, timeout, [6] sum of steps, [7] sum of collisions.
-constexpr int ACC = 8;
+// left, timeout, [6] sum of steps, [7] sum of collisions, [8] ions whose status
+// is not a final one (never finished: the engine reports these as an error).
+constexpr int ACC = 9;
+constexpr int ACC_UNFINISHED = 8;
 
 // FP32 cell record of the batched path: trilinear polynomial coefficients
 // c0..c7 (1, tx, ty, tz, tx ty, ty tz, tx tz, tx ty tz) x 3 components for the

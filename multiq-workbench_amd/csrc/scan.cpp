@@ -97,7 +97,13 @@ std::string mq_rust_display_f64(double x)
 
 namespace {
 
-std::string key_of(const char* geom, const mq_job& j, const double dc[4], const mq_physics& ph)
+// The reference's key holds no random-stream or path field (its Engine has one
+// stream and its paths are statistically equivalent). This build's extension
+// fields are appended inside the Physics braces only when they differ from the
+// default, so default requests keep the reference's key byte for byte while a
+// request for the reference's own streams (rng) or for the FP64 plain path
+// never reads a line the default Philox / FP32 batched run wrote, or the reverse.
+std::string key_of(const char* geom, const mq_job& j, const double dc[4], const mq_physics& ph, int32_t path)
 {
     auto D = mq_rust_debug_f64;
     auto B = [](int32_t v) { return std::string(v ? "true" : "false"); };
@@ -116,6 +122,8 @@ std::string key_of(const char* geom, const mq_job& j, const double dc[4], const 
       << ", r_inject_mm: " << D(ph.r_inject_mm);
     if (ph.injection != MQ_INJ_REFERENCE) k << ", injection: " << ph.injection << ", inj_sigma_mm: " << D(ph.inj_sigma_mm);
     if (ph.max_steps) k << ", max_steps: " << ph.max_steps;
+    if (ph.rng != MQ_RNG_PHILOX) k << ", rng: " << ph.rng;
+    if (path != MQ_PATH_BATCHED) k << ", path: " << path;
     k << " }";
     return k.str();
 }
@@ -155,11 +163,11 @@ std::map<std::string, mq_outcome> load_cache(const std::string& path)
 
 extern "C" {
 
-int mq_scan_key(const char* geom_key, const mq_job* job, const double dc[4], const mq_physics* ph, char* buf,
-                size_t len)
+int mq_scan_key(const char* geom_key, const mq_job* job, const double dc[4], const mq_physics* ph, int32_t path,
+                char* buf, size_t len)
 {
     if (!geom_key || !job || !dc || !ph || !buf) { mq_set_last_error("NULL argument"); return -1; }
-    const std::string k = key_of(geom_key, *job, dc, *ph);
+    const std::string k = key_of(geom_key, *job, dc, *ph, path);
     if (k.size() + 1 > len) return (int)(k.size() + 1);
     std::memcpy(buf, k.c_str(), k.size() + 1);
     return 0;
@@ -184,7 +192,7 @@ int mq_cached_batch(mq_engine* eng, const char* geom_key, const mq_job* jobs, si
         std::vector<std::string> keys(njobs);
         std::vector<size_t> miss;
         for (size_t i = 0; i < njobs; ++i) {
-            keys[i] = key_of(geom_key, jobs[i], dc, *ph);
+            keys[i] = key_of(geom_key, jobs[i], dc, *ph, path);
             if (!cache.count(keys[i])) miss.push_back(i);
         }
         // duplicated keys inside one call are computed once

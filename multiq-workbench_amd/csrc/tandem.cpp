@@ -267,9 +267,9 @@ struct mq_tandem {
         HIPCHK(hipMemcpyAsync(jobs.p, jp.data(), njobs * sizeof(JobP), hipMemcpyHostToDevice, stream));
         const uint32_t nctr = 9 * TC_PORT;
         ctr.ensure(nctr * 8);
-        tot.ensure(TT_N * 8);
+        tot.ensure((TT_N + 1) * 8);
         launch_fill_u64(ctr.as<unsigned long long>(), 0ull, nctr, stream);
-        launch_fill_u64(tot.as<unsigned long long>(), 0ull, TT_N, stream);
+        launch_fill_u64(tot.as<unsigned long long>(), 0ull, TT_N + 1, stream);
         const bool cells = path == MQ_PATH_BATCHED;
         double idle[4];
         for (int b = 0; b < 4; ++b) idle[b] = d->dc_port[b] + d->bias_idle[b];
@@ -296,10 +296,12 @@ struct mq_tandem {
         launch_tandem_totals(S, n, tot.as<unsigned long long>(), stream);
         HIPCHK(hipGetLastError());
         ctr_h.resize(nctr);
-        tot_h.resize(TT_N);
+        tot_h.resize(TT_N + 1);
         HIPCHK(hipMemcpyAsync(ctr_h.data(), ctr.p, nctr * 8, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipMemcpyAsync(tot_h.data(), tot.p, TT_N * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(tot_h.data(), tot.p, (TT_N + 1) * 8, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
+        if (tot_h[TT_N])
+            throw Err(-5, "internal error: " + std::to_string(tot_h[TT_N]) + " ions never finished (status 0)");
         stats.ion_steps = tot_h[7];
         stats.kernel_ms = kms;
         stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
@@ -383,7 +385,7 @@ struct mq_tandem {
             launch_prepare(pa, stream);
             HIPCHK(hipEventRecord(ev0, stream));
             launch_tandem_batched(S, jobs.as<JobP>(), T, cg, pg, cells, ctr.as<unsigned long long>(), lin,
-                                  list[cur ^ 1].as<uint32_t>(), cin, cout, chunk, 8u, tail ? 1u : MAX_QUEUES, grid,
+                                  list[cur ^ 1].as<uint32_t>(), cin, cout, chunk, 8u, tail ? 1u : std::min<uint32_t>(MAX_QUEUES, grid), grid,
                                   block, stream);
             HIPCHK(hipEventRecord(ev1, stream));
             HIPCHK(hipGetLastError());

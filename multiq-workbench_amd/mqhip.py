@@ -254,8 +254,8 @@ SIGNATURES = {
     "mq_adaptive_fine_points": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_size_t, C.c_uint64,
                                           C.c_double, C.c_double, C.POINTER(C.c_double), C.c_size_t,
                                           C.POINTER(C.c_size_t)]),
-    "mq_scan_key": (C.c_int, [C.c_char_p, C.POINTER(Job), C.POINTER(C.c_double), C.POINTER(Physics), C.c_char_p,
-                              C.c_size_t]),
+    "mq_scan_key": (C.c_int, [C.c_char_p, C.POINTER(Job), C.POINTER(C.c_double), C.POINTER(Physics), C.c_int32,
+                              C.c_char_p, C.c_size_t]),
     "mq_format_f64": (C.c_int, [C.c_double, C.c_int32, C.c_char_p, C.c_size_t]),
     "mq_export_chion": (C.c_int, [C.POINTER(IonSoA), C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(Job), C.c_size_t,
                                   C.POINTER(Physics), C.c_void_p]),
@@ -802,10 +802,11 @@ def format_f64(x: float, display: bool = False) -> str:
     return buf.value.decode()
 
 
-def scan_key(geom_key: str, j: Job, dc, ph: Physics) -> str:
-    """Cache key of one job (scan.rs:9-11)."""
+def scan_key(geom_key: str, j: Job, dc, ph: Physics, path: int = PATH_BATCHED) -> str:
+    """Cache key of one job (scan.rs:9-11) for a run on `path`."""
     buf = C.create_string_buffer(4096)
-    _check(load().mq_scan_key(geom_key.encode(), C.byref(j), (C.c_double * 4)(*dc), C.byref(ph), buf, len(buf)))
+    _check(load().mq_scan_key(geom_key.encode(), C.byref(j), (C.c_double * 4)(*dc), C.byref(ph), int(path), buf,
+                              len(buf)))
     return buf.value.decode()
 
 

@@ -70,3 +70,25 @@ def test_adaptive_scan_structure(eng, tmp_path):
     assert all(p[3] == (80 if p[0] in fine else 50) for p in pts)
     lines = open(cache).read().splitlines()
     assert len(lines) == len(fs) + len(fine)
+
+
+def test_cached_batch_keeps_streams_and_paths_apart(eng, tmp_path):
+    """A cached Philox / FP32-batched line is not returned to a COMPAT or FP64-plain request for the same
+    job, and each request then reads back its own line."""
+    print("[gpu] cached_batch: streams and paths keyed apart", flush=True)
+    cache = str(tmp_path / "results.tsv")
+    ph = _ph()
+    jobs = [mqhip.job(499.0, 1.0, 230.0, 100.0, 500e3, 0.5, 100e3, 150, 11)]
+    outs = {}
+    for rng in (mqhip.RNG_PHILOX, mqhip.RNG_COMPAT):
+        for path in (mqhip.PATH_BATCHED, mqhip.PATH_PLAIN):
+            p2 = ph.copy(rng=rng)
+            got, computed = mqhip.cached_batch(eng, GEOM, jobs, [0, 5, 0, 0], p2, cache, path)
+            assert computed == 1, (rng, path)
+            want = eng.run_batch(jobs, [0, 5, 0, 0], p2, path)
+            assert got[0].as_dict() == want[0].as_dict()
+            outs[(rng, path)] = got[0].as_dict()
+    assert len(open(cache).read().splitlines()) == 4
+    for (rng, path), o in outs.items():
+        got, computed = mqhip.cached_batch(eng, GEOM, jobs, [0, 5, 0, 0], ph.copy(rng=rng), cache, path)
+        assert computed == 0 and got[0].as_dict() == o

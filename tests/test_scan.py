@@ -92,3 +92,26 @@ def test_adaptive_refinement_rule():
         assert got == fine_points_reference(fs, t1, 200, base, 2.0)
     # a flat scan at the baseline refines nothing
     assert mqhip.adaptive_fine_points([1.0, 2.0, 3.0], [0.0, 0.0, 0.0], 100, 0.0, 1.0) == []
+
+
+def test_key_separates_streams_and_paths():
+    """A request for the reference's own random streams (rng COMPAT) or for the FP64 plain path must not
+    find a line that a default (Philox, FP32 batched) run wrote, and the reverse; the default key stays the
+    reference's format (scan.rs:9-11), with no extension field."""
+    ph = mqhip.Physics.legacy(1e-3 * 133.322)
+    j = mqhip.job(2495.0, 5.0, 0.0, 100.0, 500e3, 0.5, 100e3, 1000, 1499)
+    dc = [0.0, 5.0, -5.0, 0.0]
+    default = mqhip.scan_key("G", j, dc, ph)
+    assert default == mqhip.scan_key("G", j, dc, ph, mqhip.PATH_BATCHED)
+    assert "rng" not in default and "path" not in default
+    compat = mqhip.scan_key("G", j, dc, ph.copy(rng=mqhip.RNG_COMPAT))
+    plain = mqhip.scan_key("G", j, dc, ph, mqhip.PATH_PLAIN)
+    both = mqhip.scan_key("G", j, dc, ph.copy(rng=mqhip.RNG_COMPAT), mqhip.PATH_PLAIN)
+    assert len({default, compat, plain, both}) == 4
+    assert compat.endswith("r_inject_mm: 0.0, rng: 1 }")
+    assert plain.endswith("r_inject_mm: 0.0, path: 1 }")
+    assert both.endswith("r_inject_mm: 0.0, rng: 1, path: 1 }")
+    # everything before the Physics extension fields is the default key
+    for k in (compat, plain, both):
+        assert k.split("|")[:6] == default.split("|")[:6]
+        assert k.startswith(default[:-2])
