@@ -177,6 +177,30 @@ int mq_run_batch(mq_engine* eng, const mq_job* jobs, size_t njobs, const double 
 int mq_run_batch_ions(mq_engine* eng, const mq_job* jobs, size_t njobs, const double dc[4],
                       const mq_physics* ph, int32_t path, const mq_ion_soa* out, uint32_t* out_job);
 
+/* Largest batch of mq_run_batch_ions_noalloc. */
+#define MQ_NOALLOC_MAX_IONS 65536
+#define MQ_NOALLOC_MAX_JOBS 4096
+
+/* mq_run_batch_ions for callers that cannot allocate device memory (code that
+ * runs under HIP graph capture elsewhere, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. Same arguments, results
+ * and errors as mq_run_batch_ions, bit for bit, with these limits:
+ *  - at most MQ_NOALLOC_MAX_IONS ions and MQ_NOALLOC_MAX_JOBS jobs (larger
+ *    batches are rejected with an error);
+ *  - ph->rng must be MQ_RNG_PHILOX (the reference's streams are injected on
+ *    the host into per-call buffers);
+ *  - the engine's field tables for this path must exist: one earlier
+ *    mq_run_batch / mq_run_batch_ions call on the same path builds them (a new
+ *    DC configuration is then recomputed into them in place).
+ * The per-call device buffers (ion state, live lists, re-sort bins, counters,
+ * job table, accumulators, about 12 MB) are a static array of the library that
+ * every GPU holds once, for the life of the process; the call calls no
+ * hipMalloc or hipFree. All engines on one GPU share that array: calls that
+ * overlap on one GPU, from several threads or engines, race. The batched path
+ * runs without hipGraph capture here. */
+int mq_run_batch_ions_noalloc(mq_engine* eng, const mq_job* jobs, size_t njobs, const double dc[4],
+                              const mq_physics* ph, int32_t path, const mq_ion_soa* out, uint32_t* out_job);
+
 /* Engine::inject (channel.rs:342-374) on the host with the reference's
  * xorshift64 stream (the injection of MQ_RNG_COMPAT): the initial state of
  * the n ions of one job in s (positions after the reference's fixed-point

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -31,6 +32,81 @@ thread_local std::string g_err;
 
 }  // namespace
 
+// ---------------------------------------------------------- no-alloc scratch
+// Device memory of mq_run_batch_ions_noalloc: the per-call buffers of a batch
+// of at most MQ_NOALLOC_MAX_IONS ions and MQ_NOALLOC_MAX_JOBS jobs (ion state,
+// live lists, re-sort bins, counters, job table, per-job accumulators) are
+// carved from this array instead of being allocated. Every GPU holds its own
+// copy, at its own address, until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns for the device (noalloc_ptr),
+// never through the array's name. Calls on one GPU share the array, so calls
+// that overlap on one GPU race.
+namespace {
+
+constexpr size_t NA_ALIGN = 256;
+constexpr size_t na_round(size_t b) { return (b + NA_ALIGN - 1) / NA_ALIGN * NA_ALIGN; }
+constexpr size_t NA_MAX_BINS = 1024u * 1024u;  // the re-sort's own limit
+constexpr size_t NA_CTR_U32 = 2 * CTR_BLOCK;
+constexpr size_t NA_IONS = MQ_NOALLOC_MAX_IONS, NA_JOBS = MQ_NOALLOC_MAX_JOBS;
+constexpr size_t NA_BYTES = 7 * na_round(NA_IONS * 8)      // x y z phi vx vy vz
+                            + 7 * na_round(NA_IONS * 4)    // fvx fvy fvz steps ncoll cd job
+                            + na_round(NA_IONS)            // status
+                            + 5 * na_round(NA_IONS * 4)    // three lists, bin_of, rank_of
+                            + na_round(NA_CTR_U32 * 4)     // two counter blocks
+                            + na_round(NA_MAX_BINS * 4)    // histogram / bin offsets
+                            + na_round(1024 * 4)           // block sums of the scan
+                            + na_round(NA_JOBS * sizeof(JobP))
+                            + na_round(NA_JOBS * ACC * 8);  // accumulators
+
+__device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_na_mu;
+void* g_na_ptr[NA_MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device` (the current device), looked up once
+// per device.
+void* noalloc_ptr(int device)
+{
+    if (device < 0 || device >= NA_MAX_DEVICES) throw Err(-1, "device index out of range for the no-alloc scratch");
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[device]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Err(-3, "hipGetSymbolAddress returned no address for the no-alloc scratch");
+        g_na_ptr[device] = p;
+    }
+    return g_na_ptr[device];
+}
+
+// Sub-buffers of the scratch array, 256-byte aligned.
+struct Carver {
+    unsigned char* base;
+    size_t off = 0;
+    template <class T> T* take(size_t count)
+    {
+        const size_t b = na_round(count * sizeof(T));
+        if (off + b > NA_BYTES) throw Err(-1, "no-alloc scratch overflow");
+        T* p = reinterpret_cast<T*>(base + off);
+        off += b;
+        return p;
+    }
+};
+
+}  // namespace
+
+// Device buffers of one batch: pooled on the engine (grown on demand), or carved
+// from the no-alloc scratch array.
+struct CallBufs {
+    JobP* jobs = nullptr;
+    unsigned long long* acc = nullptr;
+    uint32_t* list[3] = {nullptr, nullptr, nullptr};
+    uint32_t* ctr = nullptr;
+    uint32_t *bin_of = nullptr, *rank_of = nullptr, *hist = nullptr, *bsum = nullptr;
+    IonState S{};
+    bool noalloc = false;
+};
+
 struct mq_engine {
     int device = 0;
     int cus = 0;
@@ -51,6 +127,11 @@ struct mq_engine {
     DevBuf jobs, acc, list[3], ctr, bin_of, rank_of, hist, bsum;
     DevBuf sx, sy, sz, sphi, svx, svy, svz, sfx, sfy, sfz, ssteps, sncoll, scd, sjob, sstatus, srng;
     mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0, 4};
+    // no-alloc entry point: this device's scratch array and host staging
+    // reserved at creation for MQ_NOALLOC_MAX_JOBS jobs
+    void* scratch_dev = nullptr;
+    std::vector<JobP> jp_host;
+    std::vector<unsigned long long> acc_host;
     // differentiable form: context of the last forward
     bool adj_valid = false;
     uint32_t adj_n = 0;
@@ -105,6 +186,9 @@ struct mq_engine {
         if (f->mask) HIPCHK(hipMemcpyAsync(mask.p, f->mask, nodes(), hipMemcpyHostToDevice, stream));
         else HIPCHK(hipMemsetAsync(mask.p, 0, nodes(), stream));
         HIPCHK(hipStreamSynchronize(stream));
+        scratch_dev = noalloc_ptr(device);
+        jp_host.reserve(MQ_NOALLOC_MAX_JOBS);
+        acc_host.reserve((size_t)MQ_NOALLOC_MAX_JOBS * ACC);
     }
 
     void destroy()
@@ -123,8 +207,15 @@ struct mq_engine {
 
     // Combined DC field (Engine::dc_buffer, channel.rs:384-395) and, for the
     // batched path, its FP32 cell table (brick_buffer, tiled.rs:65-84).
-    void prepare_dc(const double dc[4], bool want_cells)
+    // noalloc: the tables must already have their full size (an earlier call on
+    // this path built them); a new DC configuration is recomputed in place.
+    void prepare_dc(const double dc[4], bool want_cells, bool noalloc = false)
     {
+        const size_t nc = (size_t)(nx - 1) * (ny - 1) * (nz - 1);
+        if (noalloc && (dcc.bytes < 3 * nodes() * sizeof(double) ||
+                        (want_cells && cells.bytes < nc * CELL_STRIDE_F4 * sizeof(float4))))
+            throw Err(-1, "the no-alloc entry point needs the engine's field tables: make one mq_run_batch call "
+                          "on this path first");
         bool same = dc_valid;
         for (int b = 0; b < 4; ++b) same = same && dc_key[b] == dc[b];
         if (!same) {
@@ -143,7 +234,6 @@ struct mq_engine {
             cells_valid = false;
         }
         if (want_cells && !cells_valid) {
-            const size_t nc = (size_t)(nx - 1) * (ny - 1) * (nz - 1);
             cells.ensure(nc * CELL_STRIDE_F4 * sizeof(float4));
             launch_build_cells(cells.as<float4>(), rf.as<double>(), dcc.as<double>(), ac.as<double>(),
                                mask.as<uint8_t>(), nx, ny, nz, stream);
@@ -167,9 +257,68 @@ struct mq_engine {
         return S;
     }
 
+    // Device buffers of one batch of n ions and njobs jobs on `path`; compat adds
+    // the xoshiro128** states of the reference's streams (S.rng, 4 words per ion).
+    CallBufs call_bufs(size_t n, size_t njobs, bool noalloc, int path, bool compat)
+    {
+        CallBufs c;
+        c.noalloc = noalloc;
+        const bool batched = path == MQ_PATH_BATCHED;
+        const bool resort = batched && tune.resort != 0;
+        const uint32_t nbins = resort ? sort_geom().nbins : 0u;
+        if (nbins > NA_MAX_BINS) throw Err(-1, "grid too large for the re-sort bins");
+        const int nlists = batched ? (resort ? 3 : 2) : 0;
+        if (!noalloc) {
+            jobs.ensure(njobs * sizeof(JobP));
+            c.jobs = jobs.as<JobP>();
+            acc.ensure(njobs * ACC * sizeof(unsigned long long));
+            c.acc = acc.as<unsigned long long>();
+            c.S = state_buffers(std::max<size_t>(n, 1));
+            if (compat) {
+                srng.ensure(std::max<size_t>(n, 1) * 16);
+                c.S.rng = srng.as<uint32_t>();
+            }
+            ctr.ensure(NA_CTR_U32 * 4);
+            c.ctr = ctr.as<uint32_t>();
+            for (int k = 0; k < nlists; ++k) {
+                list[k].ensure(n * 4);
+                c.list[k] = list[k].as<uint32_t>();
+            }
+            if (resort) {
+                bin_of.ensure(n * 4);
+                rank_of.ensure(n * 4);
+                hist.ensure((size_t)nbins * 4);
+                bsum.ensure(1024 * 4);
+                c.bin_of = bin_of.as<uint32_t>(); c.rank_of = rank_of.as<uint32_t>();
+                c.hist = hist.as<uint32_t>(); c.bsum = bsum.as<uint32_t>();
+            }
+            return c;
+        }
+        if (n > MQ_NOALLOC_MAX_IONS || njobs > MQ_NOALLOC_MAX_JOBS) throw Err(-1, "no-alloc batch beyond its maximum");
+        if (compat) throw Err(-1, "the no-alloc entry point runs the Philox streams only (rng = 0)");
+        Carver cv{static_cast<unsigned char*>(scratch_dev)};
+        const size_t m = std::max<size_t>(n, 1);
+        IonState& S = c.S;
+        S.x = cv.take<double>(m); S.y = cv.take<double>(m); S.z = cv.take<double>(m); S.phi = cv.take<double>(m);
+        S.vx = cv.take<double>(m); S.vy = cv.take<double>(m); S.vz = cv.take<double>(m);
+        S.fvx = cv.take<float>(m); S.fvy = cv.take<float>(m); S.fvz = cv.take<float>(m);
+        S.steps = cv.take<uint32_t>(m); S.ncoll = cv.take<uint32_t>(m); S.cd = cv.take<uint32_t>(m);
+        S.job = cv.take<uint32_t>(m); S.status = cv.take<uint8_t>(m);
+        S.rng = nullptr;
+        for (int k = 0; k < nlists; ++k) c.list[k] = cv.take<uint32_t>(m);
+        if (resort) {
+            c.bin_of = cv.take<uint32_t>(m); c.rank_of = cv.take<uint32_t>(m);
+            c.hist = cv.take<uint32_t>(nbins); c.bsum = cv.take<uint32_t>(1024);
+        }
+        c.ctr = cv.take<uint32_t>(NA_CTR_U32);
+        c.jobs = cv.take<JobP>(std::max<size_t>(njobs, 1));
+        c.acc = cv.take<unsigned long long>(std::max<size_t>(njobs, 1) * ACC);
+        return c;
+    }
+
     // The reference's injection and xoshiro128** seeds (MQ_RNG_COMPAT), on the
     // host as Engine::inject does it, uploaded into the ion state.
-    void inject_compat(IonState& S, const mq_job* jobs_h, size_t njobs, const mq_physics& ph, uint32_t n)
+    void inject_compat(const IonState& S, const mq_job* jobs_h, size_t njobs, const mq_physics& ph, uint32_t n)
     {
         std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n), phi(n);
         std::vector<uint32_t> cd(n), rng(4 * (size_t)n), job(n), zero(n, 0u);
@@ -183,8 +332,7 @@ struct mq_engine {
             for (uint64_t i = 0; i < m; ++i) job[off + i] = (uint32_t)k;
             off += m;
         }
-        srng.ensure((size_t)n * 16);
-        S.rng = srng.as<uint32_t>();
+        if (!S.rng) throw Err(-1, "internal error: no random-stream state buffer for the reference's streams");
         auto up = [&](void* dst, const void* src, size_t bytes) {
             HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
         };
@@ -225,9 +373,11 @@ struct mq_engine {
     }
 
     // One batch: inject, run to completion on `path`, summarize. Returns the
-    // device ion state (valid until the next call).
+    // device ion state (valid until the next call). noalloc: the per-call
+    // buffers come from the scratch array, and nothing is allocated on the device
+    // (the host staging vectors stay within the capacity reserved at creation).
     IonState run(const mq_job* jobs_h, size_t njobs, const double dc[4], const mq_physics& ph, int path,
-                 std::vector<unsigned long long>& acc_h, size_t& n_total)
+                 std::vector<unsigned long long>& acc_h, size_t& n_total, bool noalloc = false)
     {
         auto t_wall = std::chrono::steady_clock::now();
         adj_valid = false;
@@ -235,21 +385,33 @@ struct mq_engine {
         if (!(ph.dt > 0.0)) throw Err(-1, "dt must be positive");
         if (path != MQ_PATH_BATCHED && path != MQ_PATH_PLAIN) throw Err(-1, "unknown path");
         if (ph.rng != MQ_RNG_PHILOX && ph.rng != MQ_RNG_COMPAT) throw Err(-1, "unknown random stream (rng)");
-        std::vector<JobP> jp(njobs);
+        if (noalloc) {
+            if (ph.rng != MQ_RNG_PHILOX)
+                throw Err(-1, "the no-alloc entry point runs the Philox streams only (rng = 0): the reference's "
+                              "streams are injected on the host");
+            if (njobs > MQ_NOALLOC_MAX_JOBS)
+                throw Err(-1, "no-alloc batch has " + std::to_string(njobs) + " jobs, more than its maximum of " +
+                                  std::to_string(MQ_NOALLOC_MAX_JOBS));
+        }
+        std::vector<JobP>& jp = jp_host;
+        jp.resize(njobs);
         uint64_t off = 0;
         for (size_t k = 0; k < njobs; ++k) {
             jp[k] = job_params(jobs_h[k], ph, off);
             off += jobs_h[k].n;
         }
         if (off >= 0xFFFFFF00ull) throw Err(-1, "too many ions in one batch (limit 2^32 - 256)");
+        if (noalloc && off > MQ_NOALLOC_MAX_IONS)
+            throw Err(-1, "no-alloc batch has " + std::to_string(off) + " ions, more than its maximum of " +
+                              std::to_string(MQ_NOALLOC_MAX_IONS));
         const uint32_t n = (uint32_t)off;
         n_total = n;
-        jobs.ensure(njobs * sizeof(JobP));
-        HIPCHK(hipMemcpyAsync(jobs.p, jp.data(), njobs * sizeof(JobP), hipMemcpyHostToDevice, stream));
-        acc.ensure(njobs * ACC * sizeof(unsigned long long));
-        launch_fill_u64(acc.as<unsigned long long>(), 0ull, (uint32_t)(njobs * ACC), stream);
-        prepare_dc(dc, path == MQ_PATH_BATCHED);
-        IonState S = state_buffers(std::max<size_t>(n, 1));
+        prepare_dc(dc, path == MQ_PATH_BATCHED, noalloc);
+        const CallBufs cb = call_bufs(n, njobs, noalloc, path, ph.rng == MQ_RNG_COMPAT);
+        HIPCHK(hipMemcpyAsync(cb.jobs, jp.data(), njobs * sizeof(JobP), hipMemcpyHostToDevice, stream));
+        launch_fill_u64(cb.acc, 0ull, (uint32_t)(njobs * ACC), stream);
+        // every kernel of the batch, injection included, reads this one set of pointers
+        const IonState& S = cb.S;
         InjP ip{};
         ip.axis_x = meta.axis_x; ip.axis_y = meta.axis_y; ip.z_inject = meta.z_inject;
         ip.temp_k = ph.temp_k; ip.r_inject_mm = ph.r_inject_mm; ip.inj_sigma_mm = ph.inj_sigma_mm; ip.dt = ph.dt;
@@ -257,7 +419,7 @@ struct mq_engine {
         if (ph.rng == MQ_RNG_COMPAT) {
             if (n) inject_compat(S, jobs_h, njobs, ph, n);
         } else {
-            launch_inject(S, jobs.as<JobP>(), (uint32_t)njobs, ip, n, stream);
+            launch_inject(S, cb.jobs, (uint32_t)njobs, ip, n, stream);
         }
         HIPCHK(hipGetLastError());
         const BatchP B = batch_params(ph, (uint32_t)njobs);
@@ -265,13 +427,13 @@ struct mq_engine {
         stats.n_ions = n;
         float kms = 0.0f;
         if (n) {
-            if (path == MQ_PATH_PLAIN) run_plain(S, B, n, kms);
-            else run_batched(S, B, n, kms);
+            if (path == MQ_PATH_PLAIN) run_plain(cb, B, n, kms);
+            else run_batched(cb, B, n, kms);
         }
-        launch_summarize(S, acc.as<unsigned long long>(), n, stream);
+        launch_summarize(S, cb.acc, n, stream);
         HIPCHK(hipGetLastError());
         acc_h.resize(njobs * ACC);
-        HIPCHK(hipMemcpyAsync(acc_h.data(), acc.p, njobs * ACC * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+        HIPCHK(hipMemcpyAsync(acc_h.data(), cb.acc, njobs * ACC * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                               stream));
         HIPCHK(hipStreamSynchronize(stream));
         uint64_t steps = 0, unfinished = 0;
@@ -287,18 +449,17 @@ struct mq_engine {
         return S;
     }
 
-    void run_plain(const IonState& S, const BatchP& B, uint32_t n, float& kms)
+    void run_plain(const CallBufs& cb, const BatchP& B, uint32_t n, float& kms)
     {
         uint32_t nsteps = (uint32_t)std::max(1000.0, std::floor(2e-4 / B.dt));
         nsteps = (nsteps + 3u) & ~3u;
-        ctr.ensure(256);
-        uint32_t* live = ctr.as<uint32_t>();
+        uint32_t* live = cb.ctr;
         kms = 0.0f;
         for (;;) {
             launch_fill_u32(live, 0u, 1, stream);
             HIPCHK(hipEventRecord(ev0, stream));
-            launch_plain(S, jobs.as<JobP>(), B, rf.as<double>(), dcc.as<double>(), ac.as<double>(),
-                         mask.as<uint8_t>(), n, nsteps, live, stream);
+            launch_plain(cb.S, cb.jobs, B, rf.as<double>(), dcc.as<double>(), ac.as<double>(), mask.as<uint8_t>(), n,
+                         nsteps, live, stream);
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(ev1, stream));
             stats.launches += 1;
@@ -439,32 +600,32 @@ struct mq_engine {
     // pass_bins: each launch writes the bins of the ions it carries, so that the
     // next re-sort skips the gather of their positions (off in graph mode, where
     // one captured block must serve every replay).
-    void enqueue_block(const IonState& S, const BatchP& B, uint32_t n, const SortGeom& G, int& cur, uint32_t& chunk,
+    void enqueue_block(const CallBufs& cb, const BatchP& B, uint32_t n, const SortGeom& G, int& cur, uint32_t& chunk,
                        uint32_t cap, uint32_t period, uint32_t grid, uint32_t block, int iters, bool timed,
                        std::vector<hipEvent_t>* evs, bool resort, bool& bins_known, bool pass_bins)
     {
-        uint32_t* c0 = ctr.as<uint32_t>();
+        uint32_t* c0 = cb.ctr;
         uint32_t* c1 = c0 + CTR_BLOCK;
         for (int it = 0; it < iters; ++it) {
             uint32_t* cin = cur ? c1 : c0;
             uint32_t* cout = cur ? c0 : c1;
-            const uint32_t* lin = list[cur].as<uint32_t>();
+            const uint32_t* lin = cb.list[cur];
             // The injected list is not sorted: its ions all start in the few bins
             // of the injection spot, where sorting gains nothing and the bin
             // counts serialise on a handful of atomics (0.5 ms on the mass scan).
             // The first launch takes it as it is and writes the bins for the next.
             if (resort && (bins_known || !pass_bins)) {
-                launch_sort(S, B, lin, cin, list[2].as<uint32_t>(), bin_of.as<uint32_t>(), rank_of.as<uint32_t>(),
-                            hist.as<uint32_t>(), bsum.as<uint32_t>(), G, n, bins_known, stream);
-                lin = list[2].as<uint32_t>();
+                launch_sort(cb.S, B, lin, cin, cb.list[2], cb.bin_of, cb.rank_of, cb.hist, cb.bsum, G, n, bins_known,
+                            stream);
+                lin = cb.list[2];
             }
             WorkerArgs a{};
-            a.S = S;
-            a.jobs = jobs.as<JobP>();
+            a.S = cb.S;
+            a.jobs = cb.jobs;
             a.B = B;
             a.cells = cells.as<float4>();
             a.list_in = lin;
-            a.list_out = list[cur ^ 1].as<uint32_t>();
+            a.list_out = cb.list[cur ^ 1];
             a.ctr_in = cin;
             a.ctr_out = cout;
             a.chunk = chunk;
@@ -472,7 +633,7 @@ struct mq_engine {
             // one queue per XCD, each the home of at least one workgroup
             a.nqueues = resort ? std::min<uint32_t>(MAX_QUEUES, grid) : 1u;
             a.coll_batch = (uint32_t)std::max(1, tune.coll_batch);
-            a.bin_out = resort && pass_bins ? bin_of.as<uint32_t>() : nullptr;
+            a.bin_out = resort && pass_bins ? cb.bin_of : nullptr;
             a.sg = G;
             bins_known = a.bin_out != nullptr;
             launch_prepare(a, stream);
@@ -519,22 +680,14 @@ struct mq_engine {
         return G;
     }
 
-    void run_batched(const IonState& S, const BatchP& B, uint32_t n, float& kms)
+    // noalloc: no graph capture and no per-launch events (the engine's two events
+    // time the whole run); results per ion are the same either way.
+    void run_batched(const CallBufs& cb, const BatchP& B, uint32_t n, float& kms)
     {
         const bool resort = tune.resort != 0;
-        for (int k = 0; k < (resort ? 3 : 2); ++k) list[k].ensure((size_t)n * 4);
-        ctr.ensure(2 * CTR_BLOCK * 4);
-        uint32_t* c0 = ctr.as<uint32_t>();
-        SortGeom G{};
-        if (resort) {
-            G = sort_geom();
-            if (G.nbins > 1024u * 1024u) throw Err(-1, "grid too large for the re-sort bins");
-            bin_of.ensure((size_t)n * 4);
-            rank_of.ensure((size_t)n * 4);
-            hist.ensure((size_t)G.nbins * 4);
-            bsum.ensure(1024 * 4);
-        }
-        launch_iota(list[0].as<uint32_t>(), n, stream);
+        uint32_t* c0 = cb.ctr;
+        const SortGeom G = resort ? sort_geom() : SortGeom{};
+        launch_iota(cb.list[0], n, stream);
         launch_fill_u32(c0 + CTR_COUNT, n, 1, stream);
         const uint32_t block = (uint32_t)tune.block;
         const uint32_t grid = std::max<uint32_t>(1, (uint32_t)((uint64_t)cus * tune.waves_per_cu * 64 / block));
@@ -544,7 +697,7 @@ struct mq_engine {
         const int iters = std::max(1, tune.sync_every);
         kms = 0.0f;
         int cur = 0;
-        if (tune.graph) {
+        if (tune.graph && !cb.noalloc) {
             // capture one block of `iters` launches; replay it until the list is empty
             hipGraph_t g = nullptr;
             hipGraphExec_t ge = nullptr;
@@ -553,7 +706,7 @@ struct mq_engine {
             bool known = false;
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             try {
-                enqueue_block(S, B, n, G, cur_cap, chunk_cap, cap, period, grid, block, iters, false, nullptr, resort,
+                enqueue_block(cb, B, n, G, cur_cap, chunk_cap, cap, period, grid, block, iters, false, nullptr, resort,
                               known, false);
             } catch (...) {
                 (void)hipStreamEndCapture(stream, &g);
@@ -587,11 +740,13 @@ struct mq_engine {
             (void)hipGraphDestroy(g);
             return;
         }
-        if ((int)evs.size() < 2 * iters) {
+        const bool timed = !cb.noalloc;
+        if (timed && (int)evs.size() < 2 * iters) {
             for (auto e : evs) (void)hipEventDestroy(e);
             evs.assign(2 * iters, nullptr);
             for (auto& e : evs) HIPCHK(hipEventCreate(&e));
         }
+        if (!timed) HIPCHK(hipEventRecord(ev0, stream));
         // Tail: once no more ions are live than the launch has lanes, a re-sort no
         // longer puts several ions of one region into a wavefront, and the launches
         // are bound by the step latency of the longest-lived ions. The remaining
@@ -602,7 +757,7 @@ struct mq_engine {
         bool tail = false;
         bool bins_known = false;  // list[0] is the injected list: no bins yet
         for (;;) {
-            enqueue_block(S, B, n, G, cur, chunk, cap, period, grid, block, iters, true, &evs, resort && !tail,
+            enqueue_block(cb, B, n, G, cur, chunk, cap, period, grid, block, iters, timed, &evs, resort && !tail,
                           bins_known, true);
             HIPCHK(hipGetLastError());
             stats.launches += (uint32_t)iters;
@@ -610,16 +765,23 @@ struct mq_engine {
             uint32_t* cfinal = cur ? c0 + CTR_BLOCK : c0;
             HIPCHK(hipMemcpyAsync(&alive, cfinal + CTR_COUNT, 4, hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
-            for (int it = 0; it < iters; ++it) {
-                float ms = 0.0f;
-                HIPCHK(hipEventElapsedTime(&ms, evs[2 * it], evs[2 * it + 1]));
-                kms += ms;
+            if (timed) {
+                for (int it = 0; it < iters; ++it) {
+                    float ms = 0.0f;
+                    HIPCHK(hipEventElapsedTime(&ms, evs[2 * it], evs[2 * it + 1]));
+                    kms += ms;
+                }
             }
             if (alive == 0) break;
             if (resort && !tail && alive <= lanes) {
                 tail = true;
                 cap = std::max(cap, 200000u);
             }
+        }
+        if (!timed) {
+            HIPCHK(hipEventRecord(ev1, stream));
+            HIPCHK(hipEventSynchronize(ev1));
+            HIPCHK(hipEventElapsedTime(&kms, ev0, ev1));
         }
     }
 };
@@ -796,6 +958,43 @@ int mq_run_batch(mq_engine* eng, const mq_job* jobs, size_t njobs, const double 
     })
 }
 
+// Final states of the last run into the caller's arrays. The FP32 velocities
+// of the batched path are copied into the caller's double arrays and widened in
+// place (from the last element down: element i is read before the doubles at
+// or above i, which cover floats 2i and 2i + 1, are written), so no host buffer
+// is needed.
+static void copy_out(mq_engine* eng, const IonState& S, size_t n, int32_t path, const mq_ion_soa* out,
+                     uint32_t* out_job)
+{
+    hipStream_t st = eng->stream;
+    auto d2h = [&](void* dst, const void* src, size_t bytes) {
+        if (dst) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    };
+    d2h(out->x, S.x, n * 8); d2h(out->y, S.y, n * 8); d2h(out->z, S.z, n * 8);
+    d2h(out->steps, S.steps, n * 4); d2h(out->ncoll, S.ncoll, n * 4); d2h(out->status, S.status, n);
+    d2h(out->phi_ac0, S.phi, n * 8); d2h(out->cd, S.cd, n * 4);
+    if (out_job) d2h(out_job, S.job, n * 4);
+    if (path == MQ_PATH_PLAIN) {
+        d2h(out->vx, S.vx, n * 8); d2h(out->vy, S.vy, n * 8); d2h(out->vz, S.vz, n * 8);
+        HIPCHK(hipStreamSynchronize(st));
+        return;
+    }
+    double* dsts[3] = {out->vx, out->vy, out->vz};
+    float* srcs[3] = {S.fvx, S.fvy, S.fvz};
+    for (int d = 0; d < 3; ++d) d2h(dsts[d], srcs[d], n * 4);
+    HIPCHK(hipStreamSynchronize(st));
+    for (int d = 0; d < 3; ++d) {
+        if (!dsts[d]) continue;
+        unsigned char* b = reinterpret_cast<unsigned char*>(dsts[d]);
+        for (size_t i = n; i-- > 0;) {
+            float f;
+            std::memcpy(&f, b + 4 * i, 4);
+            const double v = (double)f;
+            std::memcpy(b + 8 * i, &v, 8);
+        }
+    }
+}
+
 int mq_run_batch_ions(mq_engine* eng, const mq_job* jobs, size_t njobs, const double dc[4], const mq_physics* ph,
                       int32_t path, const mq_ion_soa* out, uint32_t* out_job)
 {
@@ -805,29 +1004,19 @@ int mq_run_batch_ions(mq_engine* eng, const mq_job* jobs, size_t njobs, const do
         size_t n = 0;
         IonState S = eng->run(jobs, njobs, dc, *ph, path, acc, n);
         if (n == 0) return 0;
-        hipStream_t st = eng->stream;
-        auto d2h = [&](void* dst, const void* src, size_t bytes) {
-            if (dst) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
-        };
-        d2h(out->x, S.x, n * 8); d2h(out->y, S.y, n * 8); d2h(out->z, S.z, n * 8);
-        d2h(out->steps, S.steps, n * 4); d2h(out->ncoll, S.ncoll, n * 4); d2h(out->status, S.status, n);
-        d2h(out->phi_ac0, S.phi, n * 8); d2h(out->cd, S.cd, n * 4);
-        if (out_job) d2h(out_job, S.job, n * 4);
-        if (path == MQ_PATH_PLAIN) {
-            d2h(out->vx, S.vx, n * 8); d2h(out->vy, S.vy, n * 8); d2h(out->vz, S.vz, n * 8);
-            HIPCHK(hipStreamSynchronize(st));
-        } else {
-            std::vector<float> f(n);
-            double* dsts[3] = {out->vx, out->vy, out->vz};
-            float* srcs[3] = {S.fvx, S.fvy, S.fvz};
-            for (int d = 0; d < 3; ++d) {
-                if (!dsts[d]) continue;
-                HIPCHK(hipMemcpyAsync(f.data(), srcs[d], n * 4, hipMemcpyDeviceToHost, st));
-                HIPCHK(hipStreamSynchronize(st));
-                for (size_t i = 0; i < n; ++i) dsts[d][i] = (double)f[i];
-            }
-            HIPCHK(hipStreamSynchronize(st));
-        }
+        copy_out(eng, S, n, path, out, out_job);
+    })
+}
+
+int mq_run_batch_ions_noalloc(mq_engine* eng, const mq_job* jobs, size_t njobs, const double dc[4],
+                              const mq_physics* ph, int32_t path, const mq_ion_soa* out, uint32_t* out_job)
+{
+    MQ_GUARD({
+        if (!eng || !ph || !dc || !out) throw Err(-1, "NULL argument");
+        size_t n = 0;
+        IonState S = eng->run(jobs, njobs, dc, *ph, path, eng->acc_host, n, true);
+        if (n == 0) return 0;
+        copy_out(eng, S, n, path, out, out_job);
     })
 }
 

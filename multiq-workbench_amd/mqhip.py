@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libmqhip.so")
 
 ALIVE, CRASHED, RETURNED, TRANSMITTED, LEFT, TIMEOUT = range(6)
 PATH_BATCHED, PATH_PLAIN = 0, 1
+NOALLOC_MAX_IONS, NOALLOC_MAX_JOBS = 65536, 4096  # MQ_NOALLOC_MAX_IONS / _JOBS
 INJ_REFERENCE, INJ_GAUSSIAN = 0, 1
 RNG_PHILOX, RNG_COMPAT = 0, 1
 
@@ -231,6 +232,9 @@ SIGNATURES = {
                                C.c_int32, C.POINTER(Outcome)]),
     "mq_run_batch_ions": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_size_t, C.POINTER(C.c_double),
                                     C.POINTER(Physics), C.c_int32, C.POINTER(IonSoA), C.POINTER(C.c_uint32)]),
+    "mq_run_batch_ions_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_size_t, C.POINTER(C.c_double),
+                                            C.POINTER(Physics), C.c_int32, C.POINTER(IonSoA),
+                                            C.POINTER(C.c_uint32)]),
     "mq_summarize": (C.c_int, [C.POINTER(C.c_uint8), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                C.POINTER(C.c_uint32), C.c_size_t, C.c_size_t, C.c_double, C.POINTER(Outcome)]),
     "mq_last_stats": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
@@ -399,7 +403,9 @@ class Engine:
         _check(load().mq_run_batch(self._h, ja, len(jobs), dcv, C.byref(ph), path, out))
         return list(out)
 
-    def run_batch_ions(self, jobs, dc, ph: Physics, path: int = PATH_BATCHED) -> dict:
+    def run_batch_ions(self, jobs, dc, ph: Physics, path: int = PATH_BATCHED, noalloc: bool = False) -> dict:
+        """Engine::run_batch_ions (channel.rs:458-475); noalloc=True calls the entry point that allocates no
+        device memory (mq_run_batch_ions_noalloc)."""
         ja = jobs_array(jobs)
         n = int(sum(j.n for j in jobs))
         o = {k: np.zeros(n, np.float64) for k in ("x", "y", "z", "vx", "vy", "vz")}
@@ -411,8 +417,9 @@ class Engine:
         o["cd"] = np.zeros(n, np.uint32)
         soa = soa_of(o)
         dcv = (C.c_double * 4)(*dc)
-        _check(load().mq_run_batch_ions(self._h, ja, len(jobs), dcv, C.byref(ph), path, C.byref(soa),
-                                        o["job"].ctypes.data_as(C.POINTER(C.c_uint32))))
+        fn = load().mq_run_batch_ions_noalloc if noalloc else load().mq_run_batch_ions
+        _check(fn(self._h, ja, len(jobs), dcv, C.byref(ph), path, C.byref(soa),
+                  o["job"].ctypes.data_as(C.POINTER(C.c_uint32))))
         return o
 
     def stats(self) -> dict:
