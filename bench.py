@@ -367,7 +367,7 @@ def legacy_potentials_bench(cpu_sweeps: int = 40):
 
 
 def read_profile_value(name: str, key: str):
-    """A figure from the committed rocprofv3 PMC summary of the worker kernel, if any."""
+    """A figure from the committed rocprofv3 PMC summary of the worker kernel on this workload, if any."""
     path = os.path.join(ROOT, "profiles", f"k_batched_{name}_traffic.md")
     if not os.path.exists(path):
         return None
@@ -377,9 +377,124 @@ def read_profile_value(name: str, key: str):
     return None
 
 
-def read_traffic(name: str):
-    """HBM bytes per worker launch from the committed rocprofv3 PMC summary, if any."""
-    return read_profile_value(name, "traffic_bytes_per_launch")
+def roofline(name: str, steps: int, kms: float, launches: int) -> dict:
+    """Roofline object of the worker kernel k_batched over a timed run (steps ion-steps, kms ms of worker-kernel
+    time from HIP events on the engine's stream, `launches` launches).
+
+    achieved = HBM bytes per ion-step measured by rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE, scaled by a copy
+    kernel of known size; profiles/k_batched_<config>_traffic.md) x this run's ion-steps / this run's kernel time.
+    The gather demand of the step (8 corners x 3 bases x 3 components x 4 B + a 4-B mask word = 292 B per
+    ion-step) is reported beside it: the worker keeps each ion's cell record in registers, so that demand is not
+    what crosses HBM and its ratio to the peak can exceed 1."""
+    src = f"profiles/k_batched_{name}_traffic.md"
+    per_step = read_profile_value(name, "traffic_bytes_per_ion_step")
+    l2_per_step = read_profile_value(name, "l2_read_bytes_per_ion_step")
+    t = kms * 1e-3
+    demand = steps * BYTES_PER_STEP_FP32 / t / 1e9
+    r = {"bound": "hbm", "kernel": "k_batched", "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "ion_steps_per_launch": steps / max(launches, 1), "avg_launch_ms": kms / max(launches, 1),
+         "ion_steps_per_s_kernel": steps / t}
+    if per_step:
+        r["achieved"] = per_step * steps / t / 1e9
+        r["frac"] = r["achieved"] / HBM_PEAK_GBS
+        r["traffic"] = per_step * steps / max(launches, 1)
+        r["traffic_bytes_per_ion_step"] = per_step
+        r["traffic_source"] = (f"{src}: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per ion-step of this workload, "
+                               "times this run's ion-steps")
+    else:
+        r["achieved"] = None
+        r["frac"] = None
+        r["traffic"] = None
+        r["traffic_source"] = f"no {src}"
+    r["gather_demand"] = {"bytes_per_ion_step": BYTES_PER_STEP_FP32, "gbs": demand, "ratio_to_peak": demand / HBM_PEAK_GBS}
+    if l2_per_step:
+        r["l2_read"] = {"bytes_per_ion_step": l2_per_step, "gbs": l2_per_step * steps / t / 1e9,
+                        "source": f"{src}: TCP_TCC_READ_REQ x 128 B per ion-step"}
+    r["valu_busy"] = read_profile_value(name, "valu_busy")
+    r["valu_busy_source"] = f"{src}: 4 x SQ_INSTS_VALU / (1024 SIMDs x GRBM_GUI_ACTIVE / 8)"
+    return r
+
+
+def config_bench(name: str, steps: int, warmup: int, plain_steps: int, gather: bool = True, gate=None):
+    """K timed run_batch calls of one workload on the batched FP32 path, then the plain FP64 path on the same
+    inputs with per-job, per-class pulls between them. gate: override of the plate-basis voltage."""
+    grid, jobs, dc, ph, desc = workload(name)
+    if gate is not None:
+        dc = [dc[0], gate, dc[2], dc[3]]
+        desc += f"; gate {gate} V instead"
+    n_traj = int(sum(j.n for j in jobs))
+    eng = mqhip.Engine(grid)
+    log(f"{name}: engine ready: {grid['nx']}x{grid['ny']}x{grid['nz']} grid, {len(jobs)} jobs, {n_traj} trajectories")
+    for w in range(warmup):
+        eng.run_batch(jobs, dc, ph, mqhip.PATH_BATCHED)
+        log(f"{name}: warmup {w + 1}/{warmup}: {eng.stats()['ion_steps']} ion-steps")
+    tot = kms = launches = 0
+    outs = None
+    t0 = time.perf_counter()
+    for s in range(steps):
+        outs = eng.run_batch(jobs, dc, ph, mqhip.PATH_BATCHED)
+        st = eng.stats()
+        tot += st["ion_steps"]
+        kms += st["kernel_ms"]
+        launches += st["launches"]
+        log(f"{name}: step {s + 1}/{steps}: {st['ion_steps']} ion-steps, worker kernels {st['kernel_ms']:.1f} ms, "
+            f"wall {st['wall_ms']:.1f} ms")
+    wall = time.perf_counter() - t0
+    out = {"value": tot / wall, "unit": "ion-steps/s", "ms_per_step": 1e3 * wall / max(steps, 1),
+           "workload": desc, "trajectories": n_traj, "jobs": len(jobs), "grid": [grid["nx"], grid["ny"], grid["nz"]],
+           "h_mm": grid["h"] * 1e3, "dc": dc, "roofline": roofline(name, tot, kms, launches)}
+    if gather:
+        # achievable random-gather bandwidth of this config's cell table (304-B records at a 320-B stride,
+        # worker occupancy): the whole table, a 128 MB window (Infinity Cache sized), a 2.5 MB window (L2 sized)
+        ncell = (grid["nx"] - 1) * (grid["ny"] - 1) * (grid["nz"] - 1)
+        gath = {"table": eng.gather_probe(0) / 1e9, "window_128MB": eng.gather_probe(128 * 2**20 // 320) / 1e9,
+                "window_2.5MB": eng.gather_probe(8192) / 1e9}
+        r = out["roofline"]
+        r["achievable_gather_gbs"] = gath
+        r["table_bytes"] = ncell * 320
+        if r.get("achieved"):
+            r["frac_of_achievable_window_128MB"] = r["achieved"] / gath["window_128MB"]
+        log(f"{name}: gather probe {gath}")
+    totals = lambda os_: {k: sum(getattr(o, k) for o in os_) for k in ("crashed", "returned", "transmitted", "left",
+                                                                       "timeout")}
+    out["outcomes_batched"] = totals(outs)
+    if plain_steps > 0:
+        psteps = pk = 0
+        pouts = None
+        t0 = time.perf_counter()
+        for s in range(plain_steps):
+            pouts = eng.run_batch(jobs, dc, ph, mqhip.PATH_PLAIN)
+            st = eng.stats()
+            psteps += st["ion_steps"]
+            pk += st["kernel_ms"]
+            log(f"{name}: plain FP64 step {s + 1}/{plain_steps}: {st['ion_steps']} ion-steps, kernels "
+                f"{st['kernel_ms']:.1f} ms, {st['launches']} launches")
+        pwall = time.perf_counter() - t0
+        pv = psteps / pwall
+        pl = pulls(outs, pouts)
+        out["plain_fp64"] = {
+            "value": pv, "unit": "ion-steps/s", "ms_per_step": 1e3 * pwall / plain_steps,
+            "gather_demand_gbs": psteps * BYTES_PER_STEP_FP64 / (pk * 1e-3) / 1e9,
+            "outcomes_plain": totals(pouts),
+            "per_job_pulls_within_3sigma": float(np.mean(np.abs(pl) <= 3.0)),
+            "max_abs_pull": float(np.max(np.abs(pl))),
+            "class_total_pulls": class_total_pulls(outs, pouts),
+        }
+        out["speedup_vs_plain_fp64"] = out["value"] / pv
+    eng.close()
+    return out, (grid, jobs, dc, ph)
+
+
+def class_total_pulls(a, b):
+    """Pull of each class total between two outcome lists (binomial over all trajectories)."""
+    n1, n2 = sum(o.n for o in a), sum(o.n for o in b)
+    out = {}
+    for k in ("crashed", "returned", "transmitted", "left", "timeout"):
+        k1, k2 = sum(getattr(o, k) for o in a), sum(getattr(o, k) for o in b)
+        p1, p2 = k1 / max(n1, 1), k2 / max(n2, 1)
+        se = np.sqrt(p1 * (1 - p1) / max(n1, 1) + p2 * (1 - p2) / max(n2, 1))
+        out[k] = 0.0 if se == 0 else float((p1 - p2) / se)
+    return out
 
 
 def main():
@@ -396,6 +511,8 @@ def main():
     ap.add_argument("--skip-tandem", action="store_true")
     ap.add_argument("--skip-refgeom", action="store_true")
     ap.add_argument("--skip-legacy", action="store_true")
+    ap.add_argument("--skip-c3", action="store_true", help="skip the 1M-trajectory sweep sub-object")
+    ap.add_argument("--skip-transmit", action="store_true", help="skip the transmitting mass-scan sub-object")
     ap.add_argument("--legacy-only", action="store_true",
                     help="run only the legacy-potential regeneration sub-bench and print its JSON object")
     ap.add_argument("--grad-only", action="store_true",
@@ -424,89 +541,29 @@ def main():
         del src, dst
         log("copy calibration: 1 GiB read + 1 GiB written by one elementwise kernel")
     log(f"building the {a.config} workload")
-    grid, jobs, dc, ph, desc = workload(a.config)
-    n_traj = int(sum(j.n for j in jobs))
-    eng = mqhip.Engine(grid)
-    log(f"engine ready: {grid['nx']}x{grid['ny']}x{grid['nz']} grid, {len(jobs)} jobs, {n_traj} trajectories")
-    for w in range(a.warmup):
-        eng.run_batch(jobs, dc, ph, mqhip.PATH_BATCHED)
-        log(f"warmup {w + 1}/{a.warmup}: {eng.stats()['ion_steps']} ion-steps")
-    steps = kms = 0
-    launches = 0
-    outs = None
-    t0 = time.perf_counter()
-    for s in range(a.steps):
-        outs = eng.run_batch(jobs, dc, ph, mqhip.PATH_BATCHED)
-        st = eng.stats()
-        steps += st["ion_steps"]
-        kms += st["kernel_ms"]
-        launches += st["launches"]
-        log(f"step {s + 1}/{a.steps}: {st['ion_steps']} ion-steps, worker kernels {st['kernel_ms']:.1f} ms, "
-            f"wall {st['wall_ms']:.1f} ms")
-    wall = time.perf_counter() - t0
-    value = steps / wall
+    head, (grid, jobs, dc, ph) = config_bench(a.config, a.steps, a.warmup, a.plain_steps)
 
     result = {
         "metric": "ion-steps/s, batched FP32 vs plain FP64, matched outcome counts; grad evals/s",
-        "value": value, "unit": "ion-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": 1e3 * wall / max(a.steps, 1), "higher_is_better": True, "scaling": "weak",
+        "value": head["value"], "unit": "ion-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 (FP64 position/time/phase)", "data": "synthetic",
-        "config": {"workload": desc, "trajectories": n_traj, "jobs": len(jobs), "grid": [grid["nx"], grid["ny"],
-                   grid["nz"]], "h_mm": grid["h"] * 1e3, "parallelism": "single GPU"},
+        "config": {"workload": head["workload"], "trajectories": head["trajectories"], "jobs": head["jobs"],
+                   "grid": head["grid"], "h_mm": head["h_mm"], "parallelism": "single GPU"},
+        "roofline": head["roofline"],
+        "outcomes_batched": head["outcomes_batched"],
     }
-    achieved = steps * BYTES_PER_STEP_FP32 / (kms * 1e-3) / 1e9
-    result["roofline"] = {"bound": "hbm", "kernel": "k_batched", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                          "traffic": read_traffic(a.config),
-                          "bytes_per_ion_step": BYTES_PER_STEP_FP32,
-                          "ion_steps_per_launch": steps / max(launches, 1),
-                          "avg_launch_ms": kms / max(launches, 1)}
-
-    # achievable random-gather bandwidth of this config's cell table (304-B records at a 320-B stride,
-    # worker occupancy): the whole table, a 128 MB window (Infinity Cache sized) and a 2.5 MB window
-    # (L2 sized)
-    ncell = (grid["nx"] - 1) * (grid["ny"] - 1) * (grid["nz"] - 1)
-    gath = {"table": eng.gather_probe(0) / 1e9, "window_128MB": eng.gather_probe(128 * 2**20 // 320) / 1e9,
-            "window_2.5MB": eng.gather_probe(8192) / 1e9}
-    traffic = result["roofline"]["traffic"]
-    r = result["roofline"]
-    r["achievable_gather_gbs"] = gath
-    r["table_bytes"] = ncell * 320
-    if traffic:
-        r["traffic_gbs"] = traffic / (r["avg_launch_ms"] * 1e-3) / 1e9
-        r["traffic_frac_of_achievable_table_gather"] = r["traffic_gbs"] / gath["table"]
-        # the same random whole-record reads over a window the Infinity Cache holds: what the
-        # memory system delivers to this access pattern once the re-sort has made it local
-        r["traffic_frac_of_achievable_window_128MB"] = r["traffic_gbs"] / gath["window_128MB"]
-    r["valu_busy"] = read_profile_value(a.config, "valu_busy")
-    log(f"gather probe: {gath}")
-
-    if a.plain_steps > 0:
-        psteps = pk = 0
-        pouts = None
-        t0 = time.perf_counter()
-        for s in range(a.plain_steps):
-            pouts = eng.run_batch(jobs, dc, ph, mqhip.PATH_PLAIN)
-            st = eng.stats()
-            psteps += st["ion_steps"]
-            pk += st["kernel_ms"]
-            log(f"plain FP64 step {s + 1}/{a.plain_steps}: {st['ion_steps']} ion-steps, kernels "
-                f"{st['kernel_ms']:.1f} ms, {st['launches']} launches")
-        pwall = time.perf_counter() - t0
-        pv = psteps / pwall
-        pl = pulls(outs, pouts)
-        tot = lambda os_, k: sum(getattr(o, k) for o in os_)
-        result["plain_fp64"] = {
-            "value": pv, "unit": "ion-steps/s", "ms_per_step": 1e3 * pwall / a.plain_steps,
-            "roofline_achieved_gbs": psteps * BYTES_PER_STEP_FP64 / (pk * 1e-3) / 1e9,
-            "outcomes_batched": {k: tot(outs, k) for k in ("crashed", "returned", "transmitted", "left", "timeout")},
-            "outcomes_plain": {k: tot(pouts, k) for k in ("crashed", "returned", "transmitted", "left", "timeout")},
-            "per_job_pulls_within_3sigma": float(np.mean(np.abs(pl) <= 3.0)),
-            "max_abs_pull": float(np.max(np.abs(pl))),
-        }
-        result["speedup_vs_plain_fp64"] = value / pv
-
-    eng.close()
+    if "plain_fp64" in head:
+        result["plain_fp64"] = head["plain_fp64"]
+        result["speedup_vs_plain_fp64"] = head["speedup_vs_plain_fp64"]
+    if a.config == "c2" and not a.skip_transmit:
+        # the same mass scan with a 0.02 V gate: at 5 V no ion crosses the gate (300 K ions carry ~0.04 eV), so
+        # this variant is where the transmitted class of the FP32 and FP64 paths is compared
+        sub, _ = config_bench("c2", 1, 1, a.plain_steps, gather=False, gate=0.02)
+        result["c2_transmitting"] = sub
+    if a.config != "c3" and not a.skip_c3:
+        sub, _ = config_bench("c3", a.steps, a.warmup, a.plain_steps, gather=True)
+        result["c3"] = sub
     if not a.skip_grad:
         result["grad"] = grad_bench(bool(a.grad_fd))
     if not a.skip_tandem:

@@ -226,13 +226,24 @@ int mq_last_stats(const mq_engine* eng, mq_run_stats* out);
  * records per lane. *bytes_per_s receives the bandwidth. */
 int mq_gather_probe(mq_engine* eng, uint64_t window_cells, uint32_t iters, double* bytes_per_s);
 
+/* The batched path's FP32 field at n positions [m] (no reference counterpart:
+ * it exposes the worker's own field evaluation -- cell location, FP32 cell
+ * record of the combined DC of dc[4], polynomial, drive amplitudes -- so it can
+ * be checked against the FP64 trilinear interpolation of the node tables).
+ * E = a_rf E_rf + s_dc E_dc + a_ac E_ac, trilinear, or the lower-corner node
+ * when nearest = 1; crash[i] = conductor bit of the node nearest position i. */
+int mq_field_probe(mq_engine* eng, const double dc[4], int32_t nearest, double a_rf, double s_dc, double a_ac,
+                   size_t n, const double* x, const double* y, const double* z, float* ex, float* ey, float* ez,
+                   uint8_t* crash);
+
 /* Tuning of the batched path; replaces TileCfg::from_env (tiled.rs:24-28). */
 int mq_set_tune(mq_engine* eng, const mq_tune* t);
 int mq_get_tune(const mq_engine* eng, mq_tune* out);
 
 /* Autotuner of the batched path (the reference tunes by environment
- * variables, tiled.rs:24-28): times candidate tunings (re-sort on/off,
- * chunk, waves per CU, collision batch) on an evenly spaced subset of the jobs of
+ * variables, tiled.rs:24-28): a coordinate search over re-sort on/off, chunk
+ * (re-sort interval), waves per CU, block size, refill (compaction) period,
+ * collision batch and hipGraph replay, timed on an evenly spaced subset of the jobs of
  * about sample_ions ions (0: 262144), sets the fastest on the engine and
  * appends it to the TSV file cache_path (NULL: no file) keyed by grid and
  * physics switches; a later call with the same key reads it back instead of

@@ -844,33 +844,63 @@ static void autotune(mq_engine& e, const mq_job* jobs, size_t njobs, const doubl
                                                                                       (double)sample_ions / (double)total)));
     std::vector<mq_job> sub;
     for (size_t k = 0; k < take; ++k) sub.push_back(jobs[(size_t)((double)k * (double)njobs / (double)take)]);
-    std::vector<mq_tune> cands;
-    for (int resort : {1, 0})
-        for (int chunk : {128, 256, 512})
-            for (int waves : {4, 8})
-                for (int cb : {1, 8}) {
-                    if (!resort && (chunk != 256 || cb != 8)) continue;
-                    mq_tune t = e.tune;
-                    t.coll_batch = cb;
-                    t.resort = resort;
-                    t.chunk = resort ? chunk : 20000;
-                    t.chunk_cap = resort ? chunk : 200000;
-                    t.waves_per_cu = waves;
-                    t.block = 256;
-                    t.refill_period = 8;
-                    t.sync_every = 16;
-                    t.graph = 0;
-                    cands.push_back(t);
-                }
+    // Coordinate search: starting from the defaults (re-sort on, 256-step interval, 8 waves per CU, 256-thread
+    // groups, refill every 8 steps, collision batch 4, no graph), each knob in turn is swept over its values with
+    // the others held at the best so far; two passes. A run whose rate the best beats by less than 1 % keeps the
+    // earlier choice, so noise does not walk the tuning around.
+    mq_tune base = e.tune;
+    base.block = 256; base.waves_per_cu = 8; base.refill_period = 8; base.chunk = 256; base.chunk_cap = 256;
+    base.resort = 1; base.sync_every = 16; base.graph = 0; base.coll_batch = 4;
+    struct Knob {
+        const char* name;
+        std::vector<int> values;
+        void (*set)(mq_tune&, int);
+    };
+    const std::vector<Knob> knobs = {
+        {"resort", {1, 0}, [](mq_tune& t, int v) {
+             t.resort = v;
+             t.chunk = v ? 256 : 20000;
+             t.chunk_cap = v ? 256 : 200000;
+         }},
+        {"chunk", {128, 256, 512}, [](mq_tune& t, int v) {
+             if (t.resort) { t.chunk = v; t.chunk_cap = v; }
+         }},
+        {"waves_per_cu", {4, 8, 12}, [](mq_tune& t, int v) { t.waves_per_cu = v; }},
+        {"block", {64, 128, 256}, [](mq_tune& t, int v) { t.block = v; }},
+        {"refill_period", {4, 8, 16}, [](mq_tune& t, int v) { t.refill_period = v; }},
+        {"coll_batch", {1, 4, 8}, [](mq_tune& t, int v) { t.coll_batch = v; }},
+        {"graph", {0, 1}, [](mq_tune& t, int v) { t.graph = v; }},
+    };
     const mq_tune saved = e.tune;
     std::vector<unsigned long long> acc;
     size_t n = 0;
-    best = saved;
-    for (const mq_tune& t : cands) {
+    std::vector<std::pair<mq_tune, double>> measured;
+    auto same = [](const mq_tune& a, const mq_tune& b) { return std::memcmp(&a, &b, sizeof a) == 0; };
+    auto rate_of = [&](const mq_tune& t) {
+        for (auto& m : measured)
+            if (same(m.first, t)) return m.second;
         e.tune = t;
         e.run(sub.data(), sub.size(), dc, ph, MQ_PATH_BATCHED, acc, n);
-        const double rate = (double)e.stats.ion_steps / (e.stats.wall_ms * 1e-3);
-        if (rate > best_rate) { best_rate = rate; best = t; }
+        const double r = (double)e.stats.ion_steps / (e.stats.wall_ms * 1e-3);
+        measured.push_back({t, r});
+        return r;
+    };
+    try {
+        best = base;
+        best_rate = rate_of(base);
+        for (int pass = 0; pass < 2; ++pass) {
+            for (const Knob& k : knobs) {
+                for (int v : k.values) {
+                    mq_tune t = best;
+                    k.set(t, v);
+                    const double r = rate_of(t);
+                    if (r > 1.01 * best_rate) { best_rate = r; best = t; }
+                }
+            }
+        }
+    } catch (...) {
+        e.tune = saved;
+        throw;
     }
     e.tune = best;
     if (cache_path) {
@@ -1069,6 +1099,46 @@ int mq_gather_probe(mq_engine* eng, uint64_t window_cells, uint32_t iters, doubl
         HIPCHK(hipEventElapsedTime(&ms, eng->ev0, eng->ev1));
         out.release();
         *bytes_per_s = (double)grid * block * iters * CELL_F4 * 16.0 / (ms * 1e-3);
+    })
+}
+
+int mq_field_probe(mq_engine* eng, const double dc[4], int32_t nearest, double a_rf, double s_dc, double a_ac,
+                   size_t n, const double* x, const double* y, const double* z, float* ex, float* ey, float* ez,
+                   uint8_t* crash)
+{
+    MQ_GUARD({
+        if (!eng || !dc || (n && (!x || !y || !z || !ex || !ey || !ez || !crash))) throw Err(-1, "NULL argument");
+        if (n >= (1ull << 31)) throw Err(-1, "too many probe points");
+        if (n == 0) return 0;
+        eng->prepare_dc(dc, true);
+        mq_physics ph{};
+        ph.dt = 1.0;
+        ph.legacy_fields = nearest ? 1 : 0;
+        const BatchP B = eng->batch_params(ph, 1);
+        DevBuf pos, fout, cout;
+        pos.ensure(3 * n * sizeof(double));
+        fout.ensure(3 * n * sizeof(float));
+        cout.ensure(n);
+        double* p = pos.as<double>();
+        float* f = fout.as<float>();
+        hipStream_t st = eng->stream;
+        try {
+            HIPCHK(hipMemcpyAsync(p, x, n * 8, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(p + n, y, n * 8, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(p + 2 * n, z, n * 8, hipMemcpyHostToDevice, st));
+            launch_field_probe(eng->cells.as<float4>(), B, (uint32_t)n, p, p + n, p + 2 * n, (float)a_rf, (float)s_dc,
+                               (float)a_ac, f, f + n, f + 2 * n, cout.as<uint8_t>(), st);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(ex, f, n * 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(ey, f + n, n * 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(ez, f + 2 * n, n * 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(crash, cout.p, n, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+        } catch (...) {
+            pos.release(); fout.release(); cout.release();
+            throw;
+        }
+        pos.release(); fout.release(); cout.release();
     })
 }
 

@@ -64,6 +64,9 @@ void launch_prepare(const WorkerArgs& a, hipStream_t st);
 void launch_gather_probe(const float4* cells, uint64_t window, uint32_t iters, float* out, uint32_t grid,
                          uint32_t block, hipStream_t st);
 void launch_batched(const WorkerArgs& a, uint32_t grid, uint32_t block, hipStream_t st);
+void launch_field_probe(const float4* cells, const BatchP& B, uint32_t n, const double* x, const double* y,
+                        const double* z, float a_rf, float s_dc, float a_ac, float* ex, float* ey, float* ez,
+                        uint8_t* crash, hipStream_t st);
 // Differentiable form (kernels_adjoint.hip).
 // The node tables of the differentiable form (double), and per-cell float
 // records of the corner values when every value of all three tables is exactly

@@ -136,6 +136,19 @@ __device__ __forceinline__ int32_t locate(const BatchP& B, Lane& L)
     return i + (B.nx - 1) * (j + (B.ny - 1) * k);
 }
 
+// Field of the record R at the lane's (tx, ty, tz): a_rf E_rf + s_dc E_dc + a_ac E_ac.
+__device__ __forceinline__ void cell_field(const float4 R[CELL_F4], const Lane& L, float e[3])
+{
+    const float pxy = L.tx * L.ty, pyz = L.ty * L.tz, pxz = L.tx * L.tz, pxyz = pxy * L.tz;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float er = poly(R, d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+        const float ed = poly(R, 24 + d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+        const float ea = poly(R, 48 + d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+        e[d] = fmaf(L.a_ac, ea, fmaf(L.sdc, ed, L.a_rf * er));
+    }
+}
+
 __device__ __forceinline__ void phases(Lane& L)
 {
     double pr = L.w_rf * (double)L.st;
@@ -240,15 +253,8 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
         int status = 0;
         for (uint32_t r = 0; r < A.period; ++r) {
             if (have && !stop && !pending) {
-                const float pxy = L.tx * L.ty, pyz = L.ty * L.tz, pxz = L.tx * L.tz, pxyz = pxy * L.tz;
                 float e[3];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    const float er = poly(R, d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
-                    const float ed = poly(R, 24 + d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
-                    const float ea = poly(R, 48 + d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
-                    e[d] = fmaf(L.a_ac, ea, fmaf(L.sdc, ed, L.a_rf * er));
-                }
+                cell_field(R, L, e);
                 L.vx = fmaf(L.kq, e[0], L.vx);
                 L.vy = fmaf(L.kq, e[1], L.vy);
                 L.vz = fmaf(L.kq, e[2], L.vz);
@@ -410,6 +416,37 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
             have = false;
         }
     }
+}
+
+// ------------------------------------------------- field probe
+// The worker's field at given positions: the same cell location, record load
+// and polynomial, with fixed drive amplitudes; crash = the conductor bit of the
+// nearest corner node (the worker's crash test without its z window).
+__global__ __launch_bounds__(256) void k_field_probe(const float4* __restrict__ cells, BatchP B, uint32_t n,
+                                                     const double* __restrict__ px, const double* __restrict__ py,
+                                                     const double* __restrict__ pz, float a_rf, float s_dc,
+                                                     float a_ac, float* __restrict__ ex, float* __restrict__ ey,
+                                                     float* __restrict__ ez, uint8_t* __restrict__ crash)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Lane L;
+    L.x = px[i]; L.y = py[i]; L.z = pz[i];
+    L.a_rf = a_rf; L.sdc = s_dc; L.a_ac = a_ac;
+    const int32_t c = locate(B, L);
+    float4 R[CELL_F4];
+    load_cell(cells, c, R);
+    float e[3];
+    cell_field(R, L, e);
+    ex[i] = e[0]; ey[i] = e[1]; ez[i] = e[2];
+    crash[i] = (uint8_t)((__float_as_uint(R[18].x) >> L.corner) & 1u);
+}
+
+void launch_field_probe(const float4* cells, const BatchP& B, uint32_t n, const double* x, const double* y,
+                        const double* z, float a_rf, float s_dc, float a_ac, float* ex, float* ey, float* ez,
+                        uint8_t* crash, hipStream_t st)
+{
+    if (n) k_field_probe<<<(n + 255) / 256, 256, 0, st>>>(cells, B, n, x, y, z, a_rf, s_dc, a_ac, ex, ey, ez, crash);
 }
 
 // ------------------------------------------------- gather probe

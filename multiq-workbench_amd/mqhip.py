@@ -194,6 +194,34 @@ class Objective(C.Structure):
                 ("r_leak", C.c_double), ("w_leak", C.c_double)]
 
 
+class PeakFit(C.Structure):
+    _fields_ = [("f0_khz", C.c_double), ("f0_err", C.c_double), ("fwhm_khz", C.c_double), ("fwhm_err", C.c_double),
+                ("amp", C.c_double), ("base", C.c_double), ("r_f", C.c_double), ("r_f_err", C.c_double),
+                ("chi2_red", C.c_double), ("converged", C.c_int32), ("npts", C.c_int32)]
+
+
+class OptimizeOpts(C.Structure):
+    _fields_ = [("f_lo_khz", C.c_double), ("f_hi_khz", C.c_double), ("f_step_khz", C.c_double), ("n", C.c_uint64),
+                ("min_r", C.c_double), ("dlnf_dlnmz", C.c_double), ("seed", C.c_uint64)]
+
+
+class OptimizeRow(C.Structure):
+    _fields_ = [("v_ac", C.c_double), ("lo_khz", C.c_double), ("hi_khz", C.c_double), ("step_khz", C.c_double),
+                ("ions_per_point", C.c_uint64), ("fit", PeakFit), ("r_mz", C.c_double), ("r_mz_err", C.c_double),
+                ("f_best_khz", C.c_double), ("t_best", C.c_double), ("t_best_err", C.c_double),
+                ("refined", C.c_int32), ("issue", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "fit"}
+        d["fit"] = {k: getattr(self.fit, k) for k, _ in PeakFit._fields_}
+        return d
+
+
+class GradOpts(C.Structure):
+    _fields_ = [("mask", C.c_uint32), ("max_iter", C.c_uint32), ("lo", C.c_double * 5), ("hi", C.c_double * 5),
+                ("step0", C.c_double * 5), ("tol", C.c_double)]
+
+
 class ScanPoint(C.Structure):
     _fields_ = [("f_khz", C.c_double), ("transmitted", C.c_double), ("crashed", C.c_double), ("n", C.c_uint64)]
 
@@ -235,6 +263,19 @@ SIGNATURES = {
     "mq_run_batch_ions_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_size_t, C.POINTER(C.c_double),
                                             C.POINTER(Physics), C.c_int32, C.POINTER(IonSoA),
                                             C.POINTER(C.c_uint32)]),
+    "mq_field_probe": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.c_double, C.c_double, C.c_double,
+                                 C.c_size_t, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                 C.POINTER(C.c_uint8)]),
+    "mq_fit_peak": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_size_t,
+                              C.POINTER(PeakFit)]),
+    "mq_optimize_grid": (C.c_int, [C.c_void_p, C.POINTER(Species), C.POINTER(RfAc), C.POINTER(C.c_double),
+                                   C.POINTER(Physics), C.c_int32, C.POINTER(C.c_double), C.c_size_t,
+                                   C.POINTER(OptimizeOpts), C.POINTER(OptimizeRow), C.POINTER(C.c_int32),
+                                   C.POINTER(OptimizeRow), C.POINTER(C.c_int32)]),
+    "mq_optimize_grad": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_double), C.POINTER(Physics),
+                                   C.POINTER(Objective), C.POINTER(GradOpts), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
     "mq_summarize": (C.c_int, [C.POINTER(C.c_uint8), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                C.POINTER(C.c_uint32), C.c_size_t, C.c_size_t, C.c_double, C.POINTER(Outcome)]),
     "mq_last_stats": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
@@ -427,6 +468,18 @@ class Engine:
         _check(load().mq_last_stats(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in s._fields_}
 
+    def field_probe(self, dc, x, y, z, a_rf=1.0, s_dc=1.0, a_ac=1.0, nearest=0):
+        """The batched path's FP32 field at positions [m] (mq_field_probe): (E (n, 3) float32, crash bits)."""
+        x, y, z = (np.ascontiguousarray(v, np.float64) for v in (x, y, z))
+        n = len(x)
+        e = np.zeros((3, n), np.float32)
+        cr = np.zeros(n, np.uint8)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        _check(load().mq_field_probe(self._h, (C.c_double * 4)(*dc), int(nearest), a_rf, s_dc, a_ac, n, _dptr(x),
+                                     _dptr(y), _dptr(z), fp(e[0]), fp(e[1]), fp(e[2]),
+                                     cr.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return e.T.copy(), cr
+
     def gather_probe(self, window_cells: int = 0, iters: int = 64) -> float:
         """Achievable random-gather bandwidth [B/s] of the cell table (whole table, or its first window_cells)."""
         b = C.c_double()
@@ -484,6 +537,47 @@ class Engine:
         t = Tune()
         _check(load().mq_get_tune(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
+
+
+def fit_peak(f_khz, t, n) -> dict:
+    """Weighted Gaussian-plus-baseline fit of a transmission curve (mq_fit_peak)."""
+    f = np.ascontiguousarray(f_khz, np.float64)
+    tt = np.ascontiguousarray(t, np.float64)
+    nn = np.ascontiguousarray(np.broadcast_to(np.asarray(n, np.uint64), f.shape), np.uint64)
+    out = PeakFit()
+    _check(load().mq_fit_peak(_dptr(f), _dptr(tt), nn.ctypes.data_as(C.POINTER(C.c_uint64)), len(f), C.byref(out)))
+    return {k: getattr(out, k) for k, _ in PeakFit._fields_}
+
+
+def optimize_grid(eng: "Engine", sp: Species, rf: RfAc, dc, ph: Physics, v_ac, f_lo_khz: float, f_hi_khz: float,
+                  f_step_khz: float, n: int, min_r: float, seed: int, dlnf_dlnmz: float = 0.0,
+                  path: int = PATH_BATCHED) -> dict:
+    """The Optimize grid search (research.rs:133-163) on the engine (mq_optimize_grid)."""
+    va = np.ascontiguousarray(v_ac, np.float64)
+    o = OptimizeOpts(f_lo_khz, f_hi_khz, f_step_khz, n, min_r, dlnf_dlnmz, seed)
+    rows = (OptimizeRow * len(va))()
+    conf = OptimizeRow()
+    best, confirmed = C.c_int32(), C.c_int32()
+    _check(load().mq_optimize_grid(eng._h, C.byref(sp), C.byref(rf), (C.c_double * 4)(*dc), C.byref(ph), path,
+                                   _dptr(va), len(va), C.byref(o), rows, C.byref(best), C.byref(conf),
+                                   C.byref(confirmed)))
+    return {"rows": [r.as_dict() for r in rows], "best": best.value,
+            "confirmation": conf.as_dict() if best.value >= 0 else None, "confirmed": bool(confirmed.value)}
+
+
+def optimize_grad(eng: "Engine", j: Job, dc, ph: Physics, obj: Objective, mask: int, lo, hi, step0,
+                  max_iter: int = 40, tol: float = 1e-3) -> dict:
+    """Projected gradient ascent of the differentiable objective (mq_optimize_grad)."""
+    jj = Job()
+    C.pointer(jj)[0] = j
+    o = GradOpts(mask, max_iter, (C.c_double * 5)(*lo), (C.c_double * 5)(*hi), (C.c_double * 5)(*step0), tol)
+    J = C.c_double()
+    ne = C.c_uint32()
+    hist = np.zeros(6 * max_iter)
+    _check(load().mq_optimize_grad(eng._h, C.byref(jj), (C.c_double * 4)(*dc), C.byref(ph), C.byref(obj),
+                                   C.byref(o), C.byref(J), C.byref(ne), _dptr(hist)))
+    theta = [jj.drive.v_rf, jj.drive.f_rf, jj.drive.v_ac, jj.drive.f_ac, jj.dc_scale]
+    return {"job": jj, "theta": theta, "J": J.value, "n_eval": ne.value, "history": hist[: 6 * ne.value].reshape(-1, 6)}
 
 
 def inject_reference(grid: dict, j: Job, ph: Physics) -> dict:

@@ -283,7 +283,19 @@ def test_autotune_persists_and_keeps_results(c1, tmp_path):
         again, rate2 = e.autotune(jobs, [0, 5, 0, 0], ph, cache_path=cache)
         assert again == best and rate2 == 0.0
         tuned = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+        # an entry tuned elsewhere with a smaller block, another refill (compaction) period and graph replay:
+        # the last entry for a key wins, and the results stay the same
+        line = open(cache).read().splitlines()[0]
+        key = line.split("\t")[0]
+        with open(cache, "a") as fh:
+            fh.write(f"{key}\t128,4,16,128,128,1,8,1,2\t0\n")
+        other, rate3 = e.autotune(jobs, [0, 5, 0, 0], ph, cache_path=cache)
+        assert [other[k] for k in ("block", "refill_period", "waves_per_cu", "graph", "coll_batch")] == [128, 16, 4, 1, 2]
+        assert e.get_tune() == other and rate3 == 0.0
+        tuned2 = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
     finally:
         e.set_tune(**t0)
+    say(f"[gpu]   autotuned: {best}")
     for k in KEYS + ("steps", "ncoll", "status"):
         assert np.array_equal(ref[k], tuned[k]), k
+        assert np.array_equal(ref[k], tuned2[k]), k
