@@ -49,7 +49,9 @@ def test_grid_search_on_the_s14_channel():
         b, c = res["rows"][res["best"]], res["confirmation"]
         say(f"[gpu]   best row {res['best']}; confirmation f0 {c['fit']['f0_khz']:.2f} kHz, T {c['t_best']:.3f}, "
             f"confirmed {res['confirmed']}")
-        assert c["ions_per_point"] == 3 * b["ions_per_point"] and c["v_ac"] == b["v_ac"]
+        # the confirmation rescans the chosen row's frequencies with three times the base ions (research.rs:155)
+        assert c["ions_per_point"] == 3 * 500 and c["v_ac"] == b["v_ac"]
+        assert (c["lo_khz"], c["step_khz"]) == (b["lo_khz"], b["step_khz"])
         assert res["confirmed"] == eligible(c, min_r)
         if c["fit"]["converged"]:
             se = np.hypot(b["fit"]["f0_err"], c["fit"]["f0_err"])
