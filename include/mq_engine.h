@@ -136,7 +136,7 @@ typedef struct {
 
 /* Tuning of the batched path (TileCfg, tiled.rs:12-29). mq_set_tune: zero
  * in block, waves_per_cu, refill_period, chunk, chunk_cap, sync_every and
- * coll_batch keeps the current value; resort and graph are always taken.
+ * coll_batch keeps the current value; resort, graph and tail_lanes are always taken.
  * Results per ion do not depend on any of these. */
 typedef struct {
     int32_t block;           /* threads per workgroup (64..256, multiple of 64) */
@@ -151,6 +151,8 @@ typedef struct {
                                 (sync_every must be even) */
     int32_t coll_batch;      /* lanes of a wavefront whose collision is pending before the
                                 collision code runs for all of them (1: at once) */
+    int32_t tail_lanes;      /* once the live ions fit the launch's lanes: ions per wavefront
+                                at most (1..64; 0: spread them evenly over every wavefront) */
 } mq_tune;
 
 typedef struct mq_engine mq_engine;

@@ -126,7 +126,7 @@ struct mq_engine {
     // per-call pools
     DevBuf jobs, acc, list[3], ctr, bin_of, rank_of, hist, bsum;
     DevBuf sx, sy, sz, sphi, svx, svy, svz, sfx, sfy, sfz, ssteps, sncoll, scd, sjob, sstatus, srng;
-    mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0, 4};
+    mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0, 4, 0};
     // no-alloc entry point: this device's scratch array and host staging
     // reserved at creation for MQ_NOALLOC_MAX_JOBS jobs
     void* scratch_dev = nullptr;
@@ -602,7 +602,8 @@ struct mq_engine {
     // one captured block must serve every replay).
     void enqueue_block(const CallBufs& cb, const BatchP& B, uint32_t n, const SortGeom& G, int& cur, uint32_t& chunk,
                        uint32_t cap, uint32_t period, uint32_t grid, uint32_t block, int iters, bool timed,
-                       std::vector<hipEvent_t>* evs, bool resort, bool& bins_known, bool pass_bins)
+                       std::vector<hipEvent_t>* evs, bool resort, bool& bins_known, bool pass_bins,
+                       uint32_t lanes = 64)
     {
         uint32_t* c0 = cb.ctr;
         uint32_t* c1 = c0 + CTR_BLOCK;
@@ -635,6 +636,7 @@ struct mq_engine {
             a.coll_batch = (uint32_t)std::max(1, tune.coll_batch);
             a.bin_out = resort && pass_bins ? cb.bin_of : nullptr;
             a.sg = G;
+            a.lanes = lanes;
             bins_known = a.bin_out != nullptr;
             launch_prepare(a, stream);
             if (timed) HIPCHK(hipEventRecord((*evs)[2 * it], stream));
@@ -754,11 +756,22 @@ struct mq_engine {
         // 200000 (tiled.rs:26-27, 214), so the tail takes a few launches, not one per
         // re-sort interval.
         const uint64_t lanes = (uint64_t)grid * block;
+        const uint64_t waves = lanes / 64;
         bool tail = false;
         bool bins_known = false;  // list[0] is the injected list: no bins yet
+        uint32_t alive_last = n;
         for (;;) {
+            // In the tail the few live ions are spread over every wavefront (tail_lanes ions per wave at most,
+            // or as few as fill all waves when tail_lanes is 0), so that a step's record reloads are not
+            // serialised over 64 lanes of divergent addresses.
+            uint32_t act = 64;
+            if (tail) {
+                act = tune.tail_lanes > 0 ? (uint32_t)tune.tail_lanes
+                                          : (uint32_t)std::min<uint64_t>(64, (alive_last + waves - 1) / waves);
+                act = std::max(1u, std::min(64u, act));
+            }
             enqueue_block(cb, B, n, G, cur, chunk, cap, period, grid, block, iters, timed, &evs, resort && !tail,
-                          bins_known, true);
+                          bins_known, true, act);
             HIPCHK(hipGetLastError());
             stats.launches += (uint32_t)iters;
             uint32_t alive = 0;
@@ -773,6 +786,7 @@ struct mq_engine {
                 }
             }
             if (alive == 0) break;
+            alive_last = alive;
             if (resort && !tail && alive <= lanes) {
                 tail = true;
                 cap = std::max(cap, 200000u);
@@ -810,9 +824,10 @@ bool read_tune_cache(const char* path, const std::string& key, mq_tune& t)
         const size_t tab = l.find('\t');
         if (tab == std::string::npos || l.substr(0, tab) != key) continue;
         mq_tune x{};
-        if (std::sscanf(l.c_str() + tab + 1, "%d,%d,%d,%d,%d,%d,%d,%d,%d", &x.block, &x.waves_per_cu,
+        x.tail_lanes = 64;  // entries written before the tail_lanes field
+        if (std::sscanf(l.c_str() + tab + 1, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", &x.block, &x.waves_per_cu,
                         &x.refill_period, &x.chunk, &x.chunk_cap, &x.resort, &x.sync_every, &x.graph,
-                        &x.coll_batch) == 9) {
+                        &x.coll_batch, &x.tail_lanes) >= 9) {
             t = x;  // the last entry for a key wins
             found = true;
         }
@@ -850,7 +865,7 @@ static void autotune(mq_engine& e, const mq_job* jobs, size_t njobs, const doubl
     // earlier choice, so noise does not walk the tuning around.
     mq_tune base = e.tune;
     base.block = 256; base.waves_per_cu = 8; base.refill_period = 8; base.chunk = 256; base.chunk_cap = 256;
-    base.resort = 1; base.sync_every = 16; base.graph = 0; base.coll_batch = 4;
+    base.resort = 1; base.sync_every = 16; base.graph = 0; base.coll_batch = 4; base.tail_lanes = 0;
     struct Knob {
         const char* name;
         std::vector<int> values;
@@ -870,6 +885,7 @@ static void autotune(mq_engine& e, const mq_job* jobs, size_t njobs, const doubl
         {"refill_period", {4, 8, 16}, [](mq_tune& t, int v) { t.refill_period = v; }},
         {"coll_batch", {1, 4, 8}, [](mq_tune& t, int v) { t.coll_batch = v; }},
         {"graph", {0, 1}, [](mq_tune& t, int v) { t.graph = v; }},
+        {"tail_lanes", {0, 64}, [](mq_tune& t, int v) { t.tail_lanes = v; }},
     };
     const mq_tune saved = e.tune;
     std::vector<unsigned long long> acc;
@@ -906,9 +922,9 @@ static void autotune(mq_engine& e, const mq_job* jobs, size_t njobs, const doubl
     if (cache_path) {
         FILE* f = std::fopen(cache_path, "a");
         if (f) {
-            std::fprintf(f, "%s\t%d,%d,%d,%d,%d,%d,%d,%d,%d\t%.6g\n", key.c_str(), best.block, best.waves_per_cu,
-                         best.refill_period, best.chunk, best.chunk_cap, best.resort, best.sync_every, best.graph,
-                         best.coll_batch, best_rate);
+            std::fprintf(f, "%s\t%d,%d,%d,%d,%d,%d,%d,%d,%d,%d\t%.6g\n", key.c_str(), best.block,
+                         best.waves_per_cu, best.refill_period, best.chunk, best.chunk_cap, best.resort,
+                         best.sync_every, best.graph, best.coll_batch, best.tail_lanes, best_rate);
             std::fclose(f);
         }
     }
@@ -1176,6 +1192,8 @@ int mq_set_tune(mq_engine* eng, const mq_tune* t)
             if (t->coll_batch < 1 || t->coll_batch > 64) throw Err(-1, "coll_batch must be in [1, 64]");
             n.coll_batch = t->coll_batch;
         }
+        if (t->tail_lanes < 0 || t->tail_lanes > 64) throw Err(-1, "tail_lanes must be in [0, 64]");
+        n.tail_lanes = t->tail_lanes;
         n.graph = t->graph;
         if (n.graph && (n.sync_every % 2)) throw Err(-1, "graph mode needs an even sync_every");
         eng->tune = n;

@@ -47,6 +47,7 @@ struct WorkerArgs {
     uint32_t coll_batch;  // pending collisions per wavefront that trigger the collision code
     uint32_t* bin_out;    // optional: re-sort bin of each carried ion, at its list_out position
     SortGeom sg;          // bins of bin_out
+    uint32_t lanes;       // lanes per wavefront that take ions (1..64; fewer spread a small tail over more waves)
 };
 
 // bins_known: bin_of already holds the bin of each list entry (written by the

@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
         // several queues the list is cut into contiguous (spatially sorted) segments
         // and the wavefronts of one XCD start on their own segment, then steal.
         while (!drained) {
-            const uint64_t idle = __ballot(!have);
+            const uint64_t idle = __ballot(!have && lane < A.lanes);
             if (!idle) break;
             const uint32_t nidle = (uint32_t)__popcll(idle);
             const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                 if (__any(have)) break;
                 continue;
             }
-            if (!have) {
+            if ((idle >> lane) & 1ull) {  // exactly the lanes counted in the claim
                 const uint32_t slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (slot < avail) {
                     const uint32_t id = A.list_in[s0 + slot];

@@ -352,7 +352,7 @@ __global__ __launch_bounds__(256) void k_tandem_batched(TBatchArgs A)
                 if (__any(have)) break;
                 continue;
             }
-            if (!have) {
+            if ((idle >> lane) & 1ull) {  // exactly the lanes counted in the claim
                 const uint32_t slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (slot < avail) {
                     const uint32_t id = A.list_in[s0 + slot];

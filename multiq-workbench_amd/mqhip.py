@@ -186,7 +186,7 @@ class RunStats(C.Structure):
 class Tune(C.Structure):
     _fields_ = [("block", C.c_int32), ("waves_per_cu", C.c_int32), ("refill_period", C.c_int32),
                 ("chunk", C.c_int32), ("chunk_cap", C.c_int32), ("resort", C.c_int32), ("sync_every", C.c_int32),
-                ("graph", C.c_int32), ("coll_batch", C.c_int32)]
+                ("graph", C.c_int32), ("coll_batch", C.c_int32), ("tail_lanes", C.c_int32)]
 
 
 class Objective(C.Structure):

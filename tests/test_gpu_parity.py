@@ -182,14 +182,22 @@ def test_batched_tail_switch_keeps_results(c1):
         e.set_tune(block=64, waves_per_cu=3, refill_period=8, chunk=64, chunk_cap=64, resort=1, sync_every=1, graph=0)
         sorted_run = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
         n_sorted = e.stats()["launches"]
+        # the tail spread over every wavefront (tail_lanes 0) and at 3 ions per wavefront
+        spread = []
+        for tl in (0, 3):
+            e.set_tune(block=64, waves_per_cu=3, refill_period=8, chunk=64, chunk_cap=64, resort=1, sync_every=1,
+                       graph=0, tail_lanes=tl)
+            spread.append(e.run_batch_ions(jobs, [0, 5, 0, 0], ph))
         e.set_tune(block=256, waves_per_cu=8, refill_period=8, chunk=20000, chunk_cap=200000, resort=0,
-                   sync_every=2, graph=0)
+                   sync_every=2, graph=0, tail_lanes=64)
         plain_launches = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
     finally:
         e.set_tune(**t0)
     say(f"[gpu]   {n_sorted} launches with re-sort and tail")
     for k in KEYS + ("steps", "ncoll", "status"):
         assert np.array_equal(sorted_run[k], plain_launches[k]), k
+        for r in spread:
+            assert np.array_equal(sorted_run[k], r[k]), k
     assert sorted_run["steps"].max() > 64 * 8   # the run outlasted several re-sort intervals
 
 
