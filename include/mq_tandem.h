@@ -10,18 +10,21 @@
  *   - each port samples one shared channel grid with its own excitation tone;
  *     a port whose tone is 0 is not driven and gets a blocking DC increment
  *     (lib.rs:196-203);
- *   - an ion that crosses the port board (z > seam) inside a mouth aperture
- *     enters that port (lib.rs:222-235, enter 326-356); outside every
- *     aperture it is lost on the board;
+ *   - in the cube an ion on a conductor node is lost (classify, lib.rs:208-212);
+ *   - above the seam, an ion inside the aperture of the nearest mouth enters
+ *     that port (lib.rs:213-225, enter 326-356); outside it, the ion keeps
+ *     moving and is lost only once it passes the top of the cube (lib.rs:226-229);
+ *   - past the cube's y = (ny - 1) h face an ion outside the entrance aperture
+ *     is lost (lib.rs:231-236); one through it that gets l_ext beyond the face
+ *     is back (from the splitter) or detected (from the combiner)
+ *     (lib.rs:237-243); otherwise an ion past any other face is lost;
  *   - an ion that leaves a port through its return plane re-enters the
  *     splitter with its state intact, and one that leaves through the exit
- *     plane passes to the combiner (leave, lib.rs:357-377);
- *   - the combiner ends at the detector plane; the splitter's entrance plane
- *     sends ions back (lib.rs:249-256).
+ *     plane passes to the combiner (leave, lib.rs:357-377).
  * Differences from trapsim: the cube is a grid the caller supplies (trapsim
- * builds its 486-post cube in geometry.rs / fields.rs); the port board is the
- * seam plane itself; both regions share one bath with chsim's thermal
- * collisions and Philox streams (mq_engine.h).
+ * builds its 486-post cube in geometry.rs / fields.rs; its posts and board
+ * are conductor nodes of the caller's mask here); both regions share one
+ * bath with chsim's thermal collisions and Philox streams (mq_engine.h).
  *
  * Conventions as mq_engine.h: 0 = success, < 0 = error (mq_last_error()).
  */
@@ -56,9 +59,10 @@ enum { MQ_T_ALIVE = 0, MQ_T_LOST = 1, MQ_T_BACK = 2, MQ_T_DETECTED = 3, MQ_T_TIM
 typedef struct {
     double seam_z;                   /* z of the port board (seam, lib.rs:126) */
     double mouth[MQ_TANDEM_PORTS][2];/* mouth centres (x, y) (port_axis_xy, lib.rs:107-110) */
-    double ap_r;                     /* mouth aperture radius (geom.ap_r) */
-    double z_back;                   /* entrance / detector plane: below it an ion is back (splitter)
-                                        or detected (combiner) */
+    double ap_r;                     /* aperture radius of the mouths and of the entrance (geom.ap_r) */
+    double entrance[2];              /* (x, z) of the entrance aperture's centre on the cube's
+                                        y = (ny - 1) h face (port_axis_xy(entrance), lib.rs:232) */
+    double l_ext;                    /* entrance tube length (geom.l_ext, lib.rs:237) */
 } mq_tandem_geom;
 
 /* Drive of one tandem run (Model::advance, lib.rs:176-183): one RF on every

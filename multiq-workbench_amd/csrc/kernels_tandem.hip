@@ -35,8 +35,8 @@ __device__ __forceinline__ double sel9(const double (&a)[9], uint32_t k)
 }
 
 // Classification and region crossing after a step (lib.rs:206-265 with
-// enter 326-356 and leave 357-377). `crash` tests the conductor mask of a
-// grid at a position inside that grid's box. Returns the new status
+// enter 326-356 and leave 357-377). `crash` tests the conductor mask at the
+// nearest node of the ion's grid (clamped to the grid). Returns the new status
 // (0 alive, 1 lost, 2 back, 3 detected).
 // mxp, myp: the mouth centre of the ion's port when it is in a port (T.mx[p],
 // T.my[p]; the caller keeps them per region), ignored in the cube.
@@ -46,8 +46,7 @@ __device__ __forceinline__ int tandem_cross(const TandemP& T, const TGrid& cube,
                                             unsigned long long* __restrict__ ctr, Crash crash, double mxp, double myp)
 {
     if (reg == REG_SPLIT || reg == REG_COMBINE) {
-        if (x < 0.0 || y < 0.0 || z < 0.0 || x > cube.lim0 || y > cube.lim1 || z > cube.lim2 || crash(false))
-            return 1;
+        if (crash(false)) return 1;  // a conductor node of the cube (classify, lib.rs:208-212)
         if (z > T.seam) {
             uint32_t p = 0;
             double best = 0.0, mx = 0.0, my = 0.0;
@@ -57,27 +56,37 @@ __device__ __forceinline__ int tandem_cross(const TandemP& T, const TGrid& cube,
                 const double d2 = dx * dx + dy * dy;
                 if (k == 0 || d2 < best) { best = d2; p = k; mx = T.mx[k]; my = T.my[k]; }
             }
-            if (!(best < T.ap_r2)) return 1;  // on the board between the mouths
-            unsigned long long* c = ctr + (size_t)p * TC_PORT;
-            x = x + (T.ax_p - mx);
-            y = y + (T.ay_p - my);
-            if (reg == REG_SPLIT) {
-                z = T.z_ret + (z - T.seam);
-                atomicAdd(&c[TC_ENTERED], 1ull);
-                if (last != NO_PORT) {
-                    atomicAdd(&c[TC_REVISITS], 1ull);
-                    atomicAdd(&c[TC_RETRY + last], 1ull);
-                    if (last != p) atomicAdd(&c[TC_FROM_OTHER], 1ull);
+            if (best < T.ap_r2) {
+                unsigned long long* c = ctr + (size_t)p * TC_PORT;
+                x = x + (T.ax_p - mx);
+                y = y + (T.ay_p - my);
+                if (reg == REG_SPLIT) {
+                    z = T.z_ret + (z - T.seam);
+                    atomicAdd(&c[TC_ENTERED], 1ull);
+                    if (last != NO_PORT) {
+                        atomicAdd(&c[TC_REVISITS], 1ull);
+                        atomicAdd(&c[TC_RETRY + last], 1ull);
+                        if (last != p) atomicAdd(&c[TC_FROM_OTHER], 1ull);
+                    }
+                } else {
+                    z = T.z_exit - (z - T.seam);
+                    vz = -vz;
+                    atomicAdd(&c[TC_REVERSE], 1ull);
                 }
-            } else {
-                z = T.z_exit - (z - T.seam);
-                vz = -vz;
-                atomicAdd(&c[TC_REVERSE], 1ull);
+                reg = 1u + p;
+                return 0;
             }
-            reg = 1u + p;
-            return 0;
+            // outside every mouth the ion keeps moving above the board, up to the cube's top (lib.rs:226-229)
+            if (z > cube.lim2) return 1;
         }
-        if (z < T.z_back) return reg == REG_SPLIT ? 2 : 3;
+        if (y > cube.lim1) {
+            // the entrance face: outside the aperture a wall (lib.rs:231-236); through it, the tube, and
+            // l_ext beyond the face the ion is back or detected (lib.rs:237-243)
+            const double ex = x - T.ent_x, ez = z - T.ent_z;
+            if (!(ex * ex + ez * ez < T.ap_r2)) return 1;
+            if (y > cube.lim1 + T.l_ext) return reg == REG_SPLIT ? 2 : 3;
+        }
+        if (x < 0.0 || y < 0.0 || z < 0.0 || x > cube.lim0) return 1;
         return 0;
     }
     const uint32_t p = reg - 1u;
@@ -197,11 +206,14 @@ __global__ __launch_bounds__(256) void k_tandem_plain(TPlainArgs A)
                     }
                 }
             }
+            // conductor bit of the nearest node; the index is clamped to the grid, since a cube ion may be
+            // past its y face inside the entrance tube
             auto crash = [&](bool in_port) {
                 const TGrid& H = in_port ? A.port : A.cube;
-                const uint32_t hy = (uint32_t)H.nx;
-                return H.mask[(uint32_t)rint(x * H.inv_h) +
-                              hy * ((uint32_t)rint(y * H.inv_h) + (uint32_t)H.ny * (uint32_t)rint(z * H.inv_h))] != 0;
+                const uint32_t i0 = (uint32_t)fmin(fmax(rint(x * H.inv_h), 0.0), (double)(H.nx - 1));
+                const uint32_t i1 = (uint32_t)fmin(fmax(rint(y * H.inv_h), 0.0), (double)(H.ny - 1));
+                const uint32_t i2 = (uint32_t)fmin(fmax(rint(z * H.inv_h), 0.0), (double)(H.nz - 1));
+                return H.mask[i0 + (uint32_t)H.nx * (i1 + (uint32_t)H.ny * i2)] != 0;
             };
             const bool in_port = reg != REG_SPLIT && reg != REG_COMBINE;
             status = tandem_cross(T, A.cube, A.port, reg, last, x, y, z, vz, jk, A.ctr, crash,

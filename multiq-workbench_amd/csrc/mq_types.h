@@ -74,7 +74,7 @@ struct TGrid {
 
 // Coupling geometry and drive, FP64; w_* = f * dt (cycles per step).
 struct TandemP {
-    double dt, seam, ap_r2, z_back, z_ret, z_exit, ax_p, ay_p;
+    double dt, seam, ap_r2, ent_x, ent_z, l_ext, z_ret, z_exit, ax_p, ay_p;
     double mx[9], my[9];
     double v_rf, w_rf, v_ac;
     double w_ac[9];

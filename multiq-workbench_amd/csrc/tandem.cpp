@@ -135,8 +135,11 @@ struct mq_tandem {
         geom = *g;
         const TGrid cg = cube.tgrid(), pg = port.tgrid();
         if (!(g->ap_r > 0.0)) throw Err(-1, "mouth aperture radius must be positive");
-        if (!(g->z_back > 0.0 && g->z_back < g->seam_z && g->seam_z < cg.lim2))
-            throw Err(-1, "need 0 < z_back < seam_z < cube height");
+        if (!(g->seam_z > 0.0 && g->seam_z < cg.lim2)) throw Err(-1, "need 0 < seam_z < cube height");
+        if (!(g->entrance[0] - g->ap_r > 0.0 && g->entrance[0] + g->ap_r < cg.lim0 && g->entrance[1] - g->ap_r > 0.0 &&
+              g->entrance[1] + g->ap_r < cg.lim2))
+            throw Err(-1, "the entrance aperture does not fit on the cube's y face");
+        if (!(g->l_ext > 0.0)) throw Err(-1, "entrance tube length l_ext must be positive");
         for (int k = 0; k < MQ_TANDEM_PORTS; ++k) {
             const double mx = g->mouth[k][0], my = g->mouth[k][1];
             if (!(mx - g->ap_r > 0.0 && my - g->ap_r > 0.0 && mx + g->ap_r < cg.lim0 && my + g->ap_r < cg.lim1))
@@ -220,7 +223,9 @@ struct mq_tandem {
         T.dt = ph.dt;
         T.seam = geom.seam_z;
         T.ap_r2 = geom.ap_r * geom.ap_r;
-        T.z_back = geom.z_back;
+        T.ent_x = geom.entrance[0];
+        T.ent_z = geom.entrance[1];
+        T.l_ext = geom.l_ext;
         T.z_ret = port.meta.z_return;
         T.z_exit = port.meta.z_exit;
         T.ax_p = port.meta.axis_x;

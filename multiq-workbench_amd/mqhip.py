@@ -142,7 +142,8 @@ class ChannelFields(C.Structure):
 
 class TandemGeom(C.Structure):
     """mq_tandem_geom (include/mq_tandem.h), metres in the cube grid's frame."""
-    _fields_ = [("seam_z", C.c_double), ("mouth", (C.c_double * 2) * 9), ("ap_r", C.c_double), ("z_back", C.c_double)]
+    _fields_ = [("seam_z", C.c_double), ("mouth", (C.c_double * 2) * 9), ("ap_r", C.c_double),
+                ("entrance", C.c_double * 2), ("l_ext", C.c_double)]
 
 
 class TandemDrive(C.Structure):
@@ -820,9 +821,12 @@ def fields_from_theirs(directory: str, potentials: bool = False) -> dict:
 TANDEM_REGION_SPLIT, TANDEM_REGION_COMBINE, TANDEM_NO_PORT = 0, 10, 255
 
 
-def tandem_geom(seam_z: float, mouths, ap_r: float, z_back: float) -> TandemGeom:
+def tandem_geom(seam_z: float, mouths, ap_r: float, entrance, l_ext: float) -> TandemGeom:
+    """mq_tandem_geom: the seam, the nine mouth centres (x, y), the aperture radius, the entrance aperture's
+    centre (x, z) on the cube's y = (ny - 1) h face and the entrance tube length."""
     g = TandemGeom()
-    g.seam_z, g.ap_r, g.z_back = seam_z, ap_r, z_back
+    g.seam_z, g.ap_r, g.l_ext = seam_z, ap_r, l_ext
+    g.entrance[0], g.entrance[1] = entrance
     for k, (x, y) in enumerate(mouths):
         g.mouth[k][0], g.mouth[k][1] = x, y
     return g

@@ -201,7 +201,8 @@ def tandem_setup(mod, size: str = "c5", n_per_charge: int | None = None, v_rf: f
         raise ValueError(size)
     ax, ay = cube["axis_x"], cube["axis_y"]
     mouths = [(ax + (c - 1) * pitch, ay + (r - 1) * pitch) for r in range(3) for c in range(3)]
-    geom = mod.tandem_geom(24.0e-3, mouths, ap_r, 0.5e-3)
+    # the entrance aperture on the cube's y face at the injection height, with a 2 mm tube (trapsim's l_ext)
+    geom = mod.tandem_geom(24.0e-3, mouths, ap_r, (ax, cube["z_inject"] + 2.0e-3), 2.0e-3)
     tones = [0.0] * 9
     for p, z in zip(DRIVEN_PORTS, UBQ_CHARGES):
         tones[p] = secular_frequency(UBQ_MASS, z, v_rf, 500e3)

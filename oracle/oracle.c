@@ -865,7 +865,9 @@ int or_grad_ions(const mq_fields* f, const mq_job* job, const double dc[4], cons
 /* ================================================================ tandem batch
  * Restates trapsim's Model::advance / enter / leave
  * (simulation/trapsim/src/lib.rs:176-385) with chsim's step and bath, in the
- * operation order of the GPU's FP64 tandem kernel. Counters per port:
+ * operation order of the GPU's FP64 tandem kernel: the cube's conductor
+ * nodes, the mouths above the seam, the top, the entrance aperture and tube on
+ * the y face, then the other faces (lib.rs:206-243). Counters per port:
  * entered, returned, forwarded, revisits, from_other, reverse entries, lost,
  * retry_from[9], by_job[8] (TC_* layout of the product). */
 
@@ -883,7 +885,7 @@ typedef struct {
 typedef struct {
     or_tgrid cube, port;
     const double* dc[4]; /* splitter, combiner, driven port, idle port */
-    double dt, seam, ap_r2, z_back, z_ret, z_exit, ax_p, ay_p, mx[9], my[9];
+    double dt, seam, ap_r2, ent_x, ent_z, l_ext, z_ret, z_exit, ax_p, ay_p, mx[9], my[9];
     double v_rf, w_rf, v_ac, w_ac[9];
     uint64_t max_steps;
     int nearest;
@@ -899,11 +901,19 @@ static void tgrid_init(or_tgrid* G, const mq_fields* f)
     G->nx = f->nx; G->ny = f->ny; G->nz = f->nz;
 }
 
+/* conductor bit of the nearest node, the index clamped to the grid (a cube ion
+ * may be past its y face, inside the entrance tube) */
+static size_t clamp_node(double v, double inv_h, int n)
+{
+    return (size_t)fmin(fmax(rint(v * inv_h), 0.0), (double)(n - 1));
+}
+
 static int tgrid_crash(const or_tgrid* G, const double x[3])
 {
     if (!G->mask) return 0;
-    return G->mask[(size_t)rint(x[0] * G->inv_h) +
-                   (size_t)G->nx * ((size_t)rint(x[1] * G->inv_h) + (size_t)G->ny * (size_t)rint(x[2] * G->inv_h))] != 0;
+    return G->mask[clamp_node(x[0], G->inv_h, G->nx) +
+                   (size_t)G->nx * (clamp_node(x[1], G->inv_h, G->ny) +
+                                    (size_t)G->ny * clamp_node(x[2], G->inv_h, G->nz))] != 0;
 }
 
 static int outside(const or_tgrid* G, const double x[3])
@@ -916,7 +926,7 @@ static int tandem_cross(const or_tctx* T, uint32_t* reg, uint32_t* last, double 
                         uint64_t* ctr)
 {
     if (*reg == 0 || *reg == 10) {
-        if (outside(&T->cube, x) || tgrid_crash(&T->cube, x)) return 1;
+        if (tgrid_crash(&T->cube, x)) return 1; /* a conductor node of the cube (lib.rs:208-212) */
         if (x[2] > T->seam) {
             uint32_t p = 0;
             double best = 0.0;
@@ -925,27 +935,37 @@ static int tandem_cross(const or_tctx* T, uint32_t* reg, uint32_t* last, double 
                 double d2 = dx * dx + dy * dy;
                 if (k == 0 || d2 < best) { best = d2; p = k; }
             }
-            if (!(best < T->ap_r2)) return 1; /* the board between the mouths */
-            uint64_t* c = ctr + (size_t)p * TC_PORT;
-            x[0] = x[0] + (T->ax_p - T->mx[p]);
-            x[1] = x[1] + (T->ay_p - T->my[p]);
-            if (*reg == 0) {
-                x[2] = T->z_ret + (x[2] - T->seam);
-                c[TC_ENTERED] += 1;
-                if (*last != OR_NO_PORT) {
-                    c[TC_REVISITS] += 1;
-                    c[TC_RETRY + *last] += 1;
-                    if (*last != p) c[TC_FROM_OTHER] += 1;
+            if (best < T->ap_r2) {
+                uint64_t* c = ctr + (size_t)p * TC_PORT;
+                x[0] = x[0] + (T->ax_p - T->mx[p]);
+                x[1] = x[1] + (T->ay_p - T->my[p]);
+                if (*reg == 0) {
+                    x[2] = T->z_ret + (x[2] - T->seam);
+                    c[TC_ENTERED] += 1;
+                    if (*last != OR_NO_PORT) {
+                        c[TC_REVISITS] += 1;
+                        c[TC_RETRY + *last] += 1;
+                        if (*last != p) c[TC_FROM_OTHER] += 1;
+                    }
+                } else {
+                    x[2] = T->z_exit - (x[2] - T->seam);
+                    v[2] = -v[2];
+                    c[TC_REVERSE] += 1;
                 }
-            } else {
-                x[2] = T->z_exit - (x[2] - T->seam);
-                v[2] = -v[2];
-                c[TC_REVERSE] += 1;
+                *reg = 1 + p;
+                return 0;
             }
-            *reg = 1 + p;
-            return 0;
+            /* outside every mouth: the ion keeps moving above the board, up to the top (lib.rs:226-229) */
+            if (x[2] > T->cube.lim[2]) return 1;
         }
-        if (x[2] < T->z_back) return *reg == 0 ? 2 : 3;
+        if (x[1] > T->cube.lim[1]) {
+            /* the entrance face: a wall outside the aperture (lib.rs:231-236); through it, the tube, and
+               l_ext beyond the face the ion is back or detected (lib.rs:237-243) */
+            const double ex = x[0] - T->ent_x, ez = x[2] - T->ent_z;
+            if (!(ex * ex + ez * ez < T->ap_r2)) return 1;
+            if (x[1] > T->cube.lim[1] + T->l_ext) return *reg == 0 ? 2 : 3;
+        }
+        if (x[0] < 0.0 || x[1] < 0.0 || x[2] < 0.0 || x[0] > T->cube.lim[0]) return 1;
         return 0;
     }
     const uint32_t p = *reg - 1;
@@ -1097,7 +1117,9 @@ int or_tandem_run(const mq_fields* cube, const mq_fields* port, const mq_tandem_
     T.dt = ph->dt;
     T.seam = g->seam_z;
     T.ap_r2 = g->ap_r * g->ap_r;
-    T.z_back = g->z_back;
+    T.ent_x = g->entrance[0];
+    T.ent_z = g->entrance[1];
+    T.l_ext = g->l_ext;
     T.z_ret = port->z_return;
     T.z_exit = port->z_exit;
     T.ax_p = port->axis_x;

@@ -14,7 +14,7 @@ import pytest
 
 import mqhip
 import synth
-from test_oracle_tandem import flow_checks, one_ion, rest_physics, setup, drive
+from test_oracle_tandem import closed_form_case, flow_checks
 
 pytestmark = pytest.mark.gpu
 
@@ -48,22 +48,18 @@ def test_plain_matches_oracle_exactly(small, oracle_mod):
     assert sum(p["entered"] for p in c) > 50 and sum(p["returned"] for p in c) > 0
 
 
-@pytest.mark.parametrize("case", ["entry", "bounce", "forward_detect", "reverse", "from_other"])
+@pytest.mark.parametrize("case", ["entry", "bounce", "forward_floor", "reverse", "from_other", "board_top",
+                                  "board_mouth", "back", "wall", "detect"])
 def test_closed_form_cases_match_oracle(oracle_mod, case):
-    ez = 40.0
-    port_e = -ez if case in ("bounce", "from_other") else ez
-    cube_e = (0.24, 0, ez) if case == "from_other" else (0, 0, ez)
-    cube, port, geom = setup([cube_e, None, None, None], [(0, 0, port_e), None, None, None])
-    dcc = {"forward_detect": (-1, 0, 0, 0), "reverse": (3, 0, 0, 0)}.get(case, (1, 0, 0, 0))
-    d = drive(dc_combine=dcc)
-    ph = rest_physics(60000 if case != "entry" else 2755)
+    """The single-ion cases of tests/test_oracle_tandem.py (closed forms there), on the GPU's FP64 path."""
+    cube, port, geom, d, jobs, ph = closed_form_case(case, oracle_mod)
     t = mqhip.Tandem(cube, port, geom)
     try:
         say(f"[gpu] tandem closed-form case {case}")
-        c, tt, s = t.run_ions(one_ion(), d, ph, mqhip.PATH_PLAIN)
+        c, tt, s = t.run_ions(jobs, d, ph, mqhip.PATH_PLAIN)
     finally:
         t.close()
-    co, to, so = oracle_mod.tandem_run(cube, port, geom, one_ion(), d, ph, 1)
+    co, to, so = oracle_mod.tandem_run(cube, port, geom, jobs, d, ph, 1)
     assert c == co and tt == to
     for k in ("x", "y", "z", "vx", "vy", "vz", "steps", "status", "region", "last"):
         assert np.array_equal(s[k], so[k]), k
@@ -109,6 +105,33 @@ def test_tandem_rejects_bad_inputs(small):
         t.run([mqhip.job(8565.0, 7.0, 1000.0, 115.0, 500e3, 0.5, 0.0, 10, 1, 0.5)], d, ph)
     with pytest.raises(mqhip.MqError, match="at most"):
         t.run(jobs * 3, d, ph)
-    bad = mqhip.tandem_geom(30.0e-3, [(geom.mouth[k][0], geom.mouth[k][1]) for k in range(9)], geom.ap_r, 0.5e-3)
+    mouths = [(geom.mouth[k][0], geom.mouth[k][1]) for k in range(9)]
+    ent = (geom.entrance[0], geom.entrance[1])
+    bad = mqhip.tandem_geom(30.0e-3, mouths, geom.ap_r, ent, geom.l_ext)
     with pytest.raises(mqhip.MqError, match="seam"):
         mqhip.Tandem(cube, port, bad)
+    bad = mqhip.tandem_geom(geom.seam_z, mouths, geom.ap_r, (geom.entrance[0], 0.1e-3), geom.l_ext)
+    with pytest.raises(mqhip.MqError, match="entrance"):
+        mqhip.Tandem(cube, port, bad)
+    bad = mqhip.tandem_geom(geom.seam_z, mouths, geom.ap_r, ent, 0.0)
+    with pytest.raises(mqhip.MqError, match="l_ext"):
+        mqhip.Tandem(cube, port, bad)
+
+
+@pytest.mark.timeout(900)
+def test_full_size_plain_matches_oracle_on_a_subset(oracle_mod):
+    """The full-size tandem grids (cube and port 128 x 128 x 512 at 0.05 mm): 4 x 64 ions of the 1M batch's
+    jobs on the FP64 path, every ion's final state and every counter equal to the oracle's."""
+    cube, port, geom, d, jobs, ph = synth.tandem_setup(mqhip, "c5", n_per_charge=64)
+    t = mqhip.Tandem(cube, port, geom)
+    try:
+        say("[gpu] tandem full-size grids, FP64 plain vs oracle, 4 x 64 ions")
+        c, tt, s = t.run_ions(jobs, d, ph, mqhip.PATH_PLAIN)
+    finally:
+        t.close()
+    co, to, so = oracle_mod.tandem_run(cube, port, geom, jobs, d, ph, 16)
+    assert tt == to and c == co
+    for k in ("status", "region", "last", "steps", "ncoll", "job", "x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(s[k], so[k]), k
+    flow_checks(c, tt, s, len(jobs))
+    say(f"[gpu]   totals {tt}")
