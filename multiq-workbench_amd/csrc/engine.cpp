@@ -800,6 +800,29 @@ struct mq_engine {
     }
 };
 
+// ------------------------------------------------------------------ engine view
+// What the live population (live.cpp) needs from an engine: its stream, CUs,
+// tuning, geometry and the FP32 cell table of the DC configuration dc.
+namespace mq {
+namespace host {
+
+EngineView engine_view(mq_engine* e, const double dc[4], const mq_physics& ph, uint32_t njobs)
+{
+    e->prepare_dc(dc, true);
+    HIPCHK(hipGetLastError());
+    EngineView v;
+    v.stream = e->stream;
+    v.cus = e->cus;
+    v.tune = e->tune;
+    v.meta = e->meta;
+    v.cells = e->cells.as<float4>();
+    v.B = e->batch_params(ph, njobs);
+    return v;
+}
+
+}  // namespace host
+}  // namespace mq
+
 // ------------------------------------------------------------------ autotuner
 
 namespace {

@@ -102,6 +102,17 @@ inline uint32_t max_steps_of(const mq_physics& ph)
 }
 
 
+// Device tables and settings of an engine for a DC configuration (engine.cpp).
+struct EngineView {
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    mq_tune tune{};
+    mq_fields meta{};
+    const float4* cells = nullptr;
+    BatchP B{};
+};
+EngineView engine_view(mq_engine* e, const double dc[4], const mq_physics& ph, uint32_t njobs);
+
 // Engine::inject with the reference's xorshift64 stream (inject_ref.cpp): the
 // n ions of one job, as the reference stores them, plus 4 xoshiro128** seed
 // words per ion.

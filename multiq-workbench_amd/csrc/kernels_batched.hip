@@ -111,7 +111,7 @@ __device__ __forceinline__ uint32_t sort_bin(const BatchP& B, const SortGeom& G,
 
 // ------------------------------------------------------------- the worker
 struct Lane {
-    double x, y, z, phi, w_rf, w_ac;
+    double x, y, z, phi, phr, w_rf, w_ac;
     float vx, vy, vz;
     float kq, vrf, vac, sdc, nsig, sgas, vb, mr, omr, floorv, mfpm;
     uint32_t st, ncoll, cd, k0, k1, ion, id, visit;
@@ -151,7 +151,7 @@ __device__ __forceinline__ void cell_field(const float4 R[CELL_F4], const Lane& 
 
 __device__ __forceinline__ void phases(Lane& L)
 {
-    double pr = L.w_rf * (double)L.st;
+    double pr = fma(L.w_rf, (double)L.st, L.phr);  // phr = 0 unless the ion was born on a running clock
     pr = pr - floor(pr);
     double pa = fma(L.w_ac, (double)L.st, L.phi);
     pa = pa - floor(pa);
@@ -215,7 +215,8 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                     const uint32_t id = A.list_in[s0 + slot];
                     const JobP& J = A.jobs[S.job[id]];
                     L.id = id;
-                    L.ion = (uint32_t)(id - J.off);
+                    L.ion = S.sid ? S.sid[id] : (uint32_t)(id - J.off);
+                    L.phr = S.phr ? S.phr[id] : 0.0;
                     L.x = S.x[id]; L.y = S.y[id]; L.z = S.z[id]; L.phi = S.phi[id];
                     L.vx = S.fvx[id]; L.vy = S.fvy[id]; L.vz = S.fvz[id];
                     L.st = S.steps[id]; L.ncoll = S.ncoll[id];

@@ -32,13 +32,10 @@ __device__ __forceinline__ uint32_t job_of(const JobP* jobs, uint32_t njobs, uin
 // ---------------------------------------------------------------- inject
 // Engine::inject (channel.rs:342-374) with the Philox stream: three blocks
 // (ion, 0, k, D_INJ), k = 0..2, give up to 12 uniforms.
-__global__ void k_inject(IonState S, const JobP* __restrict__ jobs, uint32_t njobs, InjP P, uint32_t n)
+// The initial state of ion `ion` of job J, written to slot i of S.
+__device__ __forceinline__ void inject_one(const IonState& S, const JobP& J, const InjP& P, uint32_t ion, uint32_t i,
+                                           uint32_t j)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t j = job_of(jobs, njobs, i);
-    const JobP J = jobs[j];
-    const uint32_t ion = (uint32_t)(i - J.off);
     u32x4 w0 = philox(ion, 0, 0, D_INJ, J.key0, J.key1);
     u32x4 w1 = philox(ion, 0, 1, D_INJ, J.key0, J.key1);
     u32x4 w2 = philox(ion, 0, 2, D_INJ, J.key0, J.key1);
@@ -99,6 +96,67 @@ __global__ void k_inject(IonState S, const JobP* __restrict__ jobs, uint32_t njo
     S.status[i] = 0;
 }
 
+__global__ void k_inject(IonState S, const JobP* __restrict__ jobs, uint32_t njobs, InjP P, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = job_of(jobs, njobs, i);
+    const JobP J = jobs[j];
+    inject_one(S, J, P, (uint32_t)(i - J.off), i, j);
+}
+
+// ------------------------------------------------------- live population
+// New ions of a live population (the injection of mqviz's Sim::block,
+// engine.rs:512-535): ion k goes to slot rec[k].slot with the initial state of
+// ion rec[k].sid of job (species) rec[k].job, the RF phase of the running clock
+// at its birth and the clock's AC phase; its slot is appended to the live list.
+__global__ void k_live_inject(IonState S, const JobP* __restrict__ jobs, InjP P, const LiveNew* __restrict__ rec,
+                              uint32_t k, double phr, double pha, uint32_t* __restrict__ list,
+                              uint32_t* __restrict__ count)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= k) return;
+    const LiveNew r = rec[t];
+    inject_one(S, jobs[r.job], P, r.sid, r.slot, r.job);
+    S.phr[r.slot] = phr;
+    S.phi[r.slot] = pha;
+    S.sid[r.slot] = r.sid;
+    list[count[0] + t] = r.slot;
+}
+
+// After the block's worker launch: the count of the carried list grows by the
+// k new ions (one thread, after k_live_inject).
+__global__ void k_live_count(uint32_t* __restrict__ count, uint32_t k) { count[0] += k; }
+
+// The ions of the block's list that finished during it: {slot, status, steps,
+// collisions} appended to `out` (the host tallies them and recycles their slots).
+__global__ void k_live_collect(IonState S, const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                               uint32_t* __restrict__ out, uint32_t* __restrict__ nout)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count[0]) return;
+    const uint32_t id = list[i];
+    const uint8_t st = S.status[id];
+    if (st == 0) return;
+    const uint32_t k = atomicAdd(nout, 1u);
+    out[4 * k] = id;
+    out[4 * k + 1] = st;
+    out[4 * k + 2] = S.steps[id];
+    out[4 * k + 3] = S.ncoll[id];
+}
+
+// A new AC frequency on the running clock (Clock, world.rs:437-465, whose
+// phase is continuous): every live ion's AC phase offset so that its phase at
+// its current step equals the clock's phase `pha` and then advances by w_ac.
+__global__ void k_live_rephase(IonState S, const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                               double pha, double w_ac)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count[0]) return;
+    const uint32_t id = list[i];
+    const double p = pha - w_ac * (double)S.steps[id];
+    S.phi[id] = p - floor(p);
+}
 
 // ------------------------------------------------------- DC combination
 // Fields::dc_field (channel.rs:220-226) in FP64: e = sum_b v_b E_b, basis order.
@@ -344,6 +402,26 @@ static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b -
 void launch_inject(const IonState& S, const JobP* jobs, uint32_t njobs, const InjP& P, uint32_t n, hipStream_t st)
 {
     if (n) k_inject<<<nblk(n, 256), 256, 0, st>>>(S, jobs, njobs, P, n);
+}
+
+void launch_live_inject(const IonState& S, const JobP* jobs, const InjP& P, const LiveNew* rec, uint32_t k, double phr,
+                        double pha, uint32_t* list, uint32_t* count, hipStream_t st)
+{
+    if (!k) return;
+    k_live_inject<<<nblk(k, 256), 256, 0, st>>>(S, jobs, P, rec, k, phr, pha, list, count);
+    k_live_count<<<1, 1, 0, st>>>(count, k);
+}
+
+void launch_live_collect(const IonState& S, const uint32_t* list, const uint32_t* count, uint32_t nmax, uint32_t* out,
+                         uint32_t* nout, hipStream_t st)
+{
+    if (nmax) k_live_collect<<<nblk(nmax, 256), 256, 0, st>>>(S, list, count, out, nout);
+}
+
+void launch_live_rephase(const IonState& S, const uint32_t* list, const uint32_t* count, uint32_t nmax, double pha,
+                         double w_ac, hipStream_t st)
+{
+    if (nmax) k_live_rephase<<<nblk(nmax, 256), 256, 0, st>>>(S, list, count, pha, w_ac);
 }
 
 void launch_combine_dc(double* out, const double* const d[4], const double v[4], uint32_t n3, hipStream_t st)

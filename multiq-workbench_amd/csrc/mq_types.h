@@ -41,6 +41,10 @@ struct IonState {
     uint8_t *status;
     uint8_t *region = nullptr, *last = nullptr;  // tandem batch only: domain and previous port
     uint32_t* rng = nullptr;  // MQ_RNG_COMPAT only: xoshiro128** state, 4 words per ion [4 i + k]
+    // live population only (batched path): RF phase at the ion's birth [cycles] and its random-stream
+    // number; without them the RF phase starts at 0 and the stream number is the ion's index in its job
+    double* phr = nullptr;
+    uint32_t* sid = nullptr;
 };
 
 // Per-job accumulators (u64): [0] n, [1..5] crashed, returned, transmitted,

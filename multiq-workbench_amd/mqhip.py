@@ -223,6 +223,17 @@ class GradOpts(C.Structure):
                 ("step0", C.c_double * 5), ("tol", C.c_double)]
 
 
+class LiveOpts(C.Structure):
+    _fields_ = [("block_steps", C.c_uint32), ("capacity", C.c_uint32), ("inj_rate", C.c_double),
+                ("recirculate", C.c_int32), ("max_tags", C.c_int32), ("residence_s", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class LiveState(C.Structure):
+    _fields_ = [("step", C.c_uint64), ("alive", C.c_uint64), ("injected", C.c_uint64), ("reentered", C.c_uint64),
+                ("dropped", C.c_uint64), ("finished", C.c_uint64), ("ion_steps", C.c_uint64)]
+
+
 class ScanPoint(C.Structure):
     _fields_ = [("f_khz", C.c_double), ("transmitted", C.c_double), ("crashed", C.c_double), ("n", C.c_uint64)]
 
@@ -277,6 +288,16 @@ SIGNATURES = {
     "mq_optimize_grad": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_double), C.POINTER(Physics),
                                    C.POINTER(Objective), C.POINTER(GradOpts), C.POINTER(C.c_double),
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
+    "mq_live_create": (C.c_int, [C.c_void_p, C.POINTER(Species), C.POINTER(C.c_double), C.c_size_t,
+                                 C.POINTER(RfAc), C.POINTER(C.c_double), C.POINTER(Physics), C.POINTER(LiveOpts),
+                                 C.POINTER(C.c_void_p)]),
+    "mq_live_destroy": (None, [C.c_void_p]),
+    "mq_live_inject": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "mq_live_run": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(LiveState)]),
+    "mq_live_set_ac": (C.c_int, [C.c_void_p, C.c_double, C.c_int32]),
+    "mq_live_tally": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Outcome)]),
+    "mq_live_slots": (C.c_int, [C.c_void_p, C.POINTER(IonSoA), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                C.POINTER(C.c_uint64)]),
     "mq_summarize": (C.c_int, [C.POINTER(C.c_uint8), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                C.POINTER(C.c_uint32), C.c_size_t, C.c_size_t, C.c_double, C.POINTER(Outcome)]),
     "mq_last_stats": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
@@ -538,6 +559,63 @@ class Engine:
         t = Tune()
         _check(load().mq_get_tune(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
+
+
+class Live:
+    """A live ion population on an engine (mq_live_*; mqviz Sim::block, engine.rs:448-538)."""
+
+    def __init__(self, eng: "Engine", species, weights, rf: RfAc, dc, ph: Physics, block_steps=128, capacity=65536,
+                 inj_rate=0.0, recirculate=False, max_tags=1, residence_s=0.0, seed=1):
+        sp = (Species * len(species))(*species)
+        w = None if weights is None else (C.c_double * len(species))(*weights)
+        o = LiveOpts(block_steps, capacity, inj_rate, int(recirculate), max_tags, residence_s, seed)
+        h = C.c_void_p()
+        _check(load().mq_live_create(eng._h, sp, w, len(species), C.byref(rf), (C.c_double * 4)(*dc), C.byref(ph),
+                                     C.byref(o), C.byref(h)))
+        self._h, self.nsp, self.capacity, self._eng = h, len(species), capacity, eng
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().mq_live_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def inject(self, species: int, n: int) -> None:
+        _check(load().mq_live_inject(self._h, species, n))
+
+    def run(self, nblocks: int) -> dict:
+        st = LiveState()
+        _check(load().mq_live_run(self._h, nblocks, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def set_ac(self, f_ac: float, tag: int) -> None:
+        _check(load().mq_live_set_ac(self._h, f_ac, tag))
+
+    def tally(self, tag: int = 0) -> list:
+        out = (Outcome * self.nsp)()
+        _check(load().mq_live_tally(self._h, tag, out))
+        return list(out)
+
+    def slots(self) -> dict:
+        n = self.capacity
+        o = {k: np.zeros(n, np.float64) for k in ("x", "y", "z", "vx", "vy", "vz", "phi_ac0")}
+        o["steps"] = np.zeros(n, np.uint32)
+        o["ncoll"] = np.zeros(n, np.uint32)
+        o["status"] = np.zeros(n, np.uint8)
+        o["cd"] = np.zeros(n, np.uint32)
+        o["sid"] = np.zeros(n, np.uint32)
+        o["species"] = np.zeros(n, np.uint32)
+        o["birth"] = np.zeros(n, np.uint64)
+        soa = soa_of(o)
+        u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+        _check(load().mq_live_slots(self._h, C.byref(soa), u32(o["sid"]), u32(o["species"]),
+                                    o["birth"].ctypes.data_as(C.POINTER(C.c_uint64))))
+        return o
 
 
 def fit_peak(f_khz, t, n) -> dict:
