@@ -145,7 +145,12 @@ typedef struct {
     int32_t chunk;           /* max steps per ion per launch (the re-sort interval) */
     int32_t chunk_cap;       /* without re-sort the chunk doubles per launch up to this */
     int32_t resort;          /* 1: sort the live list by grid cell before every launch,
-                                one queue per XCD over the sorted list */
+                                one queue per XCD over the sorted list; 2: as 1, and
+                                each 256-thread workgroup takes 256 consecutive entries
+                                of the sorted list for a whole chunk with the records of
+                                the 6x6x6 cells around their bin staged in LDS (the
+                                reference's tiled rounds, channel_tiled.metal:237-334;
+                                Philox runs only, COMPAT runs sort as with 1) */
     int32_t sync_every;      /* launches enqueued per host check of the live count */
     int32_t graph;           /* 1: capture those launches as one hipGraph and replay it
                                 (sync_every must be even) */

@@ -603,7 +603,7 @@ struct mq_engine {
     void enqueue_block(const CallBufs& cb, const BatchP& B, uint32_t n, const SortGeom& G, int& cur, uint32_t& chunk,
                        uint32_t cap, uint32_t period, uint32_t grid, uint32_t block, int iters, bool timed,
                        std::vector<hipEvent_t>* evs, bool resort, bool& bins_known, bool pass_bins,
-                       uint32_t lanes = 64)
+                       uint32_t lanes = 64, bool staged = false)
     {
         uint32_t* c0 = cb.ctr;
         uint32_t* c1 = c0 + CTR_BLOCK;
@@ -640,7 +640,9 @@ struct mq_engine {
             bins_known = a.bin_out != nullptr;
             launch_prepare(a, stream);
             if (timed) HIPCHK(hipEventRecord((*evs)[2 * it], stream));
-            launch_batched(a, grid, block, stream);
+            // staged tiles: 256-thread workgroups, two per CU (63 KB of LDS each)
+            if (staged) launch_staged(a, (uint32_t)cus * 2u, stream);
+            else launch_batched(a, grid, block, stream);
             if (timed) HIPCHK(hipEventRecord((*evs)[2 * it + 1], stream));
             cur ^= 1;
             if (!resort) chunk = std::min(cap, chunk * 2u);
@@ -687,6 +689,7 @@ struct mq_engine {
     void run_batched(const CallBufs& cb, const BatchP& B, uint32_t n, float& kms)
     {
         const bool resort = tune.resort != 0;
+        const bool staged = tune.resort == 2 && !B.compat;  // the staged worker draws from Philox streams only
         uint32_t* c0 = cb.ctr;
         const SortGeom G = resort ? sort_geom() : SortGeom{};
         launch_iota(cb.list[0], n, stream);
@@ -709,7 +712,7 @@ struct mq_engine {
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             try {
                 enqueue_block(cb, B, n, G, cur_cap, chunk_cap, cap, period, grid, block, iters, false, nullptr, resort,
-                              known, false);
+                              known, false, 64, staged);
             } catch (...) {
                 (void)hipStreamEndCapture(stream, &g);
                 if (g) (void)hipGraphDestroy(g);
@@ -771,7 +774,7 @@ struct mq_engine {
                 act = std::max(1u, std::min(64u, act));
             }
             enqueue_block(cb, B, n, G, cur, chunk, cap, period, grid, block, iters, timed, &evs, resort && !tail,
-                          bins_known, true, act);
+                          bins_known, true, act, staged && !tail);
             HIPCHK(hipGetLastError());
             stats.launches += (uint32_t)iters;
             uint32_t alive = 0;
@@ -895,7 +898,7 @@ static void autotune(mq_engine& e, const mq_job* jobs, size_t njobs, const doubl
         void (*set)(mq_tune&, int);
     };
     const std::vector<Knob> knobs = {
-        {"resort", {1, 0}, [](mq_tune& t, int v) {
+        {"resort", {1, 0, 2}, [](mq_tune& t, int v) {
              t.resort = v;
              t.chunk = v ? 256 : 20000;
              t.chunk_cap = v ? 256 : 200000;
@@ -1204,7 +1207,7 @@ int mq_set_tune(mq_engine* eng, const mq_tune* t)
             n.chunk = t->chunk;
         }
         if (t->chunk_cap) n.chunk_cap = t->chunk_cap;
-        if (t->resort < 0 || t->resort > 1) throw Err(-1, "resort must be 0 or 1");
+        if (t->resort < 0 || t->resort > 2) throw Err(-1, "resort must be 0, 1 or 2");
         n.resort = t->resort;
         if (t->sync_every) {
             if (t->sync_every < 1 || t->sync_every > 1024) throw Err(-1, "sync_every must be in [1, 1024]");

@@ -76,6 +76,9 @@ void launch_prepare(const WorkerArgs& a, hipStream_t st);
 void launch_gather_probe(const float4* cells, uint64_t window, uint32_t iters, float* out, uint32_t grid,
                          uint32_t block, hipStream_t st);
 void launch_batched(const WorkerArgs& a, uint32_t grid, uint32_t block, hipStream_t st);
+// The staged-tile worker (Philox streams): 256-thread workgroups, each taking 256
+// list entries per chunk; the counter ctr_in[0] hands out the chunks.
+void launch_staged(const WorkerArgs& a, uint32_t grid, hipStream_t st);
 void launch_field_probe(const float4* cells, const BatchP& B, uint32_t n, const double* x, const double* y,
                         const double* z, float a_rf, float s_dc, float a_ac, float* ex, float* ey, float* ez,
                         uint8_t* crash, hipStream_t st);

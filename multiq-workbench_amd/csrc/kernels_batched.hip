@@ -149,6 +149,36 @@ __device__ __forceinline__ void cell_field(const float4 R[CELL_F4], const Lane& 
     }
 }
 
+// ---- staged tiles (the reference's tiled rounds, tc_push, channel_tiled.metal:237-334):
+// a workgroup takes 256 consecutive entries of the re-sorted list and stages the
+// records of the 6 x 6 x 6 cells around the bin of its first ion in LDS. A reload
+// inside the tile reads LDS; outside it, global memory.
+constexpr int TILE_N = 6;
+constexpr int TILE_CELLS = TILE_N * TILE_N * TILE_N;
+constexpr uint32_t TILE_LDS_BYTES = TILE_CELLS * (CELL_F4 - 1) * 16 + TILE_CELLS * 4 + 16;  // 63 088 B
+
+struct Tile {
+    float4* rec;    // [TILE_CELLS][CELL_F4 - 1]: the coefficients
+    uint32_t* msk;  // [TILE_CELLS]: the corner conductor bits
+    int32_t* org;   // [4]: cell index of the tile's corner (i, j, k), then the chunk's first entry
+};
+
+__device__ __forceinline__ void reload_tiled(const BatchP& B, const float4* __restrict__ cells, const Tile& T,
+                                             int32_t c, float4 R[CELL_F4])
+{
+    const int32_t cx = B.nx - 1, cy = B.ny - 1;
+    const int32_t i = c % cx, j = (c / cx) % cy, k = c / (cx * cy);
+    const int32_t ti = i - T.org[0], tj = j - T.org[1], tk = k - T.org[2];
+    if ((uint32_t)ti < (uint32_t)TILE_N && (uint32_t)tj < (uint32_t)TILE_N && (uint32_t)tk < (uint32_t)TILE_N) {
+        const int32_t t = ti + TILE_N * (tj + TILE_N * tk);
+#pragma unroll
+        for (int q = 0; q < CELL_F4 - 1; ++q) R[q] = T.rec[t * (CELL_F4 - 1) + q];
+        R[CELL_F4 - 1] = make_float4(__uint_as_float(T.msk[t]), 0.0f, 0.0f, 0.0f);
+    } else {
+        load_cell(cells, c, R);
+    }
+}
+
 __device__ __forceinline__ void phases(Lane& L)
 {
     double pr = fma(L.w_rf, (double)L.st, L.phr);  // phr = 0 unless the ion was born on a running clock
@@ -170,9 +200,12 @@ __device__ __forceinline__ uint32_t countdown_f(float x)
 // thermal bath), so each instantiation keeps only its own values live.
 // COMPAT selects the reference's per-ion xoshiro128** stream (draw order of
 // channel.metal:93-113) instead of Philox.
-template <bool LEGACY, bool COMPAT>
+// STAGED: workgroup chunks of the re-sorted list with their tile staged in LDS
+// (no refill inside a chunk; the launch's chunk is one refill period).
+template <bool LEGACY, bool COMPAT, bool STAGED>
 __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
 {
+    static_assert(!(STAGED && COMPAT), "staged chunks use the Philox streams");
     const BatchP& B = A.B;
     const IonState& S = A.S;
     const uint32_t lane = threadIdx.x & 63;
@@ -186,7 +219,73 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
     bool have = false, drained = false, pending = false;
     uint32_t pcur = 0;
     L.cell = -1;
+    extern __shared__ float4 lds_tile[];
+    Tile T{};
+    if constexpr (STAGED) {
+        T.rec = lds_tile;
+        T.msk = reinterpret_cast<uint32_t*>(lds_tile + TILE_CELLS * (CELL_F4 - 1));
+        T.org = reinterpret_cast<int32_t*>(T.msk + TILE_CELLS);
+    }
     for (;;) {
+        if constexpr (STAGED) {
+            // ---- claim the next 256 entries for the workgroup and stage the tile around
+            // the re-sort bin of the first one (the entries of a chunk are bin-sorted)
+            __syncthreads();  // the previous chunk's last reads of the tile are done
+            if (threadIdx.x == 0) {
+                const uint32_t b0 = atomicAdd(&A.ctr_in[0], blockDim.x);
+                T.org[3] = (int32_t)b0;
+                if (b0 < count) {
+                    const uint32_t id0 = A.list_in[b0];
+                    const double g[3] = {S.x[id0] * B.inv_h, S.y[id0] * B.inv_h, S.z[id0] * B.inv_h};
+                    const int32_t n[3] = {B.nx, B.ny, B.nz};
+                    const uint32_t sh[3] = {A.sg.sx, A.sg.sy, A.sg.sz};
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const int32_t ci = min((int32_t)fmin(fmax(g[d], 0.0), (double)(n[d] - 1)), n[d] - 2);
+                        T.org[d] = (int32_t)(((uint32_t)ci >> sh[d]) << sh[d]) - 1;  // one cell of halo
+                    }
+                }
+            }
+            __syncthreads();
+            const uint32_t b0 = (uint32_t)T.org[3];
+            if (b0 >= count) break;  // the same value in every thread of the workgroup
+            const int32_t cx = B.nx - 1, cy = B.ny - 1, cz = B.nz - 1;
+            for (uint32_t q = threadIdx.x; q < (uint32_t)(TILE_CELLS * CELL_F4); q += blockDim.x) {
+                const int32_t t = (int32_t)(q / CELL_F4), f = (int32_t)(q % CELL_F4);
+                const int32_t gi = T.org[0] + t % TILE_N, gj = T.org[1] + (t / TILE_N) % TILE_N,
+                              gk = T.org[2] + t / (TILE_N * TILE_N);
+                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (gi >= 0 && gj >= 0 && gk >= 0 && gi < cx && gj < cy && gk < cz)
+                    v = A.cells[(size_t)(gi + cx * (gj + cy * gk)) * CELL_STRIDE_F4 + f];
+                if (f < CELL_F4 - 1) T.rec[t * (CELL_F4 - 1) + f] = v;
+                else T.msk[t] = __float_as_uint(v.x);
+            }
+            __syncthreads();
+            const uint32_t slot = b0 + threadIdx.x;
+            if (slot < count) {
+                const uint32_t id = A.list_in[slot];
+                const JobP& J = A.jobs[S.job[id]];
+                L.id = id;
+                L.ion = S.sid ? S.sid[id] : (uint32_t)(id - J.off);
+                L.phr = S.phr ? S.phr[id] : 0.0;
+                L.x = S.x[id]; L.y = S.y[id]; L.z = S.z[id]; L.phi = S.phi[id];
+                L.vx = S.fvx[id]; L.vy = S.fvy[id]; L.vz = S.fvz[id];
+                L.st = S.steps[id]; L.ncoll = S.ncoll[id];
+                if constexpr (LEGACY) L.cd = S.cd[id];
+                L.w_rf = J.w_rf; L.w_ac = J.w_ac;
+                L.kq = (float)J.kq; L.vrf = (float)J.v_rf; L.vac = (float)J.v_ac; L.sdc = (float)J.s_dc;
+                L.nsig = (float)J.nsig_dt; L.sgas = (float)J.s_gas; L.vb = (float)J.v_bound;
+                L.mr = (float)J.mr; L.omr = (float)J.one_m_mr; L.floorv = (float)J.floor_v;
+                L.mfpm = (float)J.mfp_mean;
+                L.k0 = J.key0; L.k1 = J.key1;
+                L.visit = 0;
+                L.cw1 = L.cw2 = L.cw3 = 0;
+                L.cell = locate(B, L);
+                reload_tiled(B, A.cells, T, L.cell, R);
+                phases(L);
+                have = true;
+            }
+        } else {
         // ---- refill idle lanes from the live list (one atomic per wavefront). With
         // several queues the list is cut into contiguous (spatially sorted) segments
         // and the wavefronts of one XCD start on their own segment, then steal.
@@ -242,6 +341,7 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
             break;
         }
         if (!__any(have)) break;
+        }
         // ---- advance every held ion by up to `period` steps. A lane whose collision
         // candidate fires (thermal) or whose countdown expires (legacy) becomes
         // `pending` and pauses; the collision code then runs once for every pending
@@ -252,7 +352,8 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
         // the step, and its next kick waits for its collision.
         bool stop = false;
         int status = 0;
-        for (uint32_t r = 0; r < A.period; ++r) {
+        // A staged chunk runs until every ion of the workgroup has stopped.
+        for (uint32_t r = 0; STAGED || r < A.period; ++r) {
             if (have && !stop && !pending) {
                 float e[3];
                 cell_field(R, L, e);
@@ -266,7 +367,11 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                 L.st += 1;
                 // next-step field cell (hoisted above the collision code: it depends on x only)
                 const int32_t c = locate(B, L);
-                if (c != L.cell) { L.cell = c; load_cell(A.cells, c, R); }
+                if (c != L.cell) {
+                    L.cell = c;
+                    if constexpr (STAGED) reload_tiled(B, A.cells, T, c, R);
+                    else load_cell(A.cells, c, R);
+                }
                 phases(L);
                 // ---- collision candidate of this step
                 if constexpr (!LEGACY && COMPAT) {
@@ -308,7 +413,7 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
             if (pm) {
                 const uint32_t npend = (uint32_t)__popcll(pm);
                 const uint64_t am = __ballot(have && !stop && !pending);
-                if (npend >= A.coll_batch || am == 0 || r + 1 == A.period) {
+                if (npend >= A.coll_batch || am == 0 || (!STAGED && r + 1 == A.period)) {
                     if (pending && COMPAT) {
                         if constexpr (!LEGACY) {
                             const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
@@ -387,6 +492,10 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                         pending = false;
                     }
                 }
+            }
+            // with no lane left to step, the batch above has run every pending collision
+            if constexpr (STAGED) {
+                if (!__any(have && !stop)) break;
             }
         }
         // ---- write back stopped ions; carry unfinished ones to the next launch
@@ -613,12 +722,18 @@ void launch_prepare(const WorkerArgs& a, hipStream_t st) { k_prepare<<<1, 64, 0,
 void launch_batched(const WorkerArgs& a, uint32_t grid, uint32_t block, hipStream_t st)
 {
     if (a.B.compat) {
-        if (a.B.legacy_coll) k_batched<true, true><<<grid, block, 0, st>>>(a);
-        else k_batched<false, true><<<grid, block, 0, st>>>(a);
+        if (a.B.legacy_coll) k_batched<true, true, false><<<grid, block, 0, st>>>(a);
+        else k_batched<false, true, false><<<grid, block, 0, st>>>(a);
     } else {
-        if (a.B.legacy_coll) k_batched<true, false><<<grid, block, 0, st>>>(a);
-        else k_batched<false, false><<<grid, block, 0, st>>>(a);
+        if (a.B.legacy_coll) k_batched<true, false, false><<<grid, block, 0, st>>>(a);
+        else k_batched<false, false, false><<<grid, block, 0, st>>>(a);
     }
+}
+
+void launch_staged(const WorkerArgs& a, uint32_t grid, hipStream_t st)
+{
+    if (a.B.legacy_coll) k_batched<true, false, true><<<grid, 256, TILE_LDS_BYTES, st>>>(a);
+    else k_batched<false, false, true><<<grid, 256, TILE_LDS_BYTES, st>>>(a);
 }
 
 }  // namespace mq

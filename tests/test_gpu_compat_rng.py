@@ -96,7 +96,10 @@ def test_batched_compat_independent_of_batch_tuning_and_collision_batching(c1):
     try:
         for v in (dict(block=64, waves_per_cu=4, refill_period=4, chunk=96, chunk_cap=96, resort=0, coll_batch=1),
                   dict(block=256, waves_per_cu=12, refill_period=16, chunk=512, chunk_cap=512, resort=1,
-                       coll_batch=16)):
+                       coll_batch=16),
+                  # staged tiles are Philox-only: COMPAT runs keep the unstaged worker
+                  dict(block=256, waves_per_cu=8, refill_period=8, chunk=64, chunk_cap=64, resort=2,
+                       coll_batch=4)):
             e.set_tune(**v)
             r = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
             for k in KEYS + ("steps", "ncoll", "status"):

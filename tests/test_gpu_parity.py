@@ -130,6 +130,11 @@ def test_batched_independent_of_batch_and_tuning(c1):
              coll_batch=6),
         dict(block=64, waves_per_cu=4, refill_period=4, chunk=128, chunk_cap=128, resort=1, sync_every=8, graph=0,
              coll_batch=64),
+        # staged tiles: workgroup chunks of the sorted list with their records in LDS
+        dict(block=256, waves_per_cu=8, refill_period=8, chunk=64, chunk_cap=64, resort=2, sync_every=4, graph=0,
+             coll_batch=4),
+        dict(block=256, waves_per_cu=8, refill_period=8, chunk=40, chunk_cap=40, resort=2, sync_every=4, graph=1,
+             coll_batch=1),
     ]
     runs = []
     try:
@@ -157,13 +162,13 @@ def test_batched_collision_batching_is_exact(c1):
         for case in ("half_thermal", "legacy"):
             ph = physics(case)
             jobs = jobs_for(case, 600)
-            e.set_tune(coll_batch=1)
+            e.set_tune(coll_batch=1, resort=1)
             a = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
-            for cb in (4, 16):
-                e.set_tune(coll_batch=cb)
+            for cb, rs in ((4, 1), (16, 1), (4, 2), (16, 2)):
+                e.set_tune(coll_batch=cb, resort=rs, graph=t0["graph"], tail_lanes=t0["tail_lanes"])
                 b = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
                 for k in KEYS + ("steps", "ncoll", "status"):
-                    assert np.array_equal(a[k], b[k]), (case, cb, k)
+                    assert np.array_equal(a[k], b[k]), (case, cb, rs, k)
     finally:
         e.set_tune(**t0)
 

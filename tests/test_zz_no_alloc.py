@@ -75,7 +75,9 @@ def test_noalloc_across_tunings(eng):
         for v in (dict(block=64, waves_per_cu=3, refill_period=4, chunk=100, chunk_cap=400, resort=0, sync_every=1,
                        graph=0),
                   dict(block=128, waves_per_cu=8, refill_period=16, chunk=64, chunk_cap=64, resort=1, sync_every=4,
-                       graph=1)):
+                       graph=1),
+                  dict(block=256, waves_per_cu=8, refill_period=8, chunk=64, chunk_cap=64, resort=2, sync_every=4,
+                       graph=0)):
             eng.set_tune(**v)
             say(f"[gpu] no-alloc under tune {v}")
             _same(want, eng.run_batch_ions(jobs, [0, 5, 0, 0], ph, noalloc=True), v)
