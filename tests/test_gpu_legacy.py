@@ -19,11 +19,13 @@ whole chain against the reference's own published numbers:
   family-wise), the sum of z^2 <= 52.6 against rust_legacy (chi-square, 25
   degrees of freedom, 0.999), |z| <= 3.54 against both at m/z 493 (the flank
   that separates these potentials from converged ones: 343 published, about
-  230 on converged fields), the peak at m/z 497-501. Against rust_legacy
-  alone, m/z 495 sits about 3 sigma low (620 per 1000 with 20,000 ions per
-  point, 670 published; java_original 630), which a per-point 3-sigma bound
-  against that column would flag in about a third of runs at 2,000 ions per
-  point.
+  250 on converged fields), the peak at m/z 497-501. Against rust_legacy
+  alone, m/z 495 sits 3.3 sigma low: this engine gives 620 +- 3 per 1000 on
+  the FP32 path (32,000 ions) and 623 +- 3.5 on the FP64 path (12,000 ions),
+  java_original 630 and rust_legacy 670 (the two published columns differ by
+  1.9 sigma there); none of 32 independent 1000-ion samples reached 640. A
+  per-point 3-sigma bound against that column would flag about a third of
+  runs at 2,000 ions per point.
 """
 import csv
 import hashlib

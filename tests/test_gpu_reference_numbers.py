@@ -7,11 +7,16 @@ then the integrator.
   the reference's Metal engine at the S14 calibrated point, 100,000 ions.
   Every outcome class within 3 sigma (binomial errors of both samples).
 * docs/mass-scan.csv (tests/golden/mass-scan.csv): the legacy Fig. 4 scan.
-  The published columns come from the Java-exported potentials, not
-  reproducible here; on converged fields the peak must sit at m/z 497-501
-  with a height within 3 sigma of 742/1000, both ends of the scan must be
-  empty, and with the full kick no ion may be transmitted (0 of 25,000, the
-  rust_full_kick column).
+  Its java_original and rust_legacy columns were run on the Java-exported
+  potentials (reproduced in tests/test_gpu_legacy.py), not on converged
+  fields, so on converged fields only what does not depend on the potentials'
+  convergence is asserted: the peak at m/z 497-501, both ends of the scan
+  empty, and with the full kick no ion transmitted (0 of 25,000, the
+  rust_full_kick column). The height on converged fields has no reference
+  figure that reproduces: channel.rs:707's comment "ours on converged
+  fields: 55.2 %, 12.3, 41" is not output, and the same command
+  (legacy-fig4, channel.rs:748-760) gives 72.8 % at m/z 499, FWHM 11.3 here
+  (4 x 1000 ions per m/z); it is reported, not gated.
 * Both paths: the FP32 batched and FP64 plain forms agree within 3 sigma per m/z.
 """
 import csv
@@ -99,8 +104,8 @@ def test_fig4_legacy_peak_on_converged_fields(fig4):
     for path, t in out.items():
         say(f"   path {path}: {t}")
         k = int(np.argmax(t))
+        say(f"   peak {max(t)} at m/z {mzs[k]} (rust_legacy on the exported potentials: {published[499.0]})")
         assert 497 <= mzs[k] <= 501, mzs[k]
-        assert abs(pull(max(t), 1000, published[499.0], 1000)) <= 3.0, (max(t), published[499.0])
         # both ends empty, as in the published columns (at most a few ions)
         assert sum(t[:5]) <= 3 and sum(t[-3:]) <= 10
 
