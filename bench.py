@@ -244,7 +244,8 @@ def tandem_bench(steps: int = 2):
     pwall = time.perf_counter() - t0
     psteps = t.stats()["ion_steps"]
     t.close()
-    log(f"tandem: batched {n_steps / wall:.3e} ion-steps/s, plain {psteps / pwall:.3e}")
+    log(f"tandem: batched {n_steps / wall:.3e} ion-steps/s ({n_steps // steps} ion-steps per run), "
+        f"plain {psteps / pwall:.3e} ({psteps} ion-steps)")
 
     def pull(a, b):
         return (a - b) / np.sqrt(max(a + b, 1))
