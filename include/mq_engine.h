@@ -263,10 +263,17 @@ int mq_autotune(mq_engine* eng, const mq_job* jobs, size_t njobs, const double d
 /* ------------------------------------------------- differentiable form */
 
 /* Objective of the differentiable form: every ion of one job is integrated
- * for exactly nsteps collision-free FP64 steps (no termination), then
+ * for exactly nsteps FP64 steps (no termination), then
  *   J = mean_i [ sigmoid((z_i - z_exit) / scale)
  *               - w_leak * sigmoid((r_i - r_leak) / scale) ],
- * r_i = distance from the injection axis. ckpt_every = 0 picks ceil(sqrt(nsteps)). */
+ * r_i = distance from the injection axis. ckpt_every = 0 picks ceil(sqrt(nsteps)).
+ * With no gas (pressure 0 or ccs_n2 0) the steps are collision-free. With gas
+ * the thermal collisions of the plain path apply with common random numbers:
+ * each ion's Philox draws depend on (seed, ion, step) only, the candidate and
+ * acceptance decisions are those of the run, and the gradient is the pathwise
+ * derivative through each collision's velocity map with the decisions held
+ * fixed (it omits the jumps where a decision changes with theta). Legacy
+ * collisions are rejected. */
 typedef struct {
     uint32_t nsteps, ckpt_every;
     double z_exit, scale, r_leak, w_leak;

@@ -495,8 +495,8 @@ struct mq_engine {
         if (job.n == 0 || job.n >= (1ull << 31)) throw Err(-1, "the job needs between 1 and 2^31 ions");
         if (o.nsteps == 0) throw Err(-1, "objective needs nsteps > 0");
         if (!(o.scale > 0.0)) throw Err(-1, "objective scale must be positive");
-        if (ph.pressure_pa > 0.0 || job.pressure_pa > 0.0)
-            throw Err(-1, "the differentiable form is collision-free: set pressure_pa to 0");
+        if ((ph.pressure_pa > 0.0 || job.pressure_pa > 0.0) && ph.legacy_collisions)
+            throw Err(-1, "the differentiable form supports the thermal collision model only (legacy_collisions = 0)");
         if (ph.rng != MQ_RNG_PHILOX) throw Err(-1, "the differentiable form injects with the Philox stream (rng = 0)");
         const uint32_t n = (uint32_t)job.n;
         const uint32_t K = o.ckpt_every ? o.ckpt_every : (uint32_t)std::ceil(std::sqrt((double)o.nsteps));
@@ -512,13 +512,15 @@ struct mq_engine {
         ip.model = ph.injection == MQ_INJ_GAUSSIAN ? 2 : (ph.legacy_injection ? 1 : 0);
         launch_inject(S, jobs.as<JobP>(), 1, ip, n, stream);
         HIPCHK(hipGetLastError());
-        // buffers: 6 slots of checkpoints, 3 x K segment, 3 final, J, 5 + 6 gradients
-        const size_t per_ion = (size_t)6 * slots + 3 * K + 3 + 1 + 5 + 6;
+        // buffers: 6 slots of checkpoints, 3 x K segment (6 x K with gas: the post-kick velocities too),
+        // 3 final, J, 5 + 6 gradients
+        const int nseg = jp.nsig_dt > 0.0 ? 6 : 3;
+        const size_t per_ion = (size_t)6 * slots + (size_t)nseg * K + 3 + 1 + 5 + 6;
         adj_mem.ensure(per_ion * n * sizeof(double));
         double* p = adj_mem.as<double>();
         AdjBuf A{};
         for (int k = 0; k < 6; ++k) { A.ck[k] = p; p += (size_t)slots * n; }
-        for (int k = 0; k < 3; ++k) { A.seg[k] = p; p += (size_t)K * n; }
+        for (int k = 0; k < nseg; ++k) { A.seg[k] = p; p += (size_t)K * n; }
         for (int k = 0; k < 3; ++k) { A.fin[k] = p; p += n; }
         A.Ji = p; p += n;
         for (int k = 0; k < 5; ++k) { A.gp[k] = p; p += n; }
@@ -526,6 +528,7 @@ struct mq_engine {
         AdjObj O{};
         O.z_exit = o.z_exit; O.scale = o.scale; O.r_leak = o.r_leak; O.w_leak = o.w_leak;
         O.axis_x = meta.axis_x; O.axis_y = meta.axis_y; O.nsteps = o.nsteps; O.ckpt = K;
+        O.gas = nseg == 6 ? 1u : 0u;
         BatchP B = batch_params(ph, 1);
         const AdjGrids G = adj_grids();
         stats = mq_run_stats{};

@@ -1,5 +1,6 @@
 // kernels_adjoint.hip -- the differentiable form: gradients of a smoothed
-// transmission / leakage objective through the FP64 collision-free step.
+// transmission / leakage objective through the FP64 step, collision-free or
+// with the thermal collisions of the Philox streams held at their draws.
 //
 // The reference has no gradients (its Optimize search is a grid of sweeps with
 // Lorentzian fits, src-tauri/src/research.rs:133-163); this is the hand-written
@@ -11,6 +12,12 @@
 // Backward step (lambda' = dJ/d(x', v')):
 //     mu = lambda_v' + dt lambda_x',  G = kq mu,
 //     lambda_x = lambda_x' + (dE/dx)^T G,  lambda_v = mu,  dJ/dtheta += (dE/dtheta)^T G.
+// With gas, a step that ends in a collision maps the post-kick velocity v by
+// v' = v - mr g + mr |g| d with g and d from the step's Philox draws (frozen:
+// the decisions are those of the primal run and carry no derivative); its
+// adjoint lambda_v' <- (1 - mr) lambda_v' + mr (g / |g|) (d . lambda_v') comes
+// before the step's own adjoint. The recompute pass then also stores each
+// step's post-kick velocity, from which the walk redraws the collision.
 // sqrt(N) checkpointing: the forward stores (x, v) every K steps; the backward
 // recomputes each K-step segment from its checkpoint into a per-ion buffer and
 // walks it backwards, so memory is O((N/K + K) ions) and work is 2 forwards
@@ -19,6 +26,7 @@
 
 #include <algorithm>
 
+#include "fp64_sample.h"
 #include "kernels.h"
 #include "mq_math.h"
 #include "mq_types.h"
@@ -216,6 +224,40 @@ __device__ __forceinline__ void step_fwd(const BatchP& B, const JobP& J, const G
     E = (a_rf * er[2] + J.s_dc * ed[2]) + a_ac * ea[2]; vz = fma(J.kq, E, vz); z = fma(vz, B.dt, z);
 }
 
+// Thermal collision of step n on the post-kick velocity (vx, vy, vz): the
+// Philox branch of k_plain, operation for operation, so the decisions and the
+// new velocity are the plain path's. With the draws frozen the collision is
+// the map v' = v - mr g + mr |g| d (g = v minus the drawn gas velocity, d the
+// drawn direction); on a collision it returns true with g / |g| and d, the
+// factors of its Jacobian (1 - mr) I + mr d (g / |g|)^T.
+__device__ __forceinline__ bool collide_frozen(const JobP& J, uint32_t ion, uint32_t n, double& vx, double& vy,
+                                               double& vz, double gh[3], double d[3])
+{
+    const double bound = norm3(vx, vy, vz) + J.v_bound;
+    const u32x4 cw = philox(ion, n >> 2, 0, D_CAND, J.key0, J.key1);
+    if (!(u01(pick(cw, n & 3)) < J.nsig_dt * bound)) return false;
+    const u32x4 w = philox(ion, n, 0, D_COLL, J.key0, J.key1);
+    const double r1 = sqrt(-2.0 * log_d(u01(w.a))), r2 = sqrt(-2.0 * log_d(u01(w.c)));
+    double s1, c1, s2, c2;
+    sincos2pi(u01(w.b), s1, c1);
+    sincos2pi(u01(w.d), s2, c2);
+    const double gx = vx - J.s_gas * (r1 * c1), gy = vy - J.s_gas * (r1 * s1), gz = vz - J.s_gas * (r2 * c2);
+    const double gl = norm3(gx, gy, gz);
+    const u32x4 w2 = philox(ion, n, 1, D_COLL, J.key0, J.key1);
+    if (!(u01(w2.a) * bound < gl)) return false;
+    const double c = 2.0 * u01(w2.b) - 1.0;
+    double sp, cp;
+    sincos2pi(u01(w2.c), sp, cp);
+    const double sn = sqrt(fmax(0.0, 1.0 - c * c));
+    const double mg = J.mr * gl;
+    d[0] = sn * cp; d[1] = sn * sp; d[2] = c;
+    gh[0] = gx / gl; gh[1] = gy / gl; gh[2] = gz / gl;
+    vx = (vx - J.mr * gx) + mg * d[0];
+    vy = (vy - J.mr * gy) + mg * d[1];
+    vz = (vz - J.mr * gz) + mg * d[2];
+    return true;
+}
+
 __device__ __forceinline__ double sigm(double u) { return 1.0 / (1.0 + exp(-u)); }
 
 // Objective of one ion at its final state and dJ_i/d(x, y, z).
@@ -242,7 +284,9 @@ __device__ __forceinline__ double objective(const AdjObj& O, double x, double y,
 }
 
 // Forward: N steps per ion, (x, v) stored every K steps (slot s holds step s*K).
-template <class G_>
+// GAS: thermal collisions with frozen draws (a separate instantiation, so the
+// collision-free kernels keep their register budget).
+template <class G_, bool GAS>
 __global__ __launch_bounds__(256) void k_adj_forward(IonState S, const JobP* __restrict__ jobs, BatchP B, G_ G,
                                                     AdjObj O, AdjBuf A, uint32_t n)
 {
@@ -256,12 +300,17 @@ __global__ __launch_bounds__(256) void k_adj_forward(IonState S, const JobP* __r
     // flight at the loop head make the wait before their first use in each
     // step also wait for that step's record loads.
     asm volatile("" ::"v"(J.kq), "v"(J.v_rf), "v"(J.w_rf), "v"(J.v_ac), "v"(J.w_ac), "v"(J.s_dc), "v"(phi));
+    const uint32_t ion = (uint32_t)(i - J.off);
     for (uint32_t s = 0; s < O.nsteps; ++s) {
         if (s % O.ckpt == 0) {
             const size_t o = (size_t)(s / O.ckpt) * n + i;
             A.ck[0][o] = x; A.ck[1][o] = y; A.ck[2][o] = z; A.ck[3][o] = vx; A.ck[4][o] = vy; A.ck[5][o] = vz;
         }
         step_fwd(B, J, G, K, phi, s, x, y, z, vx, vy, vz);
+        if constexpr (GAS) {
+            double gh[3], d[3];
+            (void)collide_frozen(J, ion, s, vx, vy, vz, gh, d);
+        }
     }
     double g[3];
     A.Ji[i] = objective(O, x, y, z, g);
@@ -316,7 +365,7 @@ __device__ __forceinline__ void walk_step(const BatchP& B, const JobP& J, const 
 // Backward: per ion, segments last to first; per-ion partial gradients and the
 // gradient with respect to the initial state. In the walk down a segment the
 // recomputed position of step s - 1 is loaded during step s.
-template <class G_>
+template <class G_, bool GAS>
 __global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __restrict__ jobs, BatchP B, G_ G,
                                                      AdjObj O, AdjBuf A, uint32_t n, double scale)
 {
@@ -331,6 +380,7 @@ __global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __
     const uint32_t nseg = (O.nsteps + O.ckpt - 1) / O.ckpt;
     Cache<G_> K;  // record cache, kept across segments (records are keyed by the cell index)
     asm volatile("" ::"v"(J.kq), "v"(J.v_rf), "v"(J.w_rf), "v"(J.v_ac), "v"(J.w_ac), "v"(J.s_dc), "v"(phi));  // as in k_adj_forward
+    const uint32_t ion = (uint32_t)(i - J.off);  // with GAS, seg[3..5] hold each step's post-kick velocity
     for (int seg = (int)nseg - 1; seg >= 0; --seg) {
         const uint32_t s0 = (uint32_t)seg * O.ckpt;
         const uint32_t s1 = min(O.nsteps, s0 + O.ckpt);
@@ -341,6 +391,11 @@ __global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __
             const size_t o = (size_t)(s - s0) * n + i;
             A.seg[0][o] = x; A.seg[1][o] = y; A.seg[2][o] = z;
             step_fwd(B, J, G, K, phi, s, x, y, z, vx, vy, vz);
+            if constexpr (GAS) {
+                A.seg[3][o] = vx; A.seg[4][o] = vy; A.seg[5][o] = vz;
+                double gh[3], d[3];
+                (void)collide_frozen(J, ion, s, vx, vy, vz, gh, d);
+            }
         }
         // walk it backwards (the look-ahead index is clamped to s0: a redundant read in bounds)
         const auto pos = [&](int s, double& px, double& py, double& pz) {
@@ -353,6 +408,17 @@ __global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __
             const Cell3 c = cell_of(B, px, py, pz);
             refresh(G, c, K);
             pos(s - 1, px, py, pz);
+            if constexpr (GAS) {
+                // the collision that ended step s: lambda_v <- ((1 - mr) I + mr d gh^T)^T lambda_v
+                const size_t o = (size_t)((uint32_t)s - s0) * n + i;
+                double ux = A.seg[3][o], uy = A.seg[4][o], uz = A.seg[5][o], gh[3], d[3];
+                if (collide_frozen(J, ion, (uint32_t)s, ux, uy, uz, gh, d)) {
+                    const double dl = J.mr * ((d[0] * a.lvx + d[1] * a.lvy) + d[2] * a.lvz);
+                    a.lvx = (1.0 - J.mr) * a.lvx + gh[0] * dl;
+                    a.lvy = (1.0 - J.mr) * a.lvy + gh[1] * dl;
+                    a.lvz = (1.0 - J.mr) * a.lvz + gh[2] * dl;
+                }
+            }
             walk_step(B, J, G, K, c, phi, two_pi_dt, (uint32_t)s, a);
         }
     }
@@ -401,16 +467,30 @@ void launch_adj_forward(const IonState& S, const JobP* jobs, const BatchP& B, co
                         const AdjBuf& A, uint32_t n, hipStream_t st)
 {
     if (!n) return;
-    if (G.rec) k_adj_forward<CellRecs><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, CellRecs{G.rec}, O, A, n);
-    else k_adj_forward<NodeGrids><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, node_grids(B, G), O, A, n);
+    if (O.gas) {
+        if (G.rec) k_adj_forward<CellRecs, true><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, CellRecs{G.rec}, O, A, n);
+        else k_adj_forward<NodeGrids, true><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, node_grids(B, G), O, A, n);
+    } else {
+        if (G.rec) k_adj_forward<CellRecs, false><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, CellRecs{G.rec}, O, A, n);
+        else k_adj_forward<NodeGrids, false><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, node_grids(B, G), O, A, n);
+    }
 }
 
 void launch_adj_backward(const IonState& S, const JobP* jobs, const BatchP& B, const AdjGrids& G, const AdjObj& O,
                          const AdjBuf& A, uint32_t n, double scale, hipStream_t st)
 {
     if (!n) return;
-    if (G.rec) k_adj_backward<CellRecs><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, CellRecs{G.rec}, O, A, n, scale);
-    else k_adj_backward<NodeGrids><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, node_grids(B, G), O, A, n, scale);
+    if (O.gas) {
+        if (G.rec)
+            k_adj_backward<CellRecs, true><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, CellRecs{G.rec}, O, A, n, scale);
+        else
+            k_adj_backward<NodeGrids, true><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, node_grids(B, G), O, A, n, scale);
+    } else {
+        if (G.rec)
+            k_adj_backward<CellRecs, false><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, CellRecs{G.rec}, O, A, n, scale);
+        else
+            k_adj_backward<NodeGrids, false><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, node_grids(B, G), O, A, n, scale);
+    }
 }
 
 // flag = 0 when some value of a is not exactly a float (flag starts at 1).

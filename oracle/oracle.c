@@ -761,6 +761,38 @@ static void grad_one(const or_gwork* W, uint32_t ion, uint32_t slot)
             v[d] = fma(J->kq, E[d], v[d]);
             x[d] = fma(v[d], dt, x[d]);
         }
+        /* thermal collision of step n with frozen draws (the Philox branch of
+         * advance above): the candidate and acceptance decisions are taken on the
+         * primal velocity and held fixed, so the tangent goes through the map
+         * v' = v - mr g + mr |g| d, g = v - gas velocity, whose Jacobian is
+         * (1 - mr) I + mr d (g / |g|)^T (pathwise derivative). */
+        if (!C->compat && !C->legacy_coll && J->nsig_dt > 0.0) {
+            double bound = norm3(v) + J->v_bound;
+            uint32_t w[4];
+            blk(J->key, ion, n >> 2, 0, D_CAND, w);
+            if (u01(w[n & 3]) < J->nsig_dt * bound) {
+                blk(J->key, ion, n, 0, D_COLL, w);
+                double r1 = sqrt(-2.0 * or_log(u01(w[0]))), r2 = sqrt(-2.0 * or_log(u01(w[2])));
+                double s1, c1, s2, c2;
+                or_sincos2pi(u01(w[1]), &s1, &c1);
+                or_sincos2pi(u01(w[3]), &s2, &c2);
+                double gv[3] = {v[0] - J->s_gas * (r1 * c1), v[1] - J->s_gas * (r1 * s1), v[2] - J->s_gas * (r2 * c2)};
+                double gl = norm3(gv);
+                blk(J->key, ion, n, 1, D_COLL, w);
+                if (u01(w[0]) * bound < gl) {
+                    double c = 2.0 * u01(w[1]) - 1.0, sp, cp;
+                    or_sincos2pi(u01(w[2]), &sp, &cp);
+                    double sn = sqrt(fmax(0.0, 1.0 - c * c));
+                    double nn[3] = {sn * cp, sn * sp, c};
+                    double mg = J->mr * gl;
+                    for (int k = 0; k < NT; ++k) {
+                        double gd = (gv[0] * dv[k][0] + gv[1] * dv[k][1] + gv[2] * dv[k][2]) / gl;
+                        for (int d = 0; d < 3; ++d) dv[k][d] = (1.0 - J->mr) * dv[k][d] + J->mr * nn[d] * gd;
+                    }
+                    for (int d = 0; d < 3; ++d) v[d] = (v[d] - J->mr * gv[d]) + mg * nn[d];
+                }
+            }
+        }
     }
     /* objective and its gradient at the final position */
     const mq_objective* O = W->obj;
@@ -795,9 +827,18 @@ static void* grad_worker(void* arg)
     return NULL;
 }
 
+/* Collisions in the differentiable form: the thermal model on the Philox
+ * streams only (frozen draws). */
+static int grad_physics_ok(const mq_job* job, const mq_physics* ph)
+{
+    const int gas = ph->pressure_pa > 0.0 || job->pressure_pa > 0.0;
+    return !gas || (!ph->legacy_collisions && ph->rng == MQ_RNG_PHILOX);
+}
+
 int or_grad_reference(const mq_fields* f, const mq_job* job, const double dc[4], const mq_physics* ph,
                       const mq_objective* obj, int nthreads, double* Jout, double grad[5], double* grad_state)
 {
+    if (!grad_physics_ok(job, ph)) return -3;
     double* dcc = combine_dc(f, dc);
     if (!dcc) return -1;
     or_ctx C;
@@ -839,6 +880,7 @@ int or_grad_reference(const mq_fields* f, const mq_job* job, const double dc[4],
 int or_grad_ions(const mq_fields* f, const mq_job* job, const double dc[4], const mq_physics* ph,
                  const mq_objective* obj, int nthreads, const uint32_t* ions, uint32_t nions, double* Ji, double* gi)
 {
+    if (!grad_physics_ok(job, ph)) return -3;
     for (uint32_t s = 0; s < nions; ++s)
         if (ions[s] >= (uint32_t)job->n) return -2;
     double* dcc = combine_dc(f, dc);

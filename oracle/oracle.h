@@ -82,10 +82,14 @@ int or_run_range(const mq_fields* f, const mq_job* job, const double dc[4], cons
 void or_summarize(const or_ion* ions, const uint32_t* job, size_t n, size_t njobs, double dt, mq_outcome* out);
 
 /* Differentiable form, forward-mode reference: integrates job->n ions for
- * obj->nsteps collision-free FP64 steps carrying 11 tangent directions
+ * obj->nsteps FP64 steps without termination, carrying 11 tangent directions
  * (dual numbers): the parameters (V_rf, f_rf, V_ac, f_ac, dc_scale) and the
- * initial state (x, y, z, vx, vy, vz). Returns J (mean objective), grad[5] and,
- * if grad_state is not NULL, dJ/d(initial state) as 6 arrays of n. */
+ * initial state (x, y, z, vx, vy, vz). With gas (pressure > 0) the thermal
+ * collisions of the Philox streams are applied with frozen draws: decisions on
+ * the primal velocity, tangents through the collision map (pathwise
+ * derivative). Returns J (mean objective), grad[5] and, if grad_state is not
+ * NULL, dJ/d(initial state) as 6 arrays of n; -3 for legacy collisions or the
+ * COMPAT streams with gas. */
 int or_grad_reference(const mq_fields* f, const mq_job* job, const double dc[4], const mq_physics* ph,
                       const mq_objective* obj, int nthreads, double* J, double grad[5], double* grad_state);
 

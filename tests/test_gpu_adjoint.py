@@ -154,8 +154,33 @@ def test_adjoint_errors(c1):
     e.run_batch([job], [0, 5, 0, 0], ph)
     with pytest.raises(mqhip.MqError, match="preceding mq_forward"):
         e.backward()
-    with pytest.raises(mqhip.MqError, match="collision-free"):
-        e.forward(job, [0, 5, 0, 0], ph.copy(pressure_pa=0.1), obj)
+    with pytest.raises(mqhip.MqError, match="thermal collision model only"):
+        e.forward(job, [0, 5, 0, 0], ph.copy(pressure_pa=0.1, legacy_collisions=1), obj)
+
+
+@pytest.mark.parametrize("tables", ["double", "float"])
+def test_frozen_collisions_match_dual_numbers(c1, c1_float, oracle_mod, tables):
+    """Thermal N2 at 1 mTorr (a few collisions per ion over 2,000 steps): the adjoint with the Philox draws frozen
+    against the oracle's dual numbers through the same collisions (tests/test_oracle_grad.py pins those against
+    central differences), on the double node tables and on the float cell records; without gas the same job gives
+    another J and gradient."""
+    g, e = c1 if tables == "double" else c1_float
+    _, ph, obj = setting()
+    # the float records need an exactly-float combined DC: the gate voltage goes on the job there
+    dc, s_dc = ([0, 5, 0, 0], 1.0) if tables == "double" else ([0, 1, 0, 0], 5.0)
+    job = mqhip.job(499.0, 1.0, 230.0, 100.0, 500e3, 0.5, 100e3, 256, 1499, dc_scale=s_dc)
+    gas = ph.copy(pressure_pa=0.133322)
+    print(f"[gpu] adjoint with frozen collisions vs dual numbers ({tables} tables)", flush=True)
+    J = e.forward(job, dc, gas, obj)
+    grad, gstate = e.backward(1.0, state=True)
+    Jr, gr, gsr = oracle_mod.grad_reference(g, job, dc, gas, obj, 8)
+    J0 = e.forward(job, dc, ph, obj)
+    g0 = e.backward(1.0)
+    print(f"[gpu]   J {J:.15g} vs {Jr:.15g} (no gas {J0:.6g}); grad {grad} vs {gr}", flush=True)
+    assert J == pytest.approx(Jr, rel=1e-12)
+    assert np.allclose(grad, gr, rtol=1e-9, atol=1e-12 * np.abs(gr).max())
+    assert np.allclose(gstate, gsr, rtol=1e-8, atol=1e-10 * np.abs(gsr).max())
+    assert J != J0 and not np.allclose(grad, g0, rtol=1e-3)
 
 
 def test_adjoint_per_ion_at_full_size_matches_dual_numbers(oracle_mod):

@@ -146,3 +146,66 @@ def test_per_ion_subset_matches_batch(oracle_mod):
     assert np.array_equal(Js, Ji[sub]) and np.array_equal(gs, gi[:, sub])
     with pytest.raises(RuntimeError):
         oracle_mod.grad_ions(g, job, [0, 1, 0, 0], ph, obj, np.array([n]), 1)
+
+
+def _quad_grid():
+    r0, ax = 2.765e-3, 3.2e-3
+    return make_grid(32, 32, 64, 0.2e-3, rf=lambda X, Y, Z: (2 * (Y - ax) / r0**2, 2 * (X - ax) / r0**2, 0 * X),
+                     ac=lambda X, Y, Z: (0 * X - 1 / r0, 0 * X, 0 * X),
+                     dc1=lambda X, Y, Z: (0 * X, 0 * X, -(Z - 6e-3) * 100), axis=(ax, ax), z_inject=5e-3)
+
+
+def test_frozen_collisions_match_central_differences(oracle_mod):
+    """Thermal N2 at 1 mTorr (about five collisions per ion in 1,500 steps): with the Philox draws frozen, the
+    dual numbers carry tangents through each collision's velocity map, and central differences of the oracle's
+    own objective (same seeds) measure the same pathwise derivative, ion by ion (Richardson over h = 1e-5 and h/2
+    relative). An ion whose collision decisions flip between theta - h and theta + h shows as a difference
+    quotient that changes with h; such ions are set aside, at most two per parameter and one on average."""
+    g = _quad_grid()
+    ph = mqhip.Physics.correct(0.133322).copy(dt=1e-7, injection=mqhip.INJ_GAUSSIAN, inj_sigma_mm=0.1,
+                                              legacy_force=1)
+    n = 24
+    job = mqhip.job(499.0, 1.0, 230.0, 100.0, 500e3, 0.5, 100e3, n, 1499)
+    obj = mqhip.Objective(1500, 0, 6.0e-3, 2e-4, 1.0e-3, 0.3)
+    dc = [0, 1, 0, 0]
+    ions = np.arange(n)
+    Ji, gi = oracle_mod.grad_ions(g, job, dc, ph, obj, ions, 8)
+    # the same ions without gas: the collisions change the result
+    Jf, gf = oracle_mod.grad_ions(g, job, dc, ph.copy(pressure_pa=0.0), obj, ions, 8)
+    assert not np.allclose(gi, gf, rtol=1e-3)
+    kept = 0
+    for k, (grp, name) in enumerate([("drive", "v_rf"), ("drive", "f_rf"), ("drive", "v_ac"), ("drive", "f_ac"),
+                                     (None, "dc_scale")]):
+        base = getattr(job.drive if grp else job, name)
+
+        def central(h):
+            vals = []
+            for sgn in (+1, -1):
+                j2 = mqhip.Job()
+                j2.sp, j2.drive, j2.n, j2.seed, j2.dc_scale = job.sp, job.drive, job.n, job.seed, job.dc_scale
+                setattr(j2.drive if grp else j2, name, base + sgn * h)
+                vals.append(oracle_mod.grad_ions(g, j2, dc, ph, obj, ions, 8)[0])
+            return (vals[0] - vals[1]) / (2 * h)
+
+        h = 1e-5 * abs(base)
+        c1, c2 = central(h), central(h / 2)
+        rich = (4 * c2 - c1) / 3
+        scale = np.abs(gi[k]).max()
+        smooth = np.abs(c1 - c2) <= 1e-4 * scale
+        assert smooth.sum() >= n - 2, (name, int(smooth.sum()))
+        kept += int(smooth.sum())
+        err = np.abs(rich - gi[k])[smooth] / scale
+        assert err.max() <= 1e-5 and np.median(err) <= 1e-6, (name, err.max(), np.median(err))
+    assert kept >= 5 * (n - 1)
+
+
+def test_gradient_rejects_legacy_collisions_and_compat_streams_with_gas(oracle_mod):
+    g = _quad_grid()
+    job = mqhip.job(499.0, 1.0, 230.0, 100.0, 500e3, 0.5, 100e3, 4, 1499)
+    obj = mqhip.Objective(100, 0, 6.0e-3, 2e-4, 1.0e-3, 0.3)
+    ph = mqhip.Physics.correct(0.133322).copy(dt=1e-7, injection=mqhip.INJ_GAUSSIAN, inj_sigma_mm=0.1)
+    for bad in (ph.copy(legacy_collisions=1), ph.copy(rng=mqhip.RNG_COMPAT)):
+        with pytest.raises(RuntimeError):
+            oracle_mod.grad_reference(g, job, [0, 1, 0, 0], bad, obj, 2)
+    # without gas both are accepted (no collision is drawn)
+    oracle_mod.grad_reference(g, job, [0, 1, 0, 0], ph.copy(legacy_collisions=1, pressure_pa=0.0), obj, 2)
