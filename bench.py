@@ -159,6 +159,11 @@ def grad_bench(fd: bool, evals: int = 3):
     if fd:
         axis = (grid["axis_x"], grid["axis_y"])
         out["fd_check_h_rel_1e-6"] = fd_check(eng, job, dc, ph, obj, grad, axis)
+        # smaller steps keep even the most sensitive trajectories in their linear range at 20,000 steps
+        # (|dJ_i/dV_rf| ~ 2e5 per volt times a 1e-8 V step); FP64 leaves room for them: a phase f dt n of
+        # order 1e3 carries 1e-13 of rounding against a shift of 1e-7 at h = 1e-10
+        for hr in (1e-9, 1e-10, 1e-11, 1e-12):
+            out[f"fd_check_h_rel_{hr:g}"] = fd_check(eng, job, dc, ph, obj, grad, axis, hr)
         # the same operating point over 2,000 steps, where every trajectory stays in the
         # linear range of the step: all ions, no selection
         obj_s = mqhip.Objective(2000, 0, 18.6e-3, 2e-5, 1.0e-3, 0.3)
@@ -169,8 +174,8 @@ def grad_bench(fd: bool, evals: int = 3):
     return out
 
 
-def fd_check(eng, job, dc, ph, obj, grad, axis):
-    """Adjoint vs central differences (h = 1e-6 relative) and their Richardson extrapolation
+def fd_check(eng, job, dc, ph, obj, grad, axis, h_rel=1e-6):
+    """Adjoint vs central differences (h = h_rel relative, default 1e-6) and their Richardson extrapolation
     (h, h/2), over all ions and over the ions that the step keeps in its linear range: an ion
     whose final z or radius moves by more than 1 % of the objective's length scale between
     theta - h and theta + h is outside it (the step then samples the sigmoid's curvature, or
@@ -201,7 +206,7 @@ def fd_check(eng, job, dc, ph, obj, grad, axis):
 
     rel = {}
     for k, (grp, name, label) in enumerate(names):
-        h = 1e-6 * abs(getattr(job.drive if grp else job, name))
+        h = h_rel * abs(getattr(job.drive if grp else job, name))
         d1, p1, f1 = central(grp, name, h)
         d2, p2, _ = central(grp, name, h / 2)
         rich, prich = (4 * d2 - d1) / 3, (4 * p2 - p1) / 3
@@ -220,7 +225,7 @@ def fd_check(eng, job, dc, ph, obj, grad, axis):
                       "largest_ions": [{"ion": int(i), "adjoint": float(gi[k][i] * n),
                                         "central_h": float(p1[i] * n), "central_h2": float(p2[i] * n),
                                         "linear": bool(lin[i])} for i in top]}
-        log(f"gradient check {label}: adjoint {g:.8g} richardson {rich:.8g}; over the {lin.mean():.5f} of ions "
+        log(f"gradient check {label} (h {h_rel:g}): adjoint {g:.8g} richardson {rich:.8g}; over the {lin.mean():.5f} of ions "
             f"in the step's linear range: adjoint {adj_l:.10g} richardson {fd_l:.10g}")
     return rel
 
