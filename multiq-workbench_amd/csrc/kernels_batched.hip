@@ -28,7 +28,8 @@
 namespace mq {
 
 // ------------------------------------------------------------- cell table
-__global__ void k_build_cells(float4* __restrict__ cells, const double* __restrict__ rf,
+// 128-thread launches: the bound lets a thread keep the 72 FP64 corner values without spilling.
+__global__ __launch_bounds__(128) void k_build_cells(float4* __restrict__ cells, const double* __restrict__ rf,
                               const double* __restrict__ dc, const double* __restrict__ ac,
                               const uint8_t* __restrict__ mask, int nx, int ny, int nz)
 {
