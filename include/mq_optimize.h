@@ -47,6 +47,7 @@ typedef struct {
                                               frequency resolving power into the mass resolving power
                                               (the reference takes it from the Mathieu terrain); 0 means 1 */
     uint64_t seed;                         /* seed of the first drive; drive i uses seed + 2 i */
+    uint64_t leak_n;                       /* ions of the leakage species per point (0: none) */
 } mq_optimize_opts;
 
 /* Issues of a row (fit_quality, research.rs:67-75). */
@@ -67,22 +68,41 @@ typedef struct {
     uint64_t ions_per_point;
     mq_peak_fit fit;
     double r_mz, r_mz_err;
-    double f_best_khz, t_best, t_best_err;
+    double f_best_khz, t_best, t_best_err; /* t_best_err: Wilson-score error of t_k / t_n */
+    uint64_t t_k, t_n;                     /* transmitted and ions at f_best */
+    uint64_t leak_k, leak_n;               /* the leakage species at f_best (0, 0 without one) */
+    double leakage, leakage_err;           /* leak_k/leak_n over t_k/t_n (mq_ratio; NaN when undefined) */
+    double leakage_ul95;                   /* its 95 % upper limit (mq_ratio) */
     int32_t refined; /* 1: a narrow peak (FWHM < 2 kHz) was rescanned around its centre with 2n ions */
     int32_t issue;   /* MQ_OPT_* */
 } mq_optimize_row;
 
-/* experiment (research.rs:133-163) on the engine: for each of the n_ac AC
- * amplitudes, a scan of the AC frequency (one batch: one job of o->n ions per
- * frequency, drive rf with v_ac and f_ac replaced), a peak fit, and for a
- * narrow peak a rescan of 60 steps over centre -/+ clamp(3 FWHM, 2, 6) kHz
- * with 2n ions; then the eligible row (no issue, r_mz - 2 r_mz_err >= min_r)
- * with the largest t_best - 2 t_best_err (research.rs:101-107), and an
+/* Statistics of the Optimize rows (mqviz/src/stats.rs). mq_frac_sigma: the
+ * Wilson-score 1-sigma error of k/n, sqrt(p(1-p)/n + 1/(4 n^2)) / (1 + 1/n)
+ * (Frac::sigma, stats.rs:17-22; NaN for n = 0). mq_ratio: the ratio of two
+ * fractions (ka/na) / (kb/nb) with its log-delta error on Jeffreys-shifted
+ * counts (ratio, stats.rs:51-62) and the 95 % upper limit from the
+ * Wilson-Hilferty Poisson limit of ka (ratio_ul95 and poisson_ul95,
+ * stats.rs:29-33, 65-69); NaN where the reference returns NaN. Host only. */
+int mq_frac_sigma(uint64_t k, uint64_t n, double* sigma);
+int mq_ratio(uint64_t ka, uint64_t na, uint64_t kb, uint64_t nb, double* v, double* e, double* ul95);
+
+/* experiment (research.rs:133-163) on the engine, with research_start's
+ * input check (research.rs:166-167: min_r 5-150 and 500-5000 ions per coarse
+ * point). For each of the n_ac AC amplitudes: a scan of the AC frequency (one
+ * batch: one job of o->n ions per frequency, drive rf with v_ac and f_ac
+ * replaced, plus one job of o->leak_n ions of the leakage species leak per
+ * frequency when leak_n > 0; the reference draws both species from one live
+ * beam by abundance), a peak fit, and for a narrow peak a rescan of 60 steps
+ * over centre -/+ clamp(3 FWHM, 2, 6) kHz with 2n ions; each row reports the
+ * scan point nearest the fitted centre with its leakage (annotate_leakage,
+ * research.rs:88-95). Then the eligible row (no issue, r_mz - 2 r_mz_err >=
+ * min_r) with the largest t_best - 2 t_best_err (research.rs:96-107), and an
  * independent confirmation of it with 3n ions and fresh seeds. rows[n_ac] is
  * filled; *best = its index or -1; *confirmation and *confirmed (1 when the
  * confirmation is eligible as well) are written when *best >= 0. */
-int mq_optimize_grid(mq_engine* eng, const mq_species* sp, const mq_rfac* rf, const double dc[4],
-                     const mq_physics* ph, int32_t path, const double* v_ac, size_t n_ac,
+int mq_optimize_grid(mq_engine* eng, const mq_species* sp, const mq_species* leak, const mq_rfac* rf,
+                     const double dc[4], const mq_physics* ph, int32_t path, const double* v_ac, size_t n_ac,
                      const mq_optimize_opts* o, mq_optimize_row* rows, int32_t* best,
                      mq_optimize_row* confirmation, int32_t* confirmed);
 

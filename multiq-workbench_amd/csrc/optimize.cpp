@@ -177,47 +177,105 @@ mq_peak_fit fit_peak(const double* f, const double* t, const uint64_t* n, size_t
     return out;
 }
 
+// Frac::sigma (mqviz/src/stats.rs:17-22): Wilson-score 1-sigma error of k/n.
+double frac_sigma(uint64_t k, uint64_t n)
+{
+    if (n == 0) return NAN;
+    const double nn = (double)n, p = (double)k / nn;
+    return std::sqrt(p * (1.0 - p) / nn + 0.25 / (nn * nn)) / (1.0 + 1.0 / nn);
+}
+
+// poisson_ul95 (stats.rs:29-33): Wilson-Hilferty one-sided 95 % upper limit of a Poisson mean.
+double poisson_ul95(uint64_t k)
+{
+    const double a = (double)(k + 1);
+    const double c = 1.0 - 1.0 / (9.0 * a) + 1.644854 / (3.0 * std::sqrt(a));
+    return a * c * c * c;
+}
+
+// ratio and ratio_ul95 (stats.rs:51-69): (ka/na) / (kb/nb), log-delta error on Jeffreys-shifted counts.
+void ratio(uint64_t ka, uint64_t na, uint64_t kb, uint64_t nb, double& v, double& e, double& ul)
+{
+    if (na == 0 || nb == 0 || kb == 0) {
+        v = e = ul = NAN;
+        return;
+    }
+    const double pa = (double)ka / (double)na, pb = (double)kb / (double)nb;
+    const double vb = (1.0 - pb) / ((double)kb + 0.5);
+    if (ka == 0) {
+        v = 0.0;
+        e = (1.0 / (double)na) / pb * std::sqrt(1.0 + vb);
+    } else {
+        const double va = (1.0 - pa) / ((double)ka + 0.5);
+        v = pa / pb;
+        e = v * std::sqrt(va + vb);
+    }
+    ul = (poisson_ul95(ka) / (double)na) / pb * (1.0 + 1.645 * std::sqrt(vb));
+}
+
 uint64_t point_seed(uint64_t base, size_t k) { return base * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull * (k + 1); }
 
 // One scan of the AC frequency: a batch of one job per frequency.
-void scan(mq_engine* eng, const mq_species& sp, const mq_rfac& rf, double v_ac, const double dc[4],
-          const mq_physics& ph, int32_t path, double lo, double hi, double step, uint64_t n, uint64_t seed,
-          std::vector<double>& fs, std::vector<double>& ts, std::vector<uint64_t>& ns)
+struct Scan {
+    std::vector<double> f, t;
+    std::vector<uint64_t> k, n, lk, ln;  // transmitted / ions of the main and the leakage species per point
+};
+
+// One scan of the AC frequency: a batch of one job per frequency (and one of the
+// leakage species per frequency when leak_n > 0).
+void scan(mq_engine* eng, const mq_species& sp, const mq_species* leak, uint64_t leak_n, const mq_rfac& rf,
+          double v_ac, const double dc[4], const mq_physics& ph, int32_t path, double lo, double hi, double step,
+          uint64_t n, uint64_t seed, Scan& S)
 {
+    std::vector<double>& fs = S.f;
     fs.clear();
     for (int i = 0;; ++i) {
         const double f = lo + step * (double)i;
         if (!(f <= hi + 1e-9)) break;
         fs.push_back(f);
     }
-    std::vector<mq_job> jobs(fs.size());
-    for (size_t k = 0; k < fs.size(); ++k) {
-        mq_job& j = jobs[k];
+    const size_t m = fs.size();
+    const bool with_leak = leak && leak_n > 0;
+    std::vector<mq_job> jobs(with_leak ? 2 * m : m);
+    for (size_t q = 0; q < jobs.size(); ++q) {
+        const size_t k = q % m;
+        mq_job& j = jobs[q];
         j = mq_job{};
-        j.sp = sp;
+        j.sp = q < m ? sp : *leak;
         j.drive = rf;
         j.drive.v_ac = v_ac;
         j.drive.f_ac = fs[k] * 1e3;
-        j.n = n;
-        j.seed = point_seed(seed, k);
+        j.n = q < m ? n : leak_n;
+        j.seed = q < m ? point_seed(seed, k) : point_seed(seed ^ 0x1EA4ull, k);
         j.dc_scale = 1.0;
     }
     std::vector<mq_outcome> out(jobs.size());
     const int rc = mq_run_batch(eng, jobs.data(), jobs.size(), dc, &ph, path, out.data());
     if (rc != 0) throw OptErr(rc, mq_last_error());
-    ts.resize(fs.size());
-    ns.resize(fs.size());
-    for (size_t k = 0; k < fs.size(); ++k) {
-        ns[k] = out[k].n;
-        ts[k] = out[k].n ? (double)out[k].transmitted / (double)out[k].n : 0.0;
+    S.t.assign(m, 0.0);
+    S.k.assign(m, 0);
+    S.n.assign(m, 0);
+    S.lk.assign(m, 0);
+    S.ln.assign(m, 0);
+    for (size_t k = 0; k < m; ++k) {
+        S.n[k] = out[k].n;
+        S.k[k] = out[k].transmitted;
+        S.t[k] = out[k].n ? (double)out[k].transmitted / (double)out[k].n : 0.0;
+        if (with_leak) {
+            S.ln[k] = out[m + k].n;
+            S.lk[k] = out[m + k].transmitted;
+        }
     }
 }
 
 // The row summary of one scan (research.rs:76-95 and fit_quality :67-75).
-mq_optimize_row summarize_row(double v_ac, double lo, double hi, double step, uint64_t n, const std::vector<double>& fs,
-                              const std::vector<double>& ts, const std::vector<uint64_t>& ns, double dlnf)
+mq_optimize_row summarize_row(double v_ac, double lo, double hi, double step, uint64_t n, const Scan& S, double dlnf)
 {
+    const std::vector<double>& fs = S.f;
+    const std::vector<double>& ts = S.t;
+    const std::vector<uint64_t>& ns = S.n;
     mq_optimize_row r{};
+    r.leakage = r.leakage_err = r.leakage_ul95 = NAN;
     r.v_ac = v_ac;
     r.lo_khz = lo;
     r.hi_khz = fs.empty() ? hi : fs.back();
@@ -233,7 +291,12 @@ mq_optimize_row summarize_row(double v_ac, double lo, double hi, double step, ui
         if (std::fabs(fs[k] - r.fit.f0_khz) < std::fabs(fs[i] - r.fit.f0_khz)) i = k;
     r.f_best_khz = fs[i];
     r.t_best = ts[i];
-    r.t_best_err = ns[i] ? std::sqrt(ts[i] * (1.0 - ts[i]) / (double)ns[i]) : 0.0;
+    r.t_best_err = frac_sigma(S.k[i], ns[i]);
+    r.t_k = S.k[i];
+    r.t_n = ns[i];
+    r.leak_k = S.lk[i];
+    r.leak_n = S.ln[i];
+    if (r.leak_n) ratio(r.leak_k, r.leak_n, r.t_k, r.t_n, r.leakage, r.leakage_err, r.leakage_ul95);
     r.r_mz = r.fit.r_f * dlnf;
     r.r_mz_err = r.fit.r_f_err * dlnf;
     if (r.fit.fwhm_khz < 4.0 * step) r.issue = MQ_OPT_TOO_NARROW;
@@ -279,24 +342,42 @@ int mq_fit_peak(const double* f_khz, const double* t, const uint64_t* n, size_t 
     return 0;
 }
 
-int mq_optimize_grid(mq_engine* eng, const mq_species* sp, const mq_rfac* rf, const double dc[4],
-                     const mq_physics* ph, int32_t path, const double* v_ac, size_t n_ac,
+int mq_frac_sigma(uint64_t k, uint64_t n, double* sigma)
+{
+    if (!sigma) { mq_set_last_error("NULL argument"); return -1; }
+    *sigma = frac_sigma(k, n);
+    return 0;
+}
+
+int mq_ratio(uint64_t ka, uint64_t na, uint64_t kb, uint64_t nb, double* v, double* e, double* ul95)
+{
+    if (!v || !e || !ul95) { mq_set_last_error("NULL argument"); return -1; }
+    ratio(ka, na, kb, nb, *v, *e, *ul95);
+    return 0;
+}
+
+int mq_optimize_grid(mq_engine* eng, const mq_species* sp, const mq_species* leak, const mq_rfac* rf,
+                     const double dc[4], const mq_physics* ph, int32_t path, const double* v_ac, size_t n_ac,
                      const mq_optimize_opts* o, mq_optimize_row* rows, int32_t* best,
                      mq_optimize_row* confirmation, int32_t* confirmed)
 {
     try {
         if (!eng || !sp || !rf || !dc || !ph || !v_ac || !o || !rows || !best || !confirmation || !confirmed)
             throw OptErr(-1, "NULL argument");
+        if (o->leak_n > 0 && !leak) throw OptErr(-1, "leak_n > 0 needs the leakage species");
+        // research_start's check (research.rs:167)
+        if (!std::isfinite(o->min_r) || o->min_r < 5.0 || o->min_r > 150.0 || o->n < 500 || o->n > 5000)
+            throw OptErr(-1, "Choose resolving power 5-150 and 500-5000 ions per coarse point");
         if (n_ac == 0) throw OptErr(-1, "no AC amplitudes to search");
-        if (!(o->f_step_khz > 0.0) || !(o->f_hi_khz > o->f_lo_khz) || o->n == 0)
-            throw OptErr(-1, "bad scan: need f_hi > f_lo, f_step > 0 and n > 0");
+        if (!(o->f_step_khz > 0.0) || !(o->f_hi_khz > o->f_lo_khz))
+            throw OptErr(-1, "bad scan: need f_hi > f_lo and f_step > 0");
         const double dlnf = o->dlnf_dlnmz > 0.0 ? o->dlnf_dlnmz : 1.0;
-        std::vector<double> fs, ts;
-        std::vector<uint64_t> ns;
+        Scan S;
         for (size_t i = 0; i < n_ac; ++i) {
             uint64_t seed = o->seed + 2 * (uint64_t)i;
-            scan(eng, *sp, *rf, v_ac[i], dc, *ph, path, o->f_lo_khz, o->f_hi_khz, o->f_step_khz, o->n, seed, fs, ts, ns);
-            mq_optimize_row row = summarize_row(v_ac[i], o->f_lo_khz, o->f_hi_khz, o->f_step_khz, o->n, fs, ts, ns, dlnf);
+            scan(eng, *sp, leak, o->leak_n, *rf, v_ac[i], dc, *ph, path, o->f_lo_khz, o->f_hi_khz, o->f_step_khz, o->n,
+                 seed, S);
+            mq_optimize_row row = summarize_row(v_ac[i], o->f_lo_khz, o->f_hi_khz, o->f_step_khz, o->n, S, dlnf);
             if (row.fit.converged && row.fit.fwhm_khz < 2.0) {
                 // a narrow peak: resolve it around its centre (research.rs:143-150)
                 const double halfw = std::min(std::max(3.0 * row.fit.fwhm_khz, 2.0), 6.0);
@@ -304,8 +385,8 @@ int mq_optimize_grid(mq_engine* eng, const mq_species* sp, const mq_rfac* rf, co
                 const double hi = std::min(row.fit.f0_khz + halfw, o->f_hi_khz);
                 const double step = (hi - lo) / 60.0;
                 seed += 100;
-                scan(eng, *sp, *rf, v_ac[i], dc, *ph, path, lo, hi, step, 2 * o->n, seed, fs, ts, ns);
-                row = summarize_row(v_ac[i], lo, hi, step, 2 * o->n, fs, ts, ns, dlnf);
+                scan(eng, *sp, leak, 2 * o->leak_n, *rf, v_ac[i], dc, *ph, path, lo, hi, step, 2 * o->n, seed, S);
+                row = summarize_row(v_ac[i], lo, hi, step, 2 * o->n, S, dlnf);
                 row.refined = 1;
             }
             rows[i] = row;
@@ -323,9 +404,9 @@ int mq_optimize_grid(mq_engine* eng, const mq_species* sp, const mq_rfac* rf, co
         if (b >= 0) {
             // independent confirmation with three times the ions and fresh seeds (research.rs:154-160)
             const mq_optimize_row& r = rows[b];
-            scan(eng, *sp, *rf, r.v_ac, dc, *ph, path, r.lo_khz, r.hi_khz, r.step_khz, 3 * o->n,
-                 o->seed + 0x5EED0000ull, fs, ts, ns);
-            *confirmation = summarize_row(r.v_ac, r.lo_khz, r.hi_khz, r.step_khz, 3 * o->n, fs, ts, ns, dlnf);
+            scan(eng, *sp, leak, 3 * o->leak_n, *rf, r.v_ac, dc, *ph, path, r.lo_khz, r.hi_khz, r.step_khz, 3 * o->n,
+                 o->seed + 0x5EED0000ull, S);
+            *confirmation = summarize_row(r.v_ac, r.lo_khz, r.hi_khz, r.step_khz, 3 * o->n, S, dlnf);
             *confirmed = eligible(*confirmation, o->min_r) ? 1 : 0;
         }
         return 0;

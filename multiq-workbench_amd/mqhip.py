@@ -203,13 +203,15 @@ class PeakFit(C.Structure):
 
 class OptimizeOpts(C.Structure):
     _fields_ = [("f_lo_khz", C.c_double), ("f_hi_khz", C.c_double), ("f_step_khz", C.c_double), ("n", C.c_uint64),
-                ("min_r", C.c_double), ("dlnf_dlnmz", C.c_double), ("seed", C.c_uint64)]
+                ("min_r", C.c_double), ("dlnf_dlnmz", C.c_double), ("seed", C.c_uint64), ("leak_n", C.c_uint64)]
 
 
 class OptimizeRow(C.Structure):
     _fields_ = [("v_ac", C.c_double), ("lo_khz", C.c_double), ("hi_khz", C.c_double), ("step_khz", C.c_double),
                 ("ions_per_point", C.c_uint64), ("fit", PeakFit), ("r_mz", C.c_double), ("r_mz_err", C.c_double),
                 ("f_best_khz", C.c_double), ("t_best", C.c_double), ("t_best_err", C.c_double),
+                ("t_k", C.c_uint64), ("t_n", C.c_uint64), ("leak_k", C.c_uint64), ("leak_n", C.c_uint64),
+                ("leakage", C.c_double), ("leakage_err", C.c_double), ("leakage_ul95", C.c_double),
                 ("refined", C.c_int32), ("issue", C.c_int32)]
 
     def as_dict(self) -> dict:
@@ -281,7 +283,11 @@ SIGNATURES = {
                                  C.POINTER(C.c_uint8)]),
     "mq_fit_peak": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_size_t,
                               C.POINTER(PeakFit)]),
-    "mq_optimize_grid": (C.c_int, [C.c_void_p, C.POINTER(Species), C.POINTER(RfAc), C.POINTER(C.c_double),
+    "mq_frac_sigma": (C.c_int, [C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]),
+    "mq_ratio": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
+                           C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mq_optimize_grid": (C.c_int, [C.c_void_p, C.POINTER(Species), C.POINTER(Species), C.POINTER(RfAc),
+                                   C.POINTER(C.c_double),
                                    C.POINTER(Physics), C.c_int32, C.POINTER(C.c_double), C.c_size_t,
                                    C.POINTER(OptimizeOpts), C.POINTER(OptimizeRow), C.POINTER(C.c_int32),
                                    C.POINTER(OptimizeRow), C.POINTER(C.c_int32)]),
@@ -628,16 +634,31 @@ def fit_peak(f_khz, t, n) -> dict:
     return {k: getattr(out, k) for k, _ in PeakFit._fields_}
 
 
+def frac_sigma(k: int, n: int) -> float:
+    """Wilson-score 1-sigma error of k/n (mq_frac_sigma; mqviz stats.rs:17-22)."""
+    v = C.c_double()
+    _check(load().mq_frac_sigma(k, n, C.byref(v)))
+    return v.value
+
+
+def ratio(ka: int, na: int, kb: int, nb: int) -> tuple[float, float, float]:
+    """(ka/na)/(kb/nb), its error and 95 % upper limit (mq_ratio; mqviz stats.rs:51-69)."""
+    v, e, u = C.c_double(), C.c_double(), C.c_double()
+    _check(load().mq_ratio(ka, na, kb, nb, C.byref(v), C.byref(e), C.byref(u)))
+    return v.value, e.value, u.value
+
+
 def optimize_grid(eng: "Engine", sp: Species, rf: RfAc, dc, ph: Physics, v_ac, f_lo_khz: float, f_hi_khz: float,
                   f_step_khz: float, n: int, min_r: float, seed: int, dlnf_dlnmz: float = 0.0,
-                  path: int = PATH_BATCHED) -> dict:
+                  path: int = PATH_BATCHED, leak: Species | None = None, leak_n: int = 0) -> dict:
     """The Optimize grid search (research.rs:133-163) on the engine (mq_optimize_grid)."""
     va = np.ascontiguousarray(v_ac, np.float64)
-    o = OptimizeOpts(f_lo_khz, f_hi_khz, f_step_khz, n, min_r, dlnf_dlnmz, seed)
+    o = OptimizeOpts(f_lo_khz, f_hi_khz, f_step_khz, n, min_r, dlnf_dlnmz, seed, leak_n)
     rows = (OptimizeRow * len(va))()
     conf = OptimizeRow()
     best, confirmed = C.c_int32(), C.c_int32()
-    _check(load().mq_optimize_grid(eng._h, C.byref(sp), C.byref(rf), (C.c_double * 4)(*dc), C.byref(ph), path,
+    _check(load().mq_optimize_grid(eng._h, C.byref(sp), C.byref(leak) if leak is not None else None, C.byref(rf),
+                                   (C.c_double * 4)(*dc), C.byref(ph), path,
                                    _dptr(va), len(va), C.byref(o), rows, C.byref(best), C.byref(conf),
                                    C.byref(confirmed)))
     return {"rows": [r.as_dict() for r in rows], "best": best.value,

@@ -4,7 +4,9 @@
   AC-frequency scans of 70-100 kHz at 0.5 kHz with 500 ions per point, RF 70 V at 500 kHz, DC [18, 24, 15, 20],
   N2 0.1 Pa, t_max 3 ms. The chosen row is the one the reference's rule (eligible, then the largest T - 2 sigma,
   restated in tests/test_optimize.py) picks from the returned rows; the confirmation's centre agrees with the
-  chosen row's within 3 combined standard errors; its flag is the rule applied to it.
+  chosen row's within 3 combined standard errors; its flag is the rule applied to it. Each row's scan point
+  carries its counts, the Wilson error and the leakage of the Na adduct (500 ions per point of its own) with the
+  reference's ratio statistics; out-of-range requests fail with research_start's message.
 * Gradient search: projected gradient ascent of the differentiable objective over f_exc, from inside the peak,
   ends within one grid step of the best point of a grid of forward evaluations of the same objective (2,000
   collision-free steps, where the adjoint agrees with central differences).
@@ -36,8 +38,9 @@ def test_grid_search_on_the_s14_channel():
         rf = mqhip.RfAc(70.0, 500e3, 0.35, 84.25e3)
         ph = mqhip.Physics.correct(0.1).copy(t_max=3e-3)
         min_r = 5.0
+        na = mqhip.Species(2 * 863.473 + 21.982, 2.0, 430.0)   # the Na adduct (Na for H): the leakage species
         res = mqhip.optimize_grid(e, sp, rf, [18, 24, 15, 20], ph, [0.35, 0.2625, 0.175, 0.0875], 70.0, 100.0, 0.5,
-                                  500, min_r, 0xA1100101)
+                                  500, min_r, 0xA1100101, leak=na, leak_n=500)
         for r in res["rows"]:
             fit = r["fit"]
             say(f"[gpu]   AC {r['v_ac']:.4f} V: f0 {fit['f0_khz']:.2f} kHz FWHM {fit['fwhm_khz']:.2f} R "
@@ -56,8 +59,26 @@ def test_grid_search_on_the_s14_channel():
         if c["fit"]["converged"]:
             se = np.hypot(b["fit"]["f0_err"], c["fit"]["f0_err"])
             assert abs(b["fit"]["f0_khz"] - c["fit"]["f0_khz"]) <= 3 * se + 1e-9
-        for r in res["rows"]:
+        for r in res["rows"] + [c]:
             assert 70.0 <= r["lo_khz"] < r["hi_khz"] <= 100.0 + 1e-9
+            if not r["fit"]["converged"]:
+                continue
+            # the scan point's counts and statistics (stats.rs:17-22, 51-69; annotate_leakage, research.rs:88-95)
+            assert r["t_n"] == r["ions_per_point"] and r["t_best"] == r["t_k"] / r["t_n"]
+            assert r["t_best_err"] == mqhip.frac_sigma(r["t_k"], r["t_n"])
+            assert r["leak_n"] == r["ions_per_point"] and 0 <= r["leak_k"] <= r["leak_n"]
+            want = mqhip.ratio(r["leak_k"], r["leak_n"], r["t_k"], r["t_n"])
+            got = (r["leakage"], r["leakage_err"], r["leakage_ul95"])
+            assert all((np.isnan(a) and np.isnan(b)) or a == b for a, b in zip(got, want))
+            say(f"[gpu]   AC {r['v_ac']:.4f} V leakage {r['leak_k']}/{r['leak_n']}: {r['leakage']:.4g} "
+                f"+- {r['leakage_err']:.2g}, 95 % UL {r['leakage_ul95']:.3g}")
+        # research_start's check (research.rs:167)
+        for bad in (dict(n=499), dict(n=5001), dict(min_r=4.9), dict(min_r=151.0)):
+            kw = dict(n=500, min_r=5.0) | bad
+            with pytest.raises(mqhip.MqError, match="Choose resolving power 5-150 and 500-5000"):
+                mqhip.optimize_grid(e, sp, rf, [18, 24, 15, 20], ph, [0.35], 70.0, 100.0, 0.5, kw["n"], kw["min_r"], 1)
+        with pytest.raises(mqhip.MqError, match="leakage species"):
+            mqhip.optimize_grid(e, sp, rf, [18, 24, 15, 20], ph, [0.35], 70.0, 100.0, 0.5, 500, 5.0, 1, leak_n=10)
     finally:
         e.close()
 

@@ -67,3 +67,53 @@ def best_row(rows, min_r):
     """research.rs:101-107: the eligible row with the largest transmission minus two standard errors."""
     cand = [(r["t_best"] - 2 * r["t_best_err"], i) for i, r in enumerate(rows) if eligible(r, min_r)]
     return max(cand)[1] if cand else -1
+
+
+# ---- the statistics of the rows (mq_frac_sigma, mq_ratio; mqviz/src/stats.rs), pinned as the reference's own
+# tests pin them (stats.rs:170-240): interval coverage by simulation, the Poisson limit's level, the boundaries.
+
+def test_wilson_error_boundaries_and_large_n():
+    import math
+    assert mqhip.frac_sigma(0, 50) > 0.0 and mqhip.frac_sigma(50, 50) > 0.0
+    assert abs(mqhip.frac_sigma(215000, 1000000) - math.sqrt(0.215 * 0.785 / 1e6)) < 1e-6
+    assert math.isnan(mqhip.frac_sigma(0, 0))
+
+
+@pytest.mark.parametrize("p", [0.02, 0.2, 0.5, 0.9])
+def test_wilson_error_coverage(p):
+    rng = np.random.default_rng(int(p * 1000) + 3)
+    n, reps = 400, 4000
+    k = rng.binomial(n, p, reps)
+    sig = np.array([mqhip.frac_sigma(int(x), n) for x in k])
+    c1 = np.mean(np.abs(k / n - p) <= sig)
+    c2 = np.mean(np.abs(k / n - p) <= 2 * sig)
+    assert abs(c1 - 0.6827) < 0.04 and abs(c2 - 0.9545) < 0.03, (c1, c2)
+
+
+def test_poisson_upper_limit_is_95_percent():
+    from scipy.stats import poisson
+    for k in range(0, 30):
+        v, e, ul = mqhip.ratio(k, 1000, 1000, 1000)   # denominator exactly 1: ul = poisson_ul95(k) / 1000
+        lim = ul * 1000
+        assert abs(poisson.cdf(k, lim) - 0.05) < 0.012, (k, lim)
+    assert abs(mqhip.ratio(0, 1, 1, 1)[2] - 2.97) < 0.01
+
+
+def test_ratio_coverage_and_branches():
+    import math
+    rng = np.random.default_rng(21)
+    na, nb, pa, pb, reps = 20000, 8000, 0.02, 0.2, 6000
+    ka, kb = rng.binomial(na, pa, reps), rng.binomial(nb, pb, reps)
+    hit = np.mean([abs(mqhip.ratio(int(a), na, int(b), nb)[0] - pa / pb) <= mqhip.ratio(int(a), na, int(b), nb)[1]
+                   for a, b in zip(ka, kb)])
+    assert abs(hit - 0.6827) < 0.05, hit
+    # no numerator count: the value 0 with a one-count error; no denominator count: undefined
+    v, e, ul = mqhip.ratio(0, 500, 100, 1000)
+    assert v == 0.0 and e > 0.0 and ul > e
+    assert all(math.isnan(x) for x in mqhip.ratio(3, 500, 0, 1000))
+    # the upper limit covers the true ratio at least 95 % of the time, one-sided (the numerator's Poisson limit
+    # and the denominator's term add linearly, so it errs on the safe side)
+    for pa, na in ((0.002, 20000), (0.0002, 20000)):
+        ka, kb = rng.binomial(na, pa, reps), rng.binomial(nb, pb, reps)
+        cov = np.mean([pa / pb <= mqhip.ratio(int(a), na, int(b), nb)[2] for a, b in zip(ka, kb)])
+        assert 0.94 <= cov <= 0.999, (pa, cov)
