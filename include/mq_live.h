@@ -69,10 +69,61 @@ int mq_live_inject(mq_live* l, uint32_t species, uint32_t n);
 /* Runs nblocks blocks; *out (optional) receives the counters after them. */
 int mq_live_run(mq_live* l, uint32_t nblocks, mq_live_state* out);
 
-/* A new AC frequency [Hz] from the current step on, phase continuous, and the
- * tag of the ions spawned from now on (the sweep point of Sim::block,
- * engine.rs:518-533). */
+/* A new AC frequency [Hz] of the live tone from the current step on, phase
+ * continuous (the Params update of Sim::apply_params, engine.rs:399), and the
+ * tag of the ions spawned from now on. While a sweep runs the live tone stays
+ * with the sweep and takes the new frequency when the sweep stops. */
 int mq_live_set_ac(mq_live* l, double f_ac, int32_t tag);
+
+/* A frequency sweep of the population (Sim::start_sweep, stop_sweep and the
+ * sweep bookkeeping of Sim::block, engine.rs:420-432, 512-535). The points are
+ * f_lo_khz + i step_khz for i = 0 .. round((f_hi_khz - f_lo_khz) / step_khz),
+ * 101 at most (one when f_hi_khz < f_lo_khz). Each point has its own AC tone,
+ * a clock of its frequency whose phase runs on from the previous sweep's tone
+ * of the same index, or starts at 0 where that sweep had fewer points (the
+ * clock's resize, engine.rs:426); every ion spawned (fresh, injected or
+ * re-entered) while the sweep runs and does not drain carries the current
+ * point's tone for its whole life, and its outcome is counted for that point.
+ * The tone of untagged ions (the live tone) follows the current point while
+ * the sweep runs. Each fresh ion of the block's Poisson draw (mq_live_inject's
+ * do not count) advances the point after ions_per_point of them; after
+ * `sweeps` passes over the points the sweep drains: fresh injection stops
+ * until the last tagged ion has ended, then the sweep stops, the live tone
+ * returns to the AC frequency last set (mq_live_create or mq_live_set_ac) and
+ * injection resumes. The reference replaces a running sweep on a new start;
+ * here mq_live_sweep_start fails while a sweep runs or drains. */
+typedef struct {
+    double f_lo_khz, f_hi_khz, step_khz;
+    uint32_t ions_per_point; /* fresh ions per point (>= 1) */
+    uint32_t sweeps;         /* passes over the points (>= 1) */
+} mq_live_sweep_cfg;
+
+/* The state of the last sweep (sweep_view, engine.rs:434-446). */
+typedef struct {
+    uint32_t npoints;     /* 0: no sweep was started */
+    uint32_t point;       /* current point (the last one once the sweep has stopped) */
+    uint32_t sweeps_done; /* completed passes */
+    int32_t running;      /* 1 while the sweep runs or drains */
+    int32_t draining;     /* 1 while fresh injection is off and the tagged ions finish */
+    int32_t pad_;
+    double progress;      /* (sweeps_done npoints + point) / (sweeps npoints), at most 1 */
+} mq_live_sweep_state;
+
+#define MQ_LIVE_MAX_POINTS 101
+
+int mq_live_sweep_start(mq_live* l, const mq_live_sweep_cfg* c);
+
+/* Stops fresh injection; the sweep ends once its tagged ions have ended
+ * (stop_sweep, engine.rs:431-432). As there, a sweep that has already ended is
+ * marked running and draining again, so the next block draws no fresh ions and
+ * ends it. No-op before the first sweep. */
+int mq_live_sweep_stop(mq_live* l);
+
+/* The last sweep's state, its frequencies [kHz] (npoints) and its counts
+ * acc[point][species][4] of ended tagged ions: transmitted, crashed,
+ * returned, other (left the grid or timed out). Any output pointer may be
+ * NULL; f_khz and acc hold MQ_LIVE_MAX_POINTS points at most. */
+int mq_live_sweep(const mq_live* l, mq_live_sweep_state* st, double* f_khz, uint64_t* acc);
 
 /* Outcomes of the ions of one tag that have ended, one mq_outcome per species
  * (n, classes, mean collisions, mean time). */

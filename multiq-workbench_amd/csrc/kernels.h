@@ -18,13 +18,14 @@ void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t st);
 // Live population (mq_live_*): one new ion: its slot, job (species) and random-stream number.
 struct LiveNew {
     uint32_t slot, job, sid, pad;
+    double pha;  // AC phase of the ion's tone at its birth [cycles]
 };
 void launch_live_inject(const IonState& S, const JobP* jobs, const InjP& P, const LiveNew* rec, uint32_t k, double phr,
-                        double pha, uint32_t* list, uint32_t* count, hipStream_t st);
+                        uint32_t* list, uint32_t* count, hipStream_t st);
 void launch_live_collect(const IonState& S, const uint32_t* list, const uint32_t* count, uint32_t nmax, uint32_t* out,
                          uint32_t* nout, hipStream_t st);
 void launch_live_rephase(const IonState& S, const uint32_t* list, const uint32_t* count, uint32_t nmax, double pha,
-                         double w_ac, hipStream_t st);
+                         double w_ac, uint32_t job_end, hipStream_t st);
 
 // Batched path.
 void launch_build_cells(float4* cells, const double* rf, const double* dc, const double* ac, const uint8_t* mask,

@@ -108,18 +108,18 @@ __global__ void k_inject(IonState S, const JobP* __restrict__ jobs, uint32_t njo
 // ------------------------------------------------------- live population
 // New ions of a live population (the injection of mqviz's Sim::block,
 // engine.rs:512-535): ion k goes to slot rec[k].slot with the initial state of
-// ion rec[k].sid of job (species) rec[k].job, the RF phase of the running clock
-// at its birth and the clock's AC phase; its slot is appended to the live list.
+// ion rec[k].sid of job rec[k].job (a species on one AC tone), the RF phase of
+// the running clock at its birth and its tone's phase rec[k].pha; its slot is
+// appended to the live list.
 __global__ void k_live_inject(IonState S, const JobP* __restrict__ jobs, InjP P, const LiveNew* __restrict__ rec,
-                              uint32_t k, double phr, double pha, uint32_t* __restrict__ list,
-                              uint32_t* __restrict__ count)
+                              uint32_t k, double phr, uint32_t* __restrict__ list, uint32_t* __restrict__ count)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= k) return;
     const LiveNew r = rec[t];
     inject_one(S, jobs[r.job], P, r.sid, r.slot, r.job);
     S.phr[r.slot] = phr;
-    S.phi[r.slot] = pha;
+    S.phi[r.slot] = r.pha;
     S.sid[r.slot] = r.sid;
     list[count[0] + t] = r.slot;
 }
@@ -145,15 +145,17 @@ __global__ void k_live_collect(IonState S, const uint32_t* __restrict__ list, co
     out[4 * k + 3] = S.ncoll[id];
 }
 
-// A new AC frequency on the running clock (Clock, world.rs:437-465, whose
-// phase is continuous): every live ion's AC phase offset so that its phase at
-// its current step equals the clock's phase `pha` and then advances by w_ac.
+// A new frequency of one AC tone of the running clock (Clock, world.rs:437-465,
+// whose phases are continuous): the AC phase offset of every live ion on that
+// tone (job below job_end) so that its phase at its current step equals the
+// tone's phase `pha` and then advances by w_ac.
 __global__ void k_live_rephase(IonState S, const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
-                               double pha, double w_ac)
+                               double pha, double w_ac, uint32_t job_end)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count[0]) return;
     const uint32_t id = list[i];
+    if (S.job[id] >= job_end) return;
     const double p = pha - w_ac * (double)S.steps[id];
     S.phi[id] = p - floor(p);
 }
@@ -405,10 +407,10 @@ void launch_inject(const IonState& S, const JobP* jobs, uint32_t njobs, const In
 }
 
 void launch_live_inject(const IonState& S, const JobP* jobs, const InjP& P, const LiveNew* rec, uint32_t k, double phr,
-                        double pha, uint32_t* list, uint32_t* count, hipStream_t st)
+                        uint32_t* list, uint32_t* count, hipStream_t st)
 {
     if (!k) return;
-    k_live_inject<<<nblk(k, 256), 256, 0, st>>>(S, jobs, P, rec, k, phr, pha, list, count);
+    k_live_inject<<<nblk(k, 256), 256, 0, st>>>(S, jobs, P, rec, k, phr, list, count);
     k_live_count<<<1, 1, 0, st>>>(count, k);
 }
 
@@ -419,9 +421,9 @@ void launch_live_collect(const IonState& S, const uint32_t* list, const uint32_t
 }
 
 void launch_live_rephase(const IonState& S, const uint32_t* list, const uint32_t* count, uint32_t nmax, double pha,
-                         double w_ac, hipStream_t st)
+                         double w_ac, uint32_t job_end, hipStream_t st)
 {
-    if (nmax) k_live_rephase<<<nblk(nmax, 256), 256, 0, st>>>(S, list, count, pha, w_ac);
+    if (nmax) k_live_rephase<<<nblk(nmax, 256), 256, 0, st>>>(S, list, count, pha, w_ac, job_end);
 }
 
 void launch_combine_dc(double* out, const double* const d[4], const double v[4], uint32_t n3, hipStream_t st)

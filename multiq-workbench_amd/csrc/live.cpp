@@ -4,9 +4,11 @@
 //
 // Device: a pool of `capacity` ion slots (structure of arrays, as the engine's
 // batches), two ping-pong live lists of slot ids with their counter blocks,
-// and the finished-ion records of the last block. Host: the free slots, each
-// slot's species, tag and birth step, the xorshift64 stream of the block's
-// draws, the re-entry queue and the tallies.
+// the finished-ion records of the last block and the job table: one job per
+// (AC tone, species), tone 0 the live tone and tone p + 1 the clock of sweep
+// point p, job = tone * nsp + species. Host: the free slots, each slot's
+// species, tone, tag and birth step, the tones' phases, the xorshift64 stream
+// of the block's draws, the re-entry queue, the tallies and the sweep.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -72,7 +74,7 @@ struct mq_live {
     mq_live_opts o{};
     uint32_t nsp = 0, cap = 0;
     std::vector<double> cdf;
-    std::vector<JobP> jp;
+    std::vector<JobP> jp;  // [tone][species]
     InjP ip{};
     // device
     DevBuf x, y, z, phi, phr, vx, vy, vz, fvx, fvy, fvz, steps, ncoll, cd, job, status, sid;
@@ -81,12 +83,14 @@ struct mq_live {
     int cur = 0;
     // host
     uint64_t step = 0;
-    double pha = 0.0;  // AC phase of the clock at `step` [cycles]
+    std::vector<double> tone_ph;  // phase of each AC tone at `step` [cycles]
+    double f_live = 0.0;          // the live tone's frequency outside a sweep [Hz]
     uint64_t rng = 1;
     uint32_t next_id = 0;
     int32_t tag = 0;
     uint32_t fresh = 0;  // slots [fresh, cap) were never used; they are taken before recycled ones
     std::vector<uint32_t> free_slots, slot_sp, slot_sid;
+    std::vector<uint8_t> slot_tone;
     std::vector<int32_t> slot_tag;
     std::vector<uint64_t> slot_birth;
     std::priority_queue<std::pair<uint64_t, uint32_t>, std::vector<std::pair<uint64_t, uint32_t>>,
@@ -94,6 +98,15 @@ struct mq_live {
         returns;  // (due step, species)
     std::vector<uint64_t> tally;  // [tag][species][8]: n, classes 1..5, steps, collisions
     mq_live_state st{};
+    // the sweep (SweepRun, engine.rs:420-432): points [kHz], counts [point][species][4]
+    struct Sweep {
+        mq_live_sweep_cfg cfg{};
+        std::vector<double> f;
+        std::vector<uint64_t> acc;
+        uint32_t point = 0, inj_in_point = 0, sweeps_done = 0;
+        bool running = false, draining = false;
+    } sw;
+    uint64_t tagged_alive = 0;  // live ions on a sweep tone
     std::vector<LiveNew> pending;
     std::vector<uint32_t> fin_h;
 
@@ -143,7 +156,7 @@ struct mq_live {
         for (DevBuf* b : {&x, &y, &z, &phi, &phr, &vx, &vy, &vz}) b->ensure(c * 8);
         for (DevBuf* b : {&fvx, &fvy, &fvz, &steps, &ncoll, &cd, &job, &sid, &list[0], &list[1]}) b->ensure(c * 4);
         status.ensure(c);
-        jobs.ensure(nsp * sizeof(JobP));
+        jobs.ensure((size_t)nsp * (1 + MQ_LIVE_MAX_POINTS) * sizeof(JobP));
         ctr.ensure(2 * CTR_BLOCK * 4);
         fin.ensure(c * 16);
         nfin.ensure(4);
@@ -164,10 +177,12 @@ struct mq_live {
         free_slots.clear();
         slot_sp.assign(cap, 0);
         slot_sid.assign(cap, 0);
+        slot_tone.assign(cap, 0);
         slot_tag.assign(cap, 0);
         slot_birth.assign(cap, 0);
         tally.assign((size_t)o.max_tags * nsp * 8, 0);
-        pha = 0.0;
+        tone_ph.assign(1, 0.0);
+        f_live = rf.f_ac;
     }
 
     void destroy()
@@ -177,7 +192,8 @@ struct mq_live {
             b->release();
     }
 
-    // spawn_ion (engine.rs:372-382): the next free slot, or a drop at capacity.
+    // spawn_ion (engine.rs:372-382): the next free slot, or a drop at capacity;
+    // while a sweep runs and does not drain, the ion is on the current point's tone.
     void spawn(uint32_t k, bool reentry)
     {
         uint32_t slot;
@@ -191,16 +207,19 @@ struct mq_live {
             return;
         }
         const uint32_t s = next_id++;
-        pending.push_back(LiveNew{slot, k, s, 0});
+        const uint32_t tone = (sw.running && !sw.draining) ? sw.point + 1 : 0;
+        pending.push_back(LiveNew{slot, tone * nsp + k, s, 0, tone_ph[tone]});
         slot_sp[slot] = k;
         slot_sid[slot] = s;
+        slot_tone[slot] = (uint8_t)tone;
+        if (tone) tagged_alive += 1;
         slot_tag[slot] = tag;
         slot_birth[slot] = step;
         if (reentry) st.reentered += 1;
         else st.injected += 1;
     }
 
-    // The pending spawns enter the live list with the clock's RF and AC phases.
+    // The pending spawns enter the live list with the clock's RF phase and their tones' phases.
     void flush()
     {
         const uint32_t k = (uint32_t)pending.size();
@@ -208,7 +227,7 @@ struct mq_live {
         hipStream_t s = V.stream;
         HIPCHK(hipMemcpyAsync(rec.p, pending.data(), k * sizeof(LiveNew), hipMemcpyHostToDevice, s));
         const double phr_birth = frac(jp[0].w_rf * (double)step);
-        launch_live_inject(S, jobs.as<JobP>(), ip, rec.as<LiveNew>(), k, phr_birth, pha, list[cur].as<uint32_t>(),
+        launch_live_inject(S, jobs.as<JobP>(), ip, rec.as<LiveNew>(), k, phr_birth, list[cur].as<uint32_t>(),
                            ctr_of(cur) + CTR_COUNT, s);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s));  // `pending` is reused
@@ -220,7 +239,7 @@ struct mq_live {
     void block()
     {
         // the engine may have run other DC configurations since: rebuild its tables for ours if so
-        V = engine_view(eng, dc, ph, nsp);
+        V = engine_view(eng, dc, ph, (uint32_t)jp.size());
         const uint32_t B = o.block_steps;
         const uint64_t alive0 = st.alive;
         hipStream_t s = V.stream;
@@ -285,6 +304,12 @@ struct mq_live {
                 c[6] += stp;
                 c[7] += nc;
             }
+            if (const uint32_t tone = slot_tone[slot]) {
+                tagged_alive -= 1;
+                // engine.rs:491-496: transmitted, crashed, returned, other
+                const int cls = status == MQ_TRANSMITTED ? 0 : status == MQ_CRASHED ? 1 : status == MQ_RETURNED ? 2 : 3;
+                if (tone - 1 < sw.f.size()) sw.acc[((size_t)(tone - 1) * nsp + slot_sp[slot]) * 4 + cls] += 1;
+            }
             if (status == MQ_RETURNED && o.recirculate) {
                 const double u = std::max(xs_u01(rng), 1e-12);
                 const uint64_t delay = (uint64_t)std::ceil(-std::log(u) * o.residence_s / ph.dt);
@@ -296,35 +321,94 @@ struct mq_live {
         st.alive -= nf;
         st.ion_steps += done;
         step = step_end;
-        pha = frac(pha + jp[0].w_ac * (double)B);
-        // re-entries due by the end of the block, then fresh ions (engine.rs:512-535)
+        for (size_t t = 0; t < tone_ph.size(); ++t) tone_ph[t] = frac(tone_ph[t] + jp[t * nsp].w_ac * (double)B);
+        // re-entries due by the end of the block, then fresh ions unless a sweep drains (engine.rs:512-535)
         while (!returns.empty() && returns.top().first <= step_end) {
             const uint32_t k = returns.top().second;
             returns.pop();
             spawn(k, true);
         }
-        const uint64_t nnew = poisson(rng, o.inj_rate * (double)B * ph.dt);
-        for (uint64_t i = 0; i < nnew; ++i) {
-            const double u = xs_u01(rng);
-            uint32_t k = 0;
-            while (k + 1 < nsp && !(u <= cdf[k])) ++k;
-            spawn(k, false);
+        double f_tone0 = -1.0;  // the live tone's next frequency when a sweep point advances
+        if (!sw.draining) {
+            const uint64_t nnew = poisson(rng, o.inj_rate * (double)B * ph.dt);
+            for (uint64_t i = 0; i < nnew; ++i) {
+                const double u = xs_u01(rng);
+                uint32_t k = 0;
+                while (k + 1 < nsp && !(u <= cdf[k])) ++k;
+                spawn(k, false);
+                if (sw.running && ++sw.inj_in_point >= sw.cfg.ions_per_point) {
+                    sw.inj_in_point = 0;
+                    if (++sw.point >= sw.f.size()) {
+                        sw.sweeps_done += 1;
+                        sw.point = 0;
+                        if (sw.sweeps_done >= sw.cfg.sweeps) sw.draining = true;
+                    }
+                    f_tone0 = sw.f[sw.point] * 1e3;
+                }
+            }
         }
         flush();
+        // the sweep stops once the last ion on a sweep tone has ended
+        if (sw.draining && sw.running && tagged_alive == 0) {
+            sw.running = false;
+            sw.draining = false;
+            sw.point = (uint32_t)sw.f.size() - 1;
+            f_tone0 = f_live;
+        }
+        if (f_tone0 >= 0.0) set_tone0(f_tone0);
         st.step = step;
     }
 
-    void set_ac(double f_ac, int32_t t)
+    // A new frequency of the live tone from `step` on: its jobs' w_ac, and the
+    // phase offsets of the live ions on it.
+    void set_tone0(double f_ac)
     {
         const double w = f_ac * ph.dt;
-        for (JobP& j : jp) j.w_ac = w;
+        for (uint32_t k = 0; k < nsp; ++k) jp[k].w_ac = w;
         hipStream_t s = V.stream;
         HIPCHK(hipMemcpyAsync(jobs.p, jp.data(), nsp * sizeof(JobP), hipMemcpyHostToDevice, s));
         if (st.alive)
-            launch_live_rephase(S, list[cur].as<uint32_t>(), ctr_of(cur) + CTR_COUNT, (uint32_t)st.alive, pha, w, s);
+            launch_live_rephase(S, list[cur].as<uint32_t>(), ctr_of(cur) + CTR_COUNT, (uint32_t)st.alive, tone_ph[0], w,
+                                nsp, s);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s));
+    }
+
+    // apply_params (engine.rs:399): the live tone follows a running sweep.
+    void set_ac(double f_ac, int32_t t)
+    {
+        f_live = f_ac;
+        if (!sw.running) set_tone0(f_ac);
         tag = t;
+    }
+
+    // start_sweep (engine.rs:420-429): the points and their tones, the live tone
+    // on the first point. As the clock's resize there, tone t keeps the phase it
+    // had if the previous sweep had it and starts at 0 otherwise.
+    void sweep_start(const mq_live_sweep_cfg& c)
+    {
+        if (sw.running) throw Err(-1, "a sweep is running; stop it and let it drain first");
+        if (!std::isfinite(c.f_lo_khz) || !std::isfinite(c.f_hi_khz) || !(c.step_khz > 0.0) || !std::isfinite(c.step_khz))
+            throw Err(-1, "sweep frequencies must be finite and step_khz > 0");
+        if (c.ions_per_point == 0 || c.sweeps == 0) throw Err(-1, "ions_per_point and sweeps must be >= 1");
+        const double span = std::round((c.f_hi_khz - c.f_lo_khz) / c.step_khz);
+        const uint32_t n = span > 0.0 ? (uint32_t)std::min(span, (double)(MQ_LIVE_MAX_POINTS - 1)) : 0u;
+        if (c.f_lo_khz < 0.0 || c.f_lo_khz + n * c.step_khz < 0.0) throw Err(-1, "sweep frequencies must be >= 0");
+        sw = Sweep{};
+        sw.cfg = c;
+        for (uint32_t i = 0; i <= n; ++i) sw.f.push_back(c.f_lo_khz + i * c.step_khz);
+        sw.acc.assign((size_t)sw.f.size() * nsp * 4, 0);
+        const size_t ntone = 1 + sw.f.size();
+        tone_ph.resize(ntone, 0.0);
+        jp.resize(ntone * nsp);
+        for (size_t t = 1; t < ntone; ++t)
+            for (uint32_t k = 0; k < nsp; ++k) {
+                jp[t * nsp + k] = jp[k];
+                jp[t * nsp + k].w_ac = sw.f[t - 1] * 1e3 * ph.dt;
+            }
+        HIPCHK(hipMemcpyAsync(jobs.p, jp.data(), jp.size() * sizeof(JobP), hipMemcpyHostToDevice, V.stream));
+        sw.running = true;
+        set_tone0(sw.f[0] * 1e3);
     }
 };
 
@@ -396,6 +480,46 @@ int mq_live_set_ac(mq_live* l, double f_ac, int32_t tag)
         if (!l) throw Err(-1, "NULL argument");
         if (!(f_ac >= 0.0)) throw Err(-1, "AC frequency must be >= 0");
         l->set_ac(f_ac, tag);
+    })
+}
+
+int mq_live_sweep_start(mq_live* l, const mq_live_sweep_cfg* c)
+{
+    MQ_LIVE_GUARD({
+        if (!l || !c) throw Err(-1, "NULL argument");
+        l->sweep_start(*c);
+    })
+}
+
+int mq_live_sweep_stop(mq_live* l)
+{
+    MQ_LIVE_GUARD({
+        if (!l) throw Err(-1, "NULL argument");
+        // stop_sweep (engine.rs:431-432) also marks a finished sweep running and
+        // draining: the next block ends it again, without fresh ions
+        if (!l->sw.f.empty()) l->sw.draining = l->sw.running = true;
+    })
+}
+
+int mq_live_sweep(const mq_live* l, mq_live_sweep_state* st, double* f_khz, uint64_t* acc)
+{
+    MQ_LIVE_GUARD({
+        if (!l) throw Err(-1, "NULL argument");
+        const auto& w = l->sw;
+        const uint32_t np = (uint32_t)w.f.size();
+        if (st) {
+            mq_live_sweep_state r{};
+            r.npoints = np;
+            r.point = w.point;
+            r.sweeps_done = w.sweeps_done;
+            r.running = w.running;
+            r.draining = w.draining;
+            const double tot = (double)std::max<uint64_t>(1, (uint64_t)w.cfg.sweeps * np);
+            r.progress = np ? std::min(1.0, ((double)w.sweeps_done * np + w.point) / tot) : 0.0;
+            *st = r;
+        }
+        if (f_khz) std::copy(w.f.begin(), w.f.end(), f_khz);
+        if (acc) std::copy(w.acc.begin(), w.acc.end(), acc);
     })
 }
 

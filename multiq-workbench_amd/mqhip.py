@@ -236,6 +236,19 @@ class LiveState(C.Structure):
                 ("dropped", C.c_uint64), ("finished", C.c_uint64), ("ion_steps", C.c_uint64)]
 
 
+class LiveSweepCfg(C.Structure):
+    _fields_ = [("f_lo_khz", C.c_double), ("f_hi_khz", C.c_double), ("step_khz", C.c_double),
+                ("ions_per_point", C.c_uint32), ("sweeps", C.c_uint32)]
+
+
+class LiveSweepState(C.Structure):
+    _fields_ = [("npoints", C.c_uint32), ("point", C.c_uint32), ("sweeps_done", C.c_uint32), ("running", C.c_int32),
+                ("draining", C.c_int32), ("pad_", C.c_int32), ("progress", C.c_double)]
+
+
+LIVE_MAX_POINTS = 101
+
+
 class ScanPoint(C.Structure):
     _fields_ = [("f_khz", C.c_double), ("transmitted", C.c_double), ("crashed", C.c_double), ("n", C.c_uint64)]
 
@@ -302,6 +315,10 @@ SIGNATURES = {
     "mq_live_run": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(LiveState)]),
     "mq_live_set_ac": (C.c_int, [C.c_void_p, C.c_double, C.c_int32]),
     "mq_live_tally": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Outcome)]),
+    "mq_live_sweep_start": (C.c_int, [C.c_void_p, C.POINTER(LiveSweepCfg)]),
+    "mq_live_sweep_stop": (C.c_int, [C.c_void_p]),
+    "mq_live_sweep": (C.c_int, [C.c_void_p, C.POINTER(LiveSweepState), C.POINTER(C.c_double),
+                                C.POINTER(C.c_uint64)]),
     "mq_live_slots": (C.c_int, [C.c_void_p, C.POINTER(IonSoA), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                 C.POINTER(C.c_uint64)]),
     "mq_summarize": (C.c_int, [C.POINTER(C.c_uint8), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
@@ -606,6 +623,25 @@ class Live:
         out = (Outcome * self.nsp)()
         _check(load().mq_live_tally(self._h, tag, out))
         return list(out)
+
+    def sweep_start(self, f_lo_khz: float, f_hi_khz: float, step_khz: float, ions_per_point: int,
+                    sweeps: int = 1) -> None:
+        c = LiveSweepCfg(f_lo_khz, f_hi_khz, step_khz, ions_per_point, sweeps)
+        _check(load().mq_live_sweep_start(self._h, C.byref(c)))
+
+    def sweep_stop(self) -> None:
+        _check(load().mq_live_sweep_stop(self._h))
+
+    def sweep(self) -> dict:
+        """The last sweep: its state, frequencies [kHz] and counts acc[point, species, (trans, crash, return,
+        other)]."""
+        st = LiveSweepState()
+        f = np.zeros(LIVE_MAX_POINTS, np.float64)
+        acc = np.zeros((LIVE_MAX_POINTS, self.nsp, 4), np.uint64)
+        _check(load().mq_live_sweep(self._h, C.byref(st), _dptr(f), acc.ctypes.data_as(C.POINTER(C.c_uint64))))
+        d = {k: getattr(st, k) for k, _ in st._fields_ if k != "pad_"}
+        d["f_khz"], d["acc"] = f[:st.npoints], acc[:st.npoints]
+        return d
 
     def slots(self) -> dict:
         n = self.capacity

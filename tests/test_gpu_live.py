@@ -9,6 +9,11 @@
   366-370) predicts the number spawned per species exactly.
 * Bookkeeping: alive + finished = spawned; ion-steps equal the sum of the ions' steps; with recirculation and a
   zero mean delay every returned ion re-enters in the same block; tags split the tallies of an AC sweep.
+* Sweep (start_sweep, stop_sweep and the block's sweep bookkeeping, engine.rs:420-446, 512-535): on a clock where
+  every tone's phase is 0 at block boundaries, each point's ions equal, bit for bit, the same-numbered ions of a
+  population whose live tone is that point's frequency, and its counts are theirs; the live tone's ions are moved
+  to each new frequency with a continuous phase; the state machine (101-point cap, a single point when the upper
+  limit is below the lower, passes, drain, stop before and after the end, bad configurations) as the reference's.
 """
 import math
 import sys
@@ -166,5 +171,140 @@ def test_live_rejects_bad_options(eng):
         assert st["injected"] == 10 and st["dropped"] == 15 and st["alive"] == 10
         with pytest.raises(mqhip.MqError, match="species"):
             live.inject(3, 1)
+    finally:
+        live.close()
+
+
+def _sweep_pop(eng, f_ac, sweep, n0=300, n1=200, ipp=400, rate=1.5e6, seed=31):
+    """A population on the RF-aligned clock (f_rf dt = 1/16, 64-step blocks: four RF periods) with an
+    injected group at step 0, two blocks, then optionally a sweep of 125 and 250 kHz, an injected group and
+    blocks until the sweep has ended (or as many blocks as `sweep` says when it is an int)."""
+    ph = physics(dt=1.25e-7)
+    rf = mqhip.RfAc(100.0, 500e3, 0.5, f_ac)
+    cap = 60000
+    live = mqhip.Live(eng, SP[:2], [0.6, 0.4], rf, [0, 5, 0, 0], ph, block_steps=64, capacity=cap, inj_rate=rate,
+                      seed=seed)
+    live.inject(0, n0)
+    pre = live.run(2)
+    if sweep is True:
+        live.sweep_start(125.0, 250.0, 125.0, ipp, 1)
+    live.inject(1, n1)
+    nb = 2
+    while True:
+        st = live.run(1)
+        nb += 1
+        sw = live.sweep()
+        if (sweep is True and not sw["running"]) or (sweep is not True and nb >= sweep):
+            break
+        assert nb < 400
+    assert st["injected"] + st["reentered"] < cap and st["dropped"] == 0   # every ion keeps its own slot
+    return live, pre, st, nb
+
+
+def test_sweep_points_run_on_their_own_tones(eng):
+    """Sweep ions are the ions of a population whose live tone is the point's frequency.
+
+    Tones of 125 and 250 kHz advance 1 and 2 cycles per 64-step block, so every tone's phase at a block
+    boundary is 0, and a population without a sweep at that frequency gives its ions the same birth phases.
+    Populations with the same seed draw the same spawns in the same order until the sweep drains, and with
+    capacity to spare ion k sits in slot k: every ion of sweep point p equals the ion of the same birth order
+    in the population at f_p, bit for bit. The live tone starts at 93.75 kHz (0.75 cycles a block), so the
+    untagged ions of the first two blocks carry AC phase 0.5 after the sweep's start moves them to 125 kHz.
+    """
+    live, pre, st, nb = _sweep_pop(eng, 93.75e3, True)
+    try:
+        sw = live.sweep()
+        s = live.slots()
+    finally:
+        live.close()
+    say(f"[gpu] live sweep: {nb} blocks, {st}, sweep {dict((k, v) for k, v in sw.items() if k != 'acc')}")
+    assert sw["npoints"] == 2 and list(sw["f_khz"]) == [125.0, 250.0]
+    assert not sw["running"] and not sw["draining"] and sw["point"] == 1 and sw["sweeps_done"] == 1
+    n0, n1, ipp = 300, 200, 400
+    n_pre = pre["injected"]                                   # spawned before the sweep: the live tone
+    nsp = n_pre + n1 + 2 * ipp                                # then n1 injected and 2 ipp Poisson ions on sweep tones
+    tone = np.zeros(st["injected"], np.int64)
+    tone[n_pre:n_pre + n1 + ipp] = 1
+    tone[n_pre + n1 + ipp:nsp] = 2
+    assert np.array_equal(s["sid"][:st["injected"]], np.arange(st["injected"]))
+    refs = {}
+    for f, t in ((125e3, 1), (250e3, 2)):
+        ref, _, rst, _ = _sweep_pop(eng, f, nb)
+        try:
+            refs[t] = ref.slots()
+        finally:
+            ref.close()
+        sel = np.flatnonzero(tone == t)
+        for k in ("x", "y", "z", "vx", "vy", "vz", "steps", "ncoll", "status", "phi_ac0", "species"):
+            assert np.array_equal(s[k][sel], refs[t][k][sel]), (f, k)
+        assert np.all(s["status"][sel] != 0)
+        # the point's counts: transmitted, crashed, returned, other, per species
+        for sp in range(2):
+            st_sp = s["status"][sel][s["species"][sel] == sp]
+            want = [np.sum(st_sp == mqhip.TRANSMITTED), np.sum(st_sp == mqhip.CRASHED),
+                    np.sum(st_sp == mqhip.RETURNED), 0]
+            want[3] = len(st_sp) - sum(want)
+            assert list(sw["acc"][t - 1, sp]) == want, (t, sp)
+        say(f"[gpu]   point {t - 1} ({f / 1e3:.0f} kHz): {len(sel)} ions, acc {sw['acc'][t - 1].tolist()}")
+    # fresh ions per point: exactly ions_per_point (the injected group of point 0 does not count)
+    assert sw["acc"][0].sum() == n1 + ipp and sw["acc"][1].sum() == ipp
+    # the untagged ions: born on the 93.75 kHz tone (phase 0, 0.75, 0.5 at steps 0, 64, 128); those alive when the
+    # sweep started at step 128 moved to 125 kHz at its phase 0.5, and kept it (1 and 2 cycles a block) through the
+    # move to 250 kHz; every one of them ended before the sweep did
+    b = s["birth"][:n_pre].astype(np.int64)
+    alive = s["steps"][:n_pre].astype(np.int64) > 128 - b
+    born = np.where(b == 0, 0.0, np.where(b == 64, 0.75, 0.5))
+    assert set(b.tolist()) <= {0, 64, 128} and np.sum(b == 0) == n0
+    assert np.array_equal(s["phi_ac0"][:n_pre], np.where(alive, 0.5, born))
+    assert np.all(s["phi_ac0"][n_pre:nsp] == 0.0)
+
+
+def test_sweep_state_machine(eng):
+    """start_sweep / stop_sweep / the block's bookkeeping (engine.rs:420-446, 512-535) through the C ABI."""
+    ph = physics(dt=1.25e-7)
+    rf = mqhip.RfAc(100.0, 500e3, 0.5, 100e3)
+    live = mqhip.Live(eng, SP[:1], None, rf, [0, 5, 0, 0], ph, block_steps=64, capacity=100000, inj_rate=2e6,
+                      seed=0x5DEECE66D)
+    try:
+        assert live.sweep()["npoints"] == 0
+        live.sweep_stop()                                    # no sweep yet: nothing happens
+        assert live.sweep()["npoints"] == 0
+        # 301 steps requested: 101 points at most (start_sweep, engine.rs:423-424)
+        live.sweep_start(70.0, 100.0, 0.1, 1000, 1)
+        sw = live.sweep()
+        assert sw["npoints"] == 101 and sw["f_khz"][100] == 70.0 + 100 * 0.1 and sw["running"] == 1
+        with pytest.raises(mqhip.MqError, match="sweep is running"):
+            live.sweep_start(80.0, 90.0, 1.0, 10, 1)
+        live.sweep_stop()
+        sw = live.sweep()
+        assert sw["draining"] == 1 and sw["running"] == 1
+        inj = live.run(0)["injected"]
+        st = live.run(1)
+        # draining: no fresh ions; no ion was on a sweep tone, so the sweep ends in that block
+        assert st["injected"] == inj
+        sw = live.sweep()
+        assert sw["running"] == 0 and sw["draining"] == 0 and sw["point"] == 100
+        assert live.run(3)["injected"] > inj                  # the live beam resumes
+        # an upper limit below the lower one: a single point
+        live.sweep_start(90.0, 80.0, 1.0, 50, 2)
+        sw = live.sweep()
+        assert sw["npoints"] == 1 and list(sw["f_khz"]) == [90.0]
+        # two passes over one point of 50 fresh ions, then the drain
+        prog = []
+        while live.sweep()["running"]:
+            live.run(1)
+            prog.append(live.sweep()["progress"])
+        sw = live.sweep()
+        assert sw["sweeps_done"] == 2 and sw["acc"].sum() == 100 and prog[-1] == 1.0
+        assert all(0.0 <= a <= b <= 1.0 for a, b in zip(prog, prog[1:]))
+        # stop after the end: one block without fresh ions, then the beam again (stop_sweep, engine.rs:431-432)
+        live.sweep_stop()
+        assert live.sweep()["running"] == 1 and live.sweep()["draining"] == 1
+        inj = live.run(0)["injected"]
+        assert live.run(1)["injected"] == inj and live.sweep()["running"] == 0
+        for bad in ((80.0, 90.0, 0.0, 10, 1), (80.0, 90.0, 1.0, 0, 1), (80.0, 90.0, 1.0, 10, 0),
+                    (float("nan"), 90.0, 1.0, 10, 1), (-5.0, 90.0, 1.0, 10, 1)):
+            with pytest.raises(mqhip.MqError):
+                live.sweep_start(*bad)
     finally:
         live.close()
