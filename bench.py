@@ -263,6 +263,7 @@ def tandem_bench(steps: int = 2):
                   "0.05 mm (confining RF, axial push), nine ports on one 128x128x512 @ 0.05 mm channel grid; "
                   "115 V RF @ 500 kHz shared clock; ports 0, 2, 6, 8 driven at the 6+..9+ secular frequencies "
                   f"{[round(x / 1e3, 1) for x in d.tones if x > 0]} kHz (0.5 V), idle ports +4 V gate; "
+                  "combiner push of 100 V to the y-face exit (1 mm below the board, in line with ports 2, 5, 8); "
                   "thermal N2 1 mTorr; dt 40 ns; cap 30000 steps",
         "value": n_steps / wall, "unit": "ion-steps/s", "kernel_ion_steps_per_s": n_steps / max(kms * 1e-3 * steps, 1e-9),
         "plain_fp64": psteps / pwall, "speedup_vs_plain_fp64": (n_steps / wall) / (psteps / pwall),

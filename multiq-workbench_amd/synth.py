@@ -148,10 +148,12 @@ def secular_frequency(mass_da: float, z: float, v_rf: float, f_rf: float, r0: fl
 def tandem_cube(nx: int, ny: int, nz: int, h: float, seam_z: float, r0_rf: float = 6.0e-3) -> dict:
     """Splitter/combiner grid of the tandem batch (a stand-in for trapsim's
     486-post cube, geometry.rs): a confining RF quadrupole E = 2 (Y, X) / r0_rf^2
-    per volt about the box axis, no excitation, and one DC basis (slot 0)
-    that is a uniform axial push E_z = 1 / seam_z per volt (+V drives ions
-    toward the port board in the splitter, -V toward the detector in the
-    combiner). No electrodes inside the box; its faces are the walls."""
+    per volt about the box axis, no excitation, and two DC bases: slot 0 a
+    uniform axial push E_z = 1 / seam_z per volt (+V drives ions toward the
+    port board), slot 1 a uniform transverse push E_y = 1 / L per volt, L the
+    distance from the axis to the y = (ny - 1) h face where the entrance
+    aperture is (+V drives ions to that face). No electrodes inside the box;
+    its faces are the walls."""
     ax = (nx // 2) * h
     ay = (ny // 2) * h
     x = np.arange(nx) * h - ax
@@ -164,6 +166,8 @@ def tandem_cube(nx: int, ny: int, nz: int, h: float, seam_z: float, r0_rf: float
     rf[..., 2] = 0.0
     push = np.zeros((nz, ny, nx, 3))
     push[..., 2] = 1.0 / seam_z
+    exit_push = np.zeros((nz, ny, nx, 3))
+    exit_push[..., 1] = 1.0 / ((ny - 1) * h - ay)
     return {
         "nx": nx, "ny": ny, "nz": nz, "h": h,
         "axis_x": ax, "axis_y": ay, "z_inject": 1.0e-3,
@@ -171,14 +175,14 @@ def tandem_cube(nx: int, ny: int, nz: int, h: float, seam_z: float, r0_rf: float
         "z_return_legacy": 1.0e-3, "z_exit_legacy": seam_z,
         "crash_z0_legacy": 0.0, "crash_z1_legacy": 0.0,
         "rf": rf.reshape(3 * N), "ac": None,
-        "dc": [push.reshape(3 * N), None, None, None],
+        "dc": [push.reshape(3 * N), exit_push.reshape(3 * N), None, None],
         "mask": None,
     }
 
 
 def tandem_setup(mod, size: str = "c5", n_per_charge: int | None = None, v_rf: float = 115.0,
-                 v_push: float = 2.0, v_gate: float = 1.0, v_ac: float = 0.5, pitch: float = 2.0e-3,
-                 ap_r: float = 0.8e-3, pressure_pa: float = 0.133322, max_steps: int = 30000):
+                 v_push: float = 2.0, v_exit: float = 100.0, v_gate: float = 1.0, v_ac: float = 0.5,
+                 pitch: float = 2.0e-3, ap_r: float = 0.8e-3, pressure_pa: float = 0.133322, max_steps: int = 30000):
     """The tandem configuration. size "c5": cube and ports 128 x 128 x 512 at
     0.05 mm, 4 x 262144 ions (1M); size "small": cube 48 x 48 x 176 at
     0.15 mm, ports 64 x 64 x 256 at 0.1 mm, for parity checks.
@@ -188,6 +192,10 @@ def tandem_setup(mod, size: str = "c5", n_per_charge: int | None = None, v_rf: f
     3 x 3 lattice of the given pitch about the cube axis; ports 0, 2, 6, 8 are
     driven at the secular frequencies of 6+, 7+, 8+, 9+ (full kick); the
     others are idle with the gate raised by 4 V (trapsim lib.rs:196-203).
+    The splitter pushes ions to the board (v_push on slot 0); the combiner
+    pushes the forwarded ions to the y face (v_exit on slot 1), where the
+    entrance aperture sits 1 mm below the board in line with the mouths of
+    ports 2, 5 and 8, so that ions forwarded by ports 2 and 8 can be detected.
     Returns (cube, port, geom, drive, jobs, physics)."""
     if size == "c5":
         cube = tandem_cube(128, 128, 512, 0.5e-4, seam_z=24.0e-3)
@@ -201,12 +209,13 @@ def tandem_setup(mod, size: str = "c5", n_per_charge: int | None = None, v_rf: f
         raise ValueError(size)
     ax, ay = cube["axis_x"], cube["axis_y"]
     mouths = [(ax + (c - 1) * pitch, ay + (r - 1) * pitch) for r in range(3) for c in range(3)]
-    # the entrance aperture on the cube's y face at the injection height, with a 2 mm tube (trapsim's l_ext)
-    geom = mod.tandem_geom(24.0e-3, mouths, ap_r, (ax, cube["z_inject"] + 2.0e-3), 2.0e-3)
+    # the entrance aperture on the cube's y face 1 mm below the board, in line with the mouths of ports 2, 5
+    # and 8, with a 2 mm tube (trapsim's l_ext)
+    geom = mod.tandem_geom(24.0e-3, mouths, ap_r, (ax + pitch, 23.0e-3), 2.0e-3)
     tones = [0.0] * 9
     for p, z in zip(DRIVEN_PORTS, UBQ_CHARGES):
         tones[p] = secular_frequency(UBQ_MASS, z, v_rf, 500e3)
-    drive = mod.tandem_drive(v_rf, 500e3, v_ac, tones, [v_push, 0, 0, 0], [-v_push, 0, 0, 0], [0, v_gate, 0, 0],
+    drive = mod.tandem_drive(v_rf, 500e3, v_ac, tones, [v_push, 0, 0, 0], [0, v_exit, 0, 0], [0, v_gate, 0, 0],
                              [0, 4.0, 0, 0])
     ccs = {6.0: 1000.0, 7.0: 1150.0, 8.0: 1300.0, 9.0: 1450.0}
     jobs = [mod.job(UBQ_MASS, z, ccs[z], v_rf, 500e3, v_ac, 0.0, n, 6000 + int(z)) for z in UBQ_CHARGES]

@@ -2,10 +2,12 @@
 
 * FP64 plain path: every ion's final state, region and last port identical
   to the oracle's, and the per-port counters, the retry matrix and the totals
-  equal exactly (integer bookkeeping of identical trajectories).
+  equal exactly (integer bookkeeping of identical trajectories), on a run in
+  which ions are forwarded through the ports and detected past the combiner.
 * FP32 batched path: the same flow balance holds; per-port entries, returns,
-  forwards and losses and the end classes agree with the FP64 path within
-  4 sigma (Poisson); per-ion results do not depend on the batch composition.
+  forwards and losses and the end classes (detections included) agree with
+  the FP64 path within 4 sigma (Poisson); per-ion results do not depend on
+  the batch composition.
 """
 import sys
 
@@ -46,6 +48,7 @@ def test_plain_matches_oracle_exactly(small, oracle_mod):
         assert np.array_equal(s[k], so[k]), k
     flow_checks(c, tt, s, len(jobs))
     assert sum(p["entered"] for p in c) > 50 and sum(p["returned"] for p in c) > 0
+    assert sum(p["forwarded"] for p in c) > 0 and tt["detected"] > 0
 
 
 @pytest.mark.parametrize("case", ["entry", "bounce", "forward_floor", "reverse", "from_other", "board_top",
@@ -83,7 +86,7 @@ def test_batched_flow_balance_and_agreement(small):
     for p in range(9):
         for k in ("entered", "returned", "forwarded", "lost", "revisits"):
             assert close(cb[p][k], cp[p][k]), (p, k, cb[p][k], cp[p][k])
-    assert sum(p["forwarded"] for p in cb) > 0
+    assert sum(p["forwarded"] for p in cb) > 0 and tb["detected"] > 50 and tp["detected"] > 50
 
 
 @pytest.mark.parametrize("path", [mqhip.PATH_BATCHED, mqhip.PATH_PLAIN])
