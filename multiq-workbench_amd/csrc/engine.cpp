@@ -599,10 +599,11 @@ struct mq_engine {
     // device memory, so launches past the end of the work are empty) and the host
     // checks the live count once per block; with `graph` on, the block is
     // captured once as a hipGraph and replayed.
-    // bins_known: bin_of holds the bins of list[cur] (the last launch wrote them);
-    // pass_bins: each launch writes the bins of the ions it carries, so that the
-    // next re-sort skips the gather of their positions (off in graph mode, where
-    // one captured block must serve every replay).
+    // bins_known: bin_of, rank_of and hist hold the bins, ranks within their bin
+    // and counts per bin of list[cur] (the last launch wrote them); pass_bins:
+    // each launch counts and ranks the ions it carries, so that the next re-sort
+    // neither gathers their positions nor ranks them again (off in graph mode,
+    // where one captured block must serve every replay).
     void enqueue_block(const CallBufs& cb, const BatchP& B, uint32_t n, const SortGeom& G, int& cur, uint32_t& chunk,
                        uint32_t cap, uint32_t period, uint32_t grid, uint32_t block, int iters, bool timed,
                        std::vector<hipEvent_t>* evs, bool resort, bool& bins_known, bool pass_bins,
@@ -638,6 +639,10 @@ struct mq_engine {
             a.nqueues = resort ? std::min<uint32_t>(MAX_QUEUES, grid) : 1u;
             a.coll_batch = (uint32_t)std::max(1, tune.coll_batch);
             a.bin_out = resort && pass_bins ? cb.bin_of : nullptr;
+            // the worker also counts and ranks the ions it carries, so the next re-sort needs no ranking pass
+            a.rank_out = a.bin_out ? cb.rank_of : nullptr;
+            a.hist_out = a.bin_out ? cb.hist : nullptr;
+            if (a.hist_out) launch_fill_hist(cb.hist, G.nbins, stream);
             a.sg = G;
             a.lanes = lanes;
             bins_known = a.bin_out != nullptr;

@@ -22,26 +22,34 @@ __device__ __forceinline__ void load_cell(const float4* __restrict__ cells, int3
 
 // Rank of live-list entry i within its re-sort bin: hist[bin] counts the bin's
 // entries and rank_of[i] is this entry's place among them. Lanes of a wavefront
-// that share a bin take their ranks with one atomic: the bin of the lowest
-// pending lane is peeled until every lane has its rank (freshly injected ions
-// all start in a few bins, and one atomic per ion serialised on those counters).
+// that share a bin take their ranks with one atomic: the bins are peeled from
+// the lowest pending lane (freshly injected ions all start in a few bins, and
+// one atomic per ion serialised on those counters), then the lowest lane of
+// each bin adds its bin's count, all bins in one instruction, so the wavefront
+// waits for a single round trip of returning atomics.
 // All active lanes of the wavefront must call this together.
 __device__ __forceinline__ void bin_rank(uint32_t* __restrict__ hist, uint32_t bin, uint32_t i,
                                          uint32_t* __restrict__ rank_of)
 {
     const uint32_t lane = threadIdx.x & 63;
     uint64_t todo = __ballot(1);
-    for (;;) {
+    int first = (int)lane;  // lowest lane of this lane's bin
+    uint32_t below = 0, count = 0;
+    while (todo) {
         const int leader = __ffsll((unsigned long long)todo) - 1;
         const uint32_t lb = (uint32_t)__shfl((int)bin, leader);
         const uint64_t same = __ballot(bin == lb) & todo;
-        uint32_t base = 0;
-        if (lane == (uint32_t)leader) base = atomicAdd(&hist[lb], (uint32_t)__popcll(same));
-        base = (uint32_t)__shfl((int)base, leader);
-        if ((same >> lane) & 1ull) rank_of[i] = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        if ((same >> lane) & 1ull) {
+            first = leader;
+            below = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        }
+        if (lane == (uint32_t)leader) count = (uint32_t)__popcll(same);
         todo &= ~same;
-        if (todo == 0) break;
     }
+    uint32_t base = 0;
+    if (count) base = atomicAdd(&hist[bin], count);
+    base = (uint32_t)__shfl((int)base, first);
+    rank_of[i] = base + below;
 }
 
 __device__ __forceinline__ float rget(const float4 R[CELL_F4], int k)

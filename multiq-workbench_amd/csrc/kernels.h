@@ -58,15 +58,18 @@ struct WorkerArgs {
     uint32_t nqueues;   // 1, or MAX_QUEUES contiguous segments (one per XCD)
     uint32_t coll_batch;  // pending collisions per wavefront that trigger the collision code
     uint32_t* bin_out;    // optional: re-sort bin of each carried ion, at its list_out position
+    uint32_t* rank_out;   // with bin_out: the ion's rank among the carried ions of its bin (at the same position)
+    uint32_t* hist_out;   // with rank_out: carried ions per bin (zeroed before the launch)
     SortGeom sg;          // bins of bin_out
     uint32_t lanes;       // lanes per wavefront that take ions (1..64; fewer spread a small tail over more waves)
 };
 
-// bins_known: bin_of already holds the bin of each list entry (written by the
-// worker launch that produced the list), so only the ranks are computed.
+// ranked: the worker launch that wrote the list has already put each entry's
+// bin and rank in bin_of / rank_of and the counts per bin in hist, so only
+// the offsets and the scatter remain; otherwise k_bin gathers the positions.
 void launch_sort(const IonState& S, const BatchP& B, const uint32_t* list, const uint32_t* ctr, uint32_t* out,
                  uint32_t* bin_of, uint32_t* rank_of, uint32_t* hist, uint32_t* bsum, const SortGeom& G,
-                 uint32_t nmax, bool bins_known, hipStream_t st);
+                 uint32_t nmax, bool ranked, hipStream_t st);
 // Pieces of launch_sort for callers with their own binning kernel: clear the
 // histogram; then, after bin_of / rank_of / hist are filled, turn counts into
 // offsets and place every entry.

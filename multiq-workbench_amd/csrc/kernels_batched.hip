@@ -510,7 +510,12 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
             if (carry) {
                 const uint32_t pos = base + (uint32_t)__popcll(cm & ((1ull << lane) - 1ull));
                 A.list_out[pos] = L.id;
-                if (A.bin_out) A.bin_out[pos] = sort_bin(A.B, A.sg, L.x, L.y, L.z);
+                if (A.bin_out) {
+                    const uint32_t bin = sort_bin(A.B, A.sg, L.x, L.y, L.z);
+                    A.bin_out[pos] = bin;
+                    // the next re-sort's counts and ranks, one atomic per bin and wavefront
+                    if (A.hist_out) bin_rank(A.hist_out, bin, pos, A.rank_out);
+                }
             }
         }
         if (have && stop) {
@@ -591,9 +596,10 @@ void launch_gather_probe(const float4* cells, uint64_t window, uint32_t iters, f
 
 // ------------------------------------------------- re-sort by grid cell
 // Counting sort of the live list by coarse cell bin (z-major, x fastest):
-// k_bin counts, the three scan kernels turn counts into offsets, k_scatter
-// places each ion. Order inside a bin follows the atomics; per-ion results do
-// not depend on list order.
+// k_bin counts (or the worker launch that wrote the list has counted and
+// ranked the ions it carried), the three scan kernels turn counts into
+// offsets, k_scatter places each ion. Order inside a bin follows the atomics;
+// per-ion results do not depend on list order.
 __global__ void k_bin(IonState S, BatchP B, const uint32_t* __restrict__ list, const uint32_t* __restrict__ ctr,
                       uint32_t* __restrict__ bin_of, uint32_t* __restrict__ rank_of, uint32_t* __restrict__ hist,
                       SortGeom G)
@@ -604,16 +610,6 @@ __global__ void k_bin(IonState S, BatchP B, const uint32_t* __restrict__ list, c
     const uint32_t bin = sort_bin(B, G, S.x[id], S.y[id], S.z[id]);
     bin_of[i] = bin;
     bin_rank(hist, bin, i, rank_of);
-}
-
-// k_bin for a list whose bins the worker has already written (bin_of): the
-// positions are not gathered again.
-__global__ void k_rank(const uint32_t* __restrict__ ctr, const uint32_t* __restrict__ bin_of,
-                       uint32_t* __restrict__ rank_of, uint32_t* __restrict__ hist)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ctr[CTR_COUNT]) return;
-    bin_rank(hist, bin_of[i], i, rank_of);
 }
 
 // Exclusive scan of hist in place: blocks of 1024 entries (256 threads x 4).
@@ -697,12 +693,13 @@ void launch_sort_place(const uint32_t* list, const uint32_t* ctr, uint32_t* out,
 
 void launch_sort(const IonState& S, const BatchP& B, const uint32_t* list, const uint32_t* ctr, uint32_t* out,
                  uint32_t* bin_of, uint32_t* rank_of, uint32_t* hist, uint32_t* bsum, const SortGeom& G,
-                 uint32_t nmax, bool bins_known, hipStream_t st)
+                 uint32_t nmax, bool ranked, hipStream_t st)
 {
     const uint32_t nb = G.nbins;
-    launch_fill_hist(hist, nb, st);
-    if (nmax && bins_known) k_rank<<<nblk(nmax, 256), 256, 0, st>>>(ctr, bin_of, rank_of, hist);
-    else if (nmax) k_bin<<<nblk(nmax, 256), 256, 0, st>>>(S, B, list, ctr, bin_of, rank_of, hist, G);
+    if (!ranked) {
+        launch_fill_hist(hist, nb, st);
+        if (nmax) k_bin<<<nblk(nmax, 256), 256, 0, st>>>(S, B, list, ctr, bin_of, rank_of, hist, G);
+    }
     launch_sort_place(list, ctr, out, bin_of, rank_of, hist, bsum, nb, nmax, st);
 }
 
