@@ -11,8 +11,8 @@ Sweep slice: 32 Latin-hypercube settings of the sweep (V_rf 50-150 V, f_exc 80-1
 192 x 192 x 768 grid at 1/30 mm (FP32 cell table of 8.9 GB).
 
 Bars:
-* FP64 plain path vs the oracle: every ion's status, steps, collisions, position and velocity identical (the
-  north-star bar is 1e-10 relative), on a 2,000-step cap.
+* FP64 plain path vs the oracle: every ion's status, steps, collisions, position and velocity identical (stricter
+  than agreement to 1e-10 relative), on a 2,000-step cap.
 * FP32 batched path vs the oracle at the full 20,000-step cap: the totals of each class within 3 sigma (binomial);
   per job and class, at least 99 % of the cells within 3 sigma and every cell within the Bonferroni bound of all
   the cells compared (overall false-alarm probability 0.001; z = 4.75 for 500 cells), since hundreds of cells
