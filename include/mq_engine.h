@@ -132,12 +132,13 @@ typedef struct {
     uint32_t launches;       /* worker / direct kernel launches */
     double kernel_ms;        /* device time of the worker kernels (HIP events) */
     double wall_ms;          /* host wall time of the call */
+    uint32_t quad_launches;  /* of the launches, those of the four-lane tail worker (mq_tune.tail_quads) */
 } mq_run_stats;
 
 /* Tuning of the batched path (TileCfg, tiled.rs:12-29). mq_set_tune: zero
  * in block, waves_per_cu, refill_period, chunk, chunk_cap, sync_every and
- * coll_batch keeps the current value; resort, graph and tail_lanes are always taken.
- * Results per ion do not depend on any of these. */
+ * coll_batch keeps the current value; resort, graph, tail_lanes and tail_quads
+ * are always taken. Results per ion do not depend on any of these. */
 typedef struct {
     int32_t block;           /* threads per workgroup (64..256, multiple of 64) */
     int32_t waves_per_cu;    /* persistent waves launched per CU */
@@ -158,6 +159,10 @@ typedef struct {
                                 collision code runs for all of them (1: at once) */
     int32_t tail_lanes;      /* once the live ions fit the launch's lanes: ions per wavefront
                                 at most (1..64; 0: spread them evenly over every wavefront) */
+    int32_t tail_quads;      /* 1: once no more ions are live than 16 per wavefront, step
+                                each on four lanes (the step's field components, drive sines
+                                and candidate draws split over the lanes; Philox runs only);
+                                0: one lane per ion throughout */
 } mq_tune;
 
 typedef struct mq_engine mq_engine;

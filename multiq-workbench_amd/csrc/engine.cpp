@@ -107,6 +107,10 @@ struct CallBufs {
     bool noalloc = false;
 };
 
+// Steps per launch of the four-lane tail worker: each launch repacks the live
+// ions into as few wavefronts as hold them.
+constexpr uint32_t QUAD_CHUNK = 1024;
+
 struct mq_engine {
     int device = 0;
     int cus = 0;
@@ -126,7 +130,7 @@ struct mq_engine {
     // per-call pools
     DevBuf jobs, acc, list[3], ctr, bin_of, rank_of, hist, bsum;
     DevBuf sx, sy, sz, sphi, svx, svy, svz, sfx, sfy, sfz, ssteps, sncoll, scd, sjob, sstatus, srng;
-    mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0, 4, 0};
+    mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0, 4, 0, 1};
     // no-alloc entry point: this device's scratch array and host staging
     // reserved at creation for MQ_NOALLOC_MAX_JOBS jobs
     void* scratch_dev = nullptr;
@@ -607,8 +611,9 @@ struct mq_engine {
     void enqueue_block(const CallBufs& cb, const BatchP& B, uint32_t n, const SortGeom& G, int& cur, uint32_t& chunk,
                        uint32_t cap, uint32_t period, uint32_t grid, uint32_t block, int iters, bool timed,
                        std::vector<hipEvent_t>* evs, bool resort, bool& bins_known, bool pass_bins,
-                       uint32_t lanes = 64, bool staged = false)
+                       uint32_t lanes = 64, bool staged = false, uint32_t quad_grid = 0)
     {
+        const bool quad = quad_grid != 0;
         uint32_t* c0 = cb.ctr;
         uint32_t* c1 = c0 + CTR_BLOCK;
         for (int it = 0; it < iters; ++it) {
@@ -649,11 +654,20 @@ struct mq_engine {
             launch_prepare(a, stream);
             if (timed) HIPCHK(hipEventRecord((*evs)[2 * it], stream));
             // staged tiles: 256-thread workgroups, two per CU (63 KB of LDS each)
-            if (staged) launch_staged(a, (uint32_t)cus * 2u, stream);
-            else launch_batched(a, grid, block, stream);
+            if (staged) {
+                launch_staged(a, (uint32_t)cus * 2u, stream);
+            } else if (quad) {
+                // four lanes per ion: the same wavefronts in 256-thread workgroups, a refill
+                // period of at most 8 steps (the candidate blocks a quad draws cover 8)
+                if (a.nqueues != 1 || a.bin_out) throw Err(-1, "the four-lane tail takes one queue and no re-sort");
+                a.period = std::min(period, 8u);
+                launch_tail4(a, quad_grid, stream);
+            } else {
+                launch_batched(a, grid, block, stream);
+            }
             if (timed) HIPCHK(hipEventRecord((*evs)[2 * it + 1], stream));
             cur ^= 1;
-            if (!resort) chunk = std::min(cap, chunk * 2u);
+            if (!resort && !quad) chunk = std::min(cap, chunk * 2u);
         }
     }
 
@@ -776,23 +790,38 @@ struct mq_engine {
             // or as few as fill all waves when tail_lanes is 0), so that a step's record reloads are not
             // serialised over 64 lanes of divergent addresses.
             uint32_t act = 64;
-            if (tail) {
+            // Once at most 16 ions per wavefront are left (Philox streams), each takes four lanes
+            // (k_tail4): a wavefront's step then runs about half the instructions. Its wavefronts are
+            // packed with 16 ions each and only as many are launched as the ions need, so that below
+            // 16 per SIMD every SIMD steps one wavefront; a fixed chunk of QUAD_CHUNK steps repacks the
+            // shrinking population at every launch.
+            const bool quad = tail && tune.tail_quads && !B.compat && alive_last <= 16 * waves;
+            uint32_t quad_grid = 0, qchunk = QUAD_CHUNK;
+            if (quad) {
+                act = tune.tail_lanes > 0 ? std::min<uint32_t>(16u, (uint32_t)tune.tail_lanes) : 16u;
+                const uint64_t wv = (alive_last + act - 1) / act;  // wavefronts the live ions fill
+                quad_grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(waves / 4, (wv + 3) / 4));
+            } else if (tail) {
                 act = tune.tail_lanes > 0 ? (uint32_t)tune.tail_lanes
                                           : (uint32_t)std::min<uint64_t>(64, (alive_last + waves - 1) / waves);
                 act = std::max(1u, std::min(64u, act));
             }
-            enqueue_block(cb, B, n, G, cur, chunk, cap, period, grid, block, iters, timed, &evs, resort && !tail,
-                          bins_known, true, act, staged && !tail);
+            // the tail's few launches are checked one at a time, so that each picks its worker
+            // and shape from its own live count
+            const int it = tail ? 1 : iters;
+            enqueue_block(cb, B, n, G, cur, quad ? qchunk : chunk, cap, period, grid, block, it, timed, &evs,
+                          resort && !tail, bins_known, true, act, staged && !tail, quad_grid);
             HIPCHK(hipGetLastError());
-            stats.launches += (uint32_t)iters;
+            stats.launches += (uint32_t)it;
+            if (quad_grid) stats.quad_launches += (uint32_t)it;
             uint32_t alive = 0;
             uint32_t* cfinal = cur ? c0 + CTR_BLOCK : c0;
             HIPCHK(hipMemcpyAsync(&alive, cfinal + CTR_COUNT, 4, hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
             if (timed) {
-                for (int it = 0; it < iters; ++it) {
+                for (int k = 0; k < it; ++k) {
                     float ms = 0.0f;
-                    HIPCHK(hipEventElapsedTime(&ms, evs[2 * it], evs[2 * it + 1]));
+                    HIPCHK(hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]));
                     kms += ms;
                 }
             }
@@ -859,9 +888,10 @@ bool read_tune_cache(const char* path, const std::string& key, mq_tune& t)
         if (tab == std::string::npos || l.substr(0, tab) != key) continue;
         mq_tune x{};
         x.tail_lanes = 64;  // entries written before the tail_lanes field
-        if (std::sscanf(l.c_str() + tab + 1, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", &x.block, &x.waves_per_cu,
+        x.tail_quads = 1;   // entries written before the tail_quads field (results are the same either way)
+        if (std::sscanf(l.c_str() + tab + 1, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", &x.block, &x.waves_per_cu,
                         &x.refill_period, &x.chunk, &x.chunk_cap, &x.resort, &x.sync_every, &x.graph,
-                        &x.coll_batch, &x.tail_lanes) >= 9) {
+                        &x.coll_batch, &x.tail_lanes, &x.tail_quads) >= 9) {
             t = x;  // the last entry for a key wins
             found = true;
         }
@@ -900,6 +930,7 @@ static void autotune(mq_engine& e, const mq_job* jobs, size_t njobs, const doubl
     mq_tune base = e.tune;
     base.block = 256; base.waves_per_cu = 8; base.refill_period = 8; base.chunk = 256; base.chunk_cap = 256;
     base.resort = 1; base.sync_every = 16; base.graph = 0; base.coll_batch = 4; base.tail_lanes = 0;
+    base.tail_quads = 1;
     struct Knob {
         const char* name;
         std::vector<int> values;
@@ -920,6 +951,7 @@ static void autotune(mq_engine& e, const mq_job* jobs, size_t njobs, const doubl
         {"coll_batch", {1, 4, 8}, [](mq_tune& t, int v) { t.coll_batch = v; }},
         {"graph", {0, 1}, [](mq_tune& t, int v) { t.graph = v; }},
         {"tail_lanes", {0, 64}, [](mq_tune& t, int v) { t.tail_lanes = v; }},
+        {"tail_quads", {1, 0}, [](mq_tune& t, int v) { t.tail_quads = v; }},
     };
     const mq_tune saved = e.tune;
     std::vector<unsigned long long> acc;
@@ -956,9 +988,9 @@ static void autotune(mq_engine& e, const mq_job* jobs, size_t njobs, const doubl
     if (cache_path) {
         FILE* f = std::fopen(cache_path, "a");
         if (f) {
-            std::fprintf(f, "%s\t%d,%d,%d,%d,%d,%d,%d,%d,%d,%d\t%.6g\n", key.c_str(), best.block,
+            std::fprintf(f, "%s\t%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d\t%.6g\n", key.c_str(), best.block,
                          best.waves_per_cu, best.refill_period, best.chunk, best.chunk_cap, best.resort,
-                         best.sync_every, best.graph, best.coll_batch, best.tail_lanes, best_rate);
+                         best.sync_every, best.graph, best.coll_batch, best.tail_lanes, best.tail_quads, best_rate);
             std::fclose(f);
         }
     }
@@ -1228,6 +1260,8 @@ int mq_set_tune(mq_engine* eng, const mq_tune* t)
         }
         if (t->tail_lanes < 0 || t->tail_lanes > 64) throw Err(-1, "tail_lanes must be in [0, 64]");
         n.tail_lanes = t->tail_lanes;
+        if (t->tail_quads < 0 || t->tail_quads > 1) throw Err(-1, "tail_quads must be 0 or 1");
+        n.tail_quads = t->tail_quads;
         n.graph = t->graph;
         if (n.graph && (n.sync_every % 2)) throw Err(-1, "graph mode needs an even sync_every");
         eng->tune = n;

@@ -52,24 +52,29 @@ __device__ __forceinline__ void bin_rank(uint32_t* __restrict__ hist, uint32_t b
     rank_of[i] = base + below;
 }
 
-__device__ __forceinline__ float rget(const float4 R[CELL_F4], int k)
+// Record layout (component-major): float 24 d + 8 q + k holds coefficient k of
+// component d (x, y, z) of basis q (RF, combined DC, AC), so component d is the
+// six float4 R[6 d .. 6 d + 5] and basis q within it the pair R[6 d + 2 q],
+// R[6 d + 2 q + 1]; the corner conductor bits are R[18].x.
+__device__ __forceinline__ float poly8(const float4 lo, const float4 hi, float tx, float ty, float tz, float pxy,
+                                       float pyz, float pxz, float pxyz)
 {
-    const float4 v = R[k >> 2];
-    return (k & 3) == 0 ? v.x : ((k & 3) == 1 ? v.y : ((k & 3) == 2 ? v.z : v.w));
+    float f = lo.x;
+    f = fmaf(tx, lo.y, f);
+    f = fmaf(ty, lo.z, f);
+    f = fmaf(tz, lo.w, f);
+    f = fmaf(pxy, hi.x, f);
+    f = fmaf(pyz, hi.y, f);
+    f = fmaf(pxz, hi.z, f);
+    f = fmaf(pxyz, hi.w, f);
+    return f;
 }
 
-__device__ __forceinline__ float poly(const float4 R[CELL_F4], int base, float tx, float ty, float tz, float pxy,
+// Component d of basis q of the record at (tx, ty, tz).
+__device__ __forceinline__ float poly(const float4 R[CELL_F4], int q, int d, float tx, float ty, float tz, float pxy,
                                       float pyz, float pxz, float pxyz)
 {
-    float f = rget(R, base);
-    f = fmaf(tx, rget(R, base + 3), f);
-    f = fmaf(ty, rget(R, base + 6), f);
-    f = fmaf(tz, rget(R, base + 9), f);
-    f = fmaf(pxy, rget(R, base + 12), f);
-    f = fmaf(pyz, rget(R, base + 15), f);
-    f = fmaf(pxz, rget(R, base + 18), f);
-    f = fmaf(pxyz, rget(R, base + 21), f);
-    return f;
+    return poly8(R[6 * d + 2 * q], R[6 * d + 2 * q + 1], tx, ty, tz, pxy, pyz, pxz, pxyz);
 }
 
 }  // namespace mq

@@ -80,6 +80,9 @@ void launch_prepare(const WorkerArgs& a, hipStream_t st);
 void launch_gather_probe(const float4* cells, uint64_t window, uint32_t iters, float* out, uint32_t grid,
                          uint32_t block, hipStream_t st);
 void launch_batched(const WorkerArgs& a, uint32_t grid, uint32_t block, hipStream_t st);
+// The tail worker: four lanes per ion, 16 ions per wavefront (a.lanes quads of each
+// wavefront take ions); Philox streams, one queue (a.nqueues 1), refill period at most 8, no re-sort bins.
+void launch_tail4(const WorkerArgs& a, uint32_t grid, hipStream_t st);
 // The staged-tile worker (Philox streams): 256-thread workgroups, each taking 256
 // list entries per chunk; the counter ctr_in[0] hands out the chunks.
 void launch_staged(const WorkerArgs& a, uint32_t grid, hipStream_t st);

@@ -51,15 +51,15 @@ __global__ __launch_bounds__(128) void k_build_cells(float4* __restrict__ cells,
             const double f010 = F[q][3 * node[2] + d], f110 = F[q][3 * node[3] + d];
             const double f001 = F[q][3 * node[4] + d], f101 = F[q][3 * node[5] + d];
             const double f011 = F[q][3 * node[6] + d], f111 = F[q][3 * node[7] + d];
-            float* o = out + 24 * q + d;
+            float* o = out + 24 * d + 8 * q;  // component-major (fp32_cells.h)
             o[0] = (float)f000;
-            o[3] = (float)(f100 - f000);
-            o[6] = (float)(f010 - f000);
-            o[9] = (float)(f001 - f000);
-            o[12] = (float)((f110 - f100) - (f010 - f000));
-            o[15] = (float)((f011 - f010) - (f001 - f000));
-            o[18] = (float)((f101 - f100) - (f001 - f000));
-            o[21] = (float)(((f111 - f110) - (f101 - f100)) - ((f011 - f010) - (f001 - f000)));
+            o[1] = (float)(f100 - f000);
+            o[2] = (float)(f010 - f000);
+            o[3] = (float)(f001 - f000);
+            o[4] = (float)((f110 - f100) - (f010 - f000));
+            o[5] = (float)((f011 - f010) - (f001 - f000));
+            o[6] = (float)((f101 - f100) - (f001 - f000));
+            o[7] = (float)(((f111 - f110) - (f101 - f100)) - ((f011 - f010) - (f001 - f000)));
         }
     }
     uint32_t bits = 0;
@@ -143,9 +143,9 @@ __device__ __forceinline__ void cell_field(const float4 R[CELL_F4], const Lane& 
     const float pxy = L.tx * L.ty, pyz = L.ty * L.tz, pxz = L.tx * L.tz, pxyz = pxy * L.tz;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const float er = poly(R, d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
-        const float ed = poly(R, 24 + d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
-        const float ea = poly(R, 48 + d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+        const float er = poly(R, 0, d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+        const float ed = poly(R, 1, d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+        const float ea = poly(R, 2, d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
         e[d] = fmaf(L.a_ac, ea, fmaf(L.sdc, ed, L.a_rf * er));
     }
 }
@@ -195,6 +195,65 @@ __device__ __forceinline__ uint32_t countdown_f(float x)
     if (!(x < 2147483520.0f)) return 2147483647u;
     if (x < 1.0f) return 1u;
     return (uint32_t)x;
+}
+
+// Collision of a lane whose candidate fired at step pcur, on the Philox streams
+// (channel.metal:93-113): the thermal bath's acceptance test and elastic
+// scatter, or the legacy scatter and the next free path. Draws are keyed by
+// (ion, step), so where and when a worker runs it does not change the result.
+template <bool LEGACY>
+__device__ __forceinline__ void collide_philox(Lane& L, uint32_t pcur, float dtf)
+{
+    if constexpr (!LEGACY) {
+        const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
+        const u32x4 w = philox(L.ion, pcur, 0, D_COLL, L.k0, L.k1);
+        const float r1 = sqrtf(-2.0f * log_f(u01f(w.a))), r2 = sqrtf(-2.0f * log_f(u01f(w.c)));
+        const float a1 = u01f(w.b), a2 = u01f(w.d);
+        const float gx = L.vx - L.sgas * (r1 * cos_rev(a1));
+        const float gy = L.vy - L.sgas * (r1 * sin_rev(a1));
+        const float gz = L.vz - L.sgas * (r2 * cos_rev(a2));
+        const float gl = sqrtf(fmaf(gz, gz, fmaf(gy, gy, gx * gx)));
+        const u32x4 w2 = philox(L.ion, pcur, 1, D_COLL, L.k0, L.k1);
+        if (u01f(w2.a) * bound < gl) {
+            const float cth = 2.0f * u01f(w2.b) - 1.0f, a3 = u01f(w2.c);
+            const float sn = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
+            const float mg = L.mr * gl;
+            L.vx = fmaf(mg, sn * cos_rev(a3), fmaf(-L.mr, gx, L.vx));
+            L.vy = fmaf(mg, sn * sin_rev(a3), fmaf(-L.mr, gy, L.vy));
+            L.vz = fmaf(mg, cth, fmaf(-L.mr, gz, L.vz));
+            L.ncoll += 1;
+        }
+    } else {
+        const u32x4 w = philox(L.ion, pcur, 0, D_COLL, L.k0, L.k1);
+        const float vm = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx)));
+        if (vm > L.floorv) {
+            const float cth = 2.0f * u01f(w.a) - 1.0f, a3 = u01f(w.b);
+            const float sn = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
+            const float mv = L.mr * vm;
+            L.vx = fmaf(mv, sn * cos_rev(a3), L.vx * L.omr);
+            L.vy = fmaf(mv, sn * sin_rev(a3), L.vy * L.omr);
+            L.vz = fmaf(mv, cth, L.vz * L.omr);
+            L.ncoll += 1;
+        }
+        const float mfp = -log_f(fmaxf(u01f(w.c), 1e-7f)) * L.mfpm;
+        const float vm2 = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx)));
+        L.cd = countdown_f(floorf(mfp / fmaxf(vm2, 1e-3f) / dtf));
+    }
+}
+
+// Status of a lane after its step (channel.metal:114-120), in the reference's
+// order: return, exit, box, crash (the conductor bit of the nearest corner,
+// inside the crash window), then timeout. k_batched writes the same tests out
+// in its step loop.
+// Evaluated without branches: the tests are selected in reverse priority.
+__device__ __forceinline__ int classify(const BatchP& B, const Lane& L, uint32_t bits)
+{
+    int status = L.st >= B.max_steps ? 5 : 0;
+    status = (L.z > B.cz0 && L.z < B.cz1 && ((bits >> L.corner) & 1u)) ? 1 : status;
+    status = (L.x < 0.0 || L.y < 0.0 || L.z < 0.0 || L.x > B.lim0 || L.y > B.lim1 || L.z > B.lim2) ? 4 : status;
+    status = L.z > B.z_exit ? 3 : status;
+    status = L.z < B.z_ret ? 2 : status;
+    return status;
 }
 
 // LEGACY selects the collision model at compile time (legacy countdown or the
@@ -399,6 +458,9 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                 }
                 L.visit += 1;
                 // ---- classification (channel.metal:114-120)
+                // (written out here rather than through classify(): reading the record's
+                // conductor word only inside the crash window keeps the compiler from
+                // copying freshly loaded record registers, which would wait on the loads)
                 if (L.z < B.z_ret) status = 2;
                 else if (L.z > B.z_exit) status = 3;
                 else if (L.x < 0.0 || L.y < 0.0 || L.z < 0.0 || L.x > B.lim0 || L.y > B.lim1 || L.z > B.lim2) status = 4;
@@ -455,41 +517,7 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                         }
                         pending = false;
                     } else if (pending) {
-                        if constexpr (!LEGACY) {
-                            const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
-                            const u32x4 w = philox(L.ion, pcur, 0, D_COLL, L.k0, L.k1);
-                            const float r1 = sqrtf(-2.0f * log_f(u01f(w.a))), r2 = sqrtf(-2.0f * log_f(u01f(w.c)));
-                            const float a1 = u01f(w.b), a2 = u01f(w.d);
-                            const float gx = L.vx - L.sgas * (r1 * cos_rev(a1));
-                            const float gy = L.vy - L.sgas * (r1 * sin_rev(a1));
-                            const float gz = L.vz - L.sgas * (r2 * cos_rev(a2));
-                            const float gl = sqrtf(fmaf(gz, gz, fmaf(gy, gy, gx * gx)));
-                            const u32x4 w2 = philox(L.ion, pcur, 1, D_COLL, L.k0, L.k1);
-                            if (u01f(w2.a) * bound < gl) {
-                                const float cth = 2.0f * u01f(w2.b) - 1.0f, a3 = u01f(w2.c);
-                                const float sn = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
-                                const float mg = L.mr * gl;
-                                L.vx = fmaf(mg, sn * cos_rev(a3), fmaf(-L.mr, gx, L.vx));
-                                L.vy = fmaf(mg, sn * sin_rev(a3), fmaf(-L.mr, gy, L.vy));
-                                L.vz = fmaf(mg, cth, fmaf(-L.mr, gz, L.vz));
-                                L.ncoll += 1;
-                            }
-                        } else {
-                            const u32x4 w = philox(L.ion, pcur, 0, D_COLL, L.k0, L.k1);
-                            const float vm = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx)));
-                            if (vm > L.floorv) {
-                                const float cth = 2.0f * u01f(w.a) - 1.0f, a3 = u01f(w.b);
-                                const float sn = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
-                                const float mv = L.mr * vm;
-                                L.vx = fmaf(mv, sn * cos_rev(a3), L.vx * L.omr);
-                                L.vy = fmaf(mv, sn * sin_rev(a3), L.vy * L.omr);
-                                L.vz = fmaf(mv, cth, L.vz * L.omr);
-                                L.ncoll += 1;
-                            }
-                            const float mfp = -log_f(fmaxf(u01f(w.c), 1e-7f)) * L.mfpm;
-                            const float vm2 = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx)));
-                            L.cd = countdown_f(floorf(mfp / fmaxf(vm2, 1e-3f) / dtf));
-                        }
+                        collide_philox<LEGACY>(L, pcur, dtf);
                         pending = false;
                     }
                 }
@@ -529,6 +557,215 @@ __global__ __launch_bounds__(256) void k_batched(WorkerArgs A)
                 S.rng[4 * (size_t)id + 2] = L.xs2; S.rng[4 * (size_t)id + 3] = L.xs3;
             }
             S.status[id] = (uint8_t)status;
+            have = false;
+        }
+    }
+}
+
+// ------------------------------------------------- the tail worker, four lanes per ion
+// Once few ions are left, a wavefront's step costs the latency of its own
+// instruction chain (about 400 instructions), however few lanes hold ions. This
+// worker spreads each ion's step over a quad of lanes, 16 ions per wavefront:
+//  * lane d of the quad (d = 0, 1, 2; lane 3 repeats d = 2) holds component d
+//    of the cell record (the RF, DC and AC polynomials of E_d: six float4, the
+//    component-major layout of fp32_cells.h) plus the conductor bits, and
+//    evaluates E_d;
+//  * lanes 0 and 1 evaluate the RF and AC drive sines;
+//  * at each refill period lane k draws Philox candidate block (step / 4) + k of
+//    the ion, so the quad holds the candidate words of the period's steps;
+//  * results reach the quad by DPP quad broadcasts; everything else (kick,
+//    drift, cell location, collisions, classification) runs on all four lanes
+//    with identical inputs.
+// Every value an ion's step computes is the same operation on the same operands
+// as in k_batched, so the two workers give the same results bit for bit. Philox
+// streams only (the reference's streams draw sequentially per ion); one queue;
+// the refill period is at most 8 steps (three candidate blocks cover it).
+template <int S>
+__device__ __forceinline__ uint32_t qbu(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, S * 0x55, 0xF, 0xF, false);
+}
+template <int S>
+__device__ __forceinline__ float qbf(float v)
+{
+    return __uint_as_float(qbu<S>(__float_as_uint(v)));
+}
+
+// The quad's drive amplitudes at the lane's step: lane 0 evaluates the RF sine,
+// lane 1 the AC sine, from the tone (w, p0, v) each picked when the ion arrived
+// (RF: w_rf, phr, vrf; AC: w_ac, phi, vac).
+struct Tone {
+    double w, p0;
+    float v;
+};
+
+__device__ __forceinline__ void phases4(Lane& L, const Tone& T)
+{
+    double p = fma(T.w, (double)L.st, T.p0);
+    p = p - floor(p);
+    const float a = T.v * sin_rev((float)p);
+    L.a_rf = qbf<0>(a);
+    L.a_ac = qbf<1>(a);
+}
+
+__device__ __forceinline__ void load_part(const float4* __restrict__ cells, int32_t c, uint32_t d, float4 Rc[6],
+                                          uint32_t& bits)
+{
+    const float4* src = cells + (size_t)c * CELL_STRIDE_F4;
+    // the conductor bits first: the step's crash test then waits for that load only
+    bits = reinterpret_cast<const uint32_t*>(src + 18)[0];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Rc[q] = src[6 * d + q];
+}
+
+template <bool LEGACY>
+__global__ __launch_bounds__(256) void k_tail4(WorkerArgs A)
+{
+    const BatchP& B = A.B;
+    const IonState& S = A.S;
+    const uint32_t lane = threadIdx.x & 63, role = lane & 3, dc = role < 3 ? role : 2u;
+    const uint32_t count = A.ctr_in[CTR_COUNT];
+    const double dt = B.dt;
+    const float dtf = (float)B.dt;
+    Lane L;
+    Tone T{};
+    float4 Rc[6];
+    uint32_t bits = 0;
+    bool have = false, drained = false, pending = false;
+    // chk: the crash test of the quad's last step is outstanding; it needs the
+    // conductor bits of the cell just entered and is resolved at the next
+    // iteration (before that step's field, which needs the cell's record anyway),
+    // so the record load's latency overlaps the candidate and collision work
+    bool chk = false;
+    uint32_t pcur = 0;
+    L.cell = -1;
+    for (;;) {
+        // ---- refill idle quads from the live list (one atomic per wavefront)
+        if (!drained) {
+            const uint64_t idle = __ballot(role == 0 && !have && (lane >> 2) < A.lanes);
+            if (idle) {
+                const uint32_t nidle = (uint32_t)__popcll(idle);
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&A.ctr_in[0], nidle);
+                base = __shfl(base, (int)leader);
+                if (base >= count) drained = true;
+                uint32_t slot = 0xFFFFFFFFu;
+                if ((idle >> lane) & 1ull) slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                slot = qbu<0>(slot);  // the quad's claim, made by its lane 0
+                if (slot < count) {
+                    const uint32_t id = A.list_in[slot];
+                    const JobP& J = A.jobs[S.job[id]];
+                    L.id = id;
+                    L.ion = S.sid ? S.sid[id] : (uint32_t)(id - J.off);
+                    L.phr = S.phr ? S.phr[id] : 0.0;
+                    L.x = S.x[id]; L.y = S.y[id]; L.z = S.z[id]; L.phi = S.phi[id];
+                    L.vx = S.fvx[id]; L.vy = S.fvy[id]; L.vz = S.fvz[id];
+                    L.st = S.steps[id]; L.ncoll = S.ncoll[id];
+                    if constexpr (LEGACY) L.cd = S.cd[id];
+                    L.w_rf = J.w_rf; L.w_ac = J.w_ac;
+                    L.kq = (float)J.kq; L.vrf = (float)J.v_rf; L.vac = (float)J.v_ac; L.sdc = (float)J.s_dc;
+                    L.nsig = (float)J.nsig_dt; L.sgas = (float)J.s_gas; L.vb = (float)J.v_bound;
+                    L.mr = (float)J.mr; L.omr = (float)J.one_m_mr; L.floorv = (float)J.floor_v;
+                    L.mfpm = (float)J.mfp_mean;
+                    L.k0 = J.key0; L.k1 = J.key1;
+                    L.visit = 0;
+                    const int32_t c = locate(B, L);
+                    if (c != L.cell) { L.cell = c; load_part(A.cells, c, dc, Rc, bits); }
+                    T = role == 1 ? Tone{L.w_ac, L.phi, L.vac} : Tone{L.w_rf, L.phr, L.vrf};
+                    phases4(L, T);
+                    have = true;
+                }
+            }
+        }
+        if (!__any(have)) break;
+        // ---- the candidate words of this period: lane k draws block (st / 4) + k
+        uint32_t cb0 = 0;
+        u32x4 W{0u, 0u, 0u, 0u};
+        if constexpr (!LEGACY) {
+            if (have && L.nsig > 0.0f) {
+                cb0 = L.st >> 2;
+                W = philox(L.ion, cb0 + role, 0, D_CAND, L.k0, L.k1);
+            }
+        }
+        bool stop = false;
+        int status = 0;
+        for (uint32_t r = 0; r < A.period; ++r) {
+            if (chk) {
+                chk = false;
+                status = classify(B, L, bits);
+                if (status != 0 || L.visit >= A.chunk) stop = true;
+            }
+            if (have && !stop && !pending) {  // the same in the four lanes of a quad
+                const float pxy = L.tx * L.ty, pyz = L.ty * L.tz, pxz = L.tx * L.tz, pxyz = pxy * L.tz;
+                const float er = poly8(Rc[0], Rc[1], L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+                const float ed = poly8(Rc[2], Rc[3], L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+                const float ea = poly8(Rc[4], Rc[5], L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+                const float e = fmaf(L.a_ac, ea, fmaf(L.sdc, ed, L.a_rf * er));
+                L.vx = fmaf(L.kq, qbf<0>(e), L.vx);
+                L.vy = fmaf(L.kq, qbf<1>(e), L.vy);
+                L.vz = fmaf(L.kq, qbf<2>(e), L.vz);
+                L.x = fma((double)L.vx, dt, L.x);
+                L.y = fma((double)L.vy, dt, L.y);
+                L.z = fma((double)L.vz, dt, L.z);
+                const uint32_t cur = L.st;
+                L.st += 1;
+                const int32_t c = locate(B, L);
+                if (c != L.cell) { L.cell = c; load_part(A.cells, c, dc, Rc, bits); }
+                phases4(L, T);
+                if constexpr (!LEGACY) {
+                    if (L.nsig > 0.0f) {
+                        const float bound = sqrtf(fmaf(L.vz, L.vz, fmaf(L.vy, L.vy, L.vx * L.vx))) + L.vb;
+                        // word cur & 3 of block cur / 4, held by quad lane cur / 4 - cb0 (0..2)
+                        const uint32_t mine = pick(W, cur & 3), src = (cur >> 2) - cb0;
+                        const uint32_t w0 = qbu<0>(mine), w1 = qbu<1>(mine), w2 = qbu<2>(mine);
+                        const uint32_t cw = src == 0 ? w0 : (src == 1 ? w1 : w2);
+                        if (u01f(cw) < L.nsig * bound) { pending = true; pcur = cur; }
+                    }
+                } else {
+                    L.cd -= 1;
+                    if (L.cd == 0) { pending = true; pcur = cur; }
+                }
+                L.visit += 1;
+                chk = true;
+            }
+            // ---- batched collisions of the pending ions (counted once per quad)
+            const uint64_t pm = __ballot(pending && role == 0);
+            if (pm) {
+                const uint32_t npend = (uint32_t)__popcll(pm);
+                const uint64_t am = __ballot(have && !stop && !pending);
+                if (npend >= A.coll_batch || am == 0 || r + 1 == A.period) {
+                    if (pending) {
+                        collide_philox<LEGACY>(L, pcur, dtf);
+                        pending = false;
+                    }
+                }
+            }
+        }
+        if (chk) {  // the period's last step
+            chk = false;
+            status = classify(B, L, bits);
+            if (status != 0 || L.visit >= A.chunk) stop = true;
+        }
+        // ---- write back stopped ions; carry unfinished ones to the next launch
+        const bool carry = have && stop && status == 0;
+        const uint64_t cm = __ballot(carry && role == 0);
+        if (cm) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)cm) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&A.ctr_out[CTR_COUNT], (uint32_t)__popcll(cm));
+            base = __shfl(base, (int)leader);
+            if (carry && role == 0) A.list_out[base + (uint32_t)__popcll(cm & ((1ull << lane) - 1ull))] = L.id;
+        }
+        if (have && stop) {
+            if (role == 0) {
+                const uint32_t id = L.id;
+                S.x[id] = L.x; S.y[id] = L.y; S.z[id] = L.z;
+                S.fvx[id] = L.vx; S.fvy[id] = L.vy; S.fvz[id] = L.vz;
+                S.steps[id] = L.st; S.ncoll[id] = L.ncoll;
+                if constexpr (LEGACY) S.cd[id] = L.cd;
+                S.status[id] = (uint8_t)status;
+            }
             have = false;
         }
     }
@@ -726,6 +963,12 @@ void launch_batched(const WorkerArgs& a, uint32_t grid, uint32_t block, hipStrea
         if (a.B.legacy_coll) k_batched<true, false, false><<<grid, block, 0, st>>>(a);
         else k_batched<false, false, false><<<grid, block, 0, st>>>(a);
     }
+}
+
+void launch_tail4(const WorkerArgs& a, uint32_t grid, hipStream_t st)
+{
+    if (a.B.legacy_coll) k_tail4<true><<<grid, 256, 0, st>>>(a);
+    else k_tail4<false><<<grid, 256, 0, st>>>(a);
 }
 
 void launch_staged(const WorkerArgs& a, uint32_t grid, hipStream_t st)

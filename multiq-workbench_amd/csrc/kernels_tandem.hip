@@ -399,9 +399,9 @@ __global__ __launch_bounds__(256) void k_tandem_batched(TBatchArgs A)
                 float e[3];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    const float er = poly(R, d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
-                    const float ed = poly(R, 24 + d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
-                    const float ea = poly(R, 48 + d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+                    const float er = poly(R, 0, d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+                    const float ed = poly(R, 1, d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
+                    const float ea = poly(R, 2, d, L.tx, L.ty, L.tz, pxy, pyz, pxz, pxyz);
                     e[d] = fmaf(L.a_ac, ea, fmaf(L.a_rf, er, ed));
                 }
                 L.vx = fmaf(L.kq, e[0], L.vx);

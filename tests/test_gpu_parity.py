@@ -206,6 +206,42 @@ def test_batched_tail_switch_keeps_results(c1):
     assert sorted_run["steps"].max() > 64 * 8   # the run outlasted several re-sort intervals
 
 
+@pytest.mark.parametrize("case, n", [("half_thermal", 40000), ("legacy", 15000)])
+def test_batched_quad_tail_keeps_results(c1, case, n):
+    """The four-lane tail worker (k_tail4: each ion's step split over a quad of lanes, packed 16 ions per
+    wavefront, 1024-step launches) takes over once at most 16 ions per wavefront are left. Per-ion results
+    equal, bit for bit, those of the one-lane worker and of launches that never re-sort, for the thermal
+    and the legacy collision models."""
+    g, e = c1
+    ph = physics(case)
+    jobs = jobs_for(case, n)   # 1.5 n ions (the legacy ions end within a narrower span of steps)
+    t0 = e.get_tune()
+    say(f"[gpu] batched path, {case}: four-lane tail vs one lane")
+    # 256 CUs x 3 waves x 64 lanes = 49,152 lanes; the four-lane tail holds 16 x 768 = 12,288 ions
+    shape = dict(block=64, waves_per_cu=3, refill_period=8, chunk=64, chunk_cap=64, resort=1, sync_every=1,
+                 graph=0, tail_lanes=0)
+    try:
+        e.set_tune(tail_quads=1, **shape)
+        quad = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+        st_quad = e.stats()
+        e.set_tune(tail_quads=0, **shape)
+        one = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+        st_one = e.stats()
+        e.set_tune(block=256, waves_per_cu=8, refill_period=8, chunk=20000, chunk_cap=200000, resort=0,
+                   sync_every=2, graph=0, tail_lanes=64, tail_quads=0)
+        plain_launches = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+    finally:
+        e.set_tune(**t0)
+    say(f"[gpu]   {st_quad['launches']} launches, {st_quad['quad_launches']} of them four-lane")
+    assert st_quad["quad_launches"] > 0 and st_one["quad_launches"] == 0
+    assert st_quad["ion_steps"] == st_one["ion_steps"]
+    for k in KEYS + ("steps", "ncoll", "status"):
+        assert np.array_equal(quad[k], one[k]), k
+        assert np.array_equal(quad[k], plain_launches[k]), k
+    if case == "legacy":
+        assert quad["ncoll"].sum() > 0
+
+
 def test_plain_independent_of_batch(c1):
     g, e = c1
     ph = physics("legacy")
