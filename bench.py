@@ -385,8 +385,9 @@ def read_profile_value(name: str, key: str):
 
 
 def roofline(name: str, steps: int, kms: float, launches: int) -> dict:
-    """Roofline object of the worker kernel k_batched over a timed run (steps ion-steps, kms ms of worker-kernel
-    time from HIP events on the engine's stream, `launches` launches).
+    """Roofline object of the worker over a timed run (steps ion-steps, kms ms of worker-kernel time from HIP events
+    on the engine's stream, `launches` launches): k_batched, and k_tail4 (four lanes per ion) for the launches once
+    at most 16 ions per wavefront are live.
 
     achieved = HBM bytes per ion-step measured by rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE, scaled by a copy
     kernel of known size; profiles/k_batched_<config>_traffic.md) x this run's ion-steps / this run's kernel time.
@@ -398,7 +399,7 @@ def roofline(name: str, steps: int, kms: float, launches: int) -> dict:
     l2_per_step = read_profile_value(name, "l2_read_bytes_per_ion_step")
     t = kms * 1e-3
     demand = steps * BYTES_PER_STEP_FP32 / t / 1e9
-    r = {"bound": "hbm", "kernel": "k_batched", "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    r = {"bound": "hbm", "kernel": "k_batched (+ k_tail4 in the last launches)", "peak": HBM_PEAK_GBS, "unit": "GB/s",
          "ion_steps_per_launch": steps / max(launches, 1), "avg_launch_ms": kms / max(launches, 1),
          "ion_steps_per_s_kernel": steps / t}
     if per_step:
