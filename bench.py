@@ -317,6 +317,91 @@ def reference_geometry_bench():
     return out
 
 
+def drivers_bench():
+    """The drivers around the path, timed on the engine (each is parity-tested in tests/test_gpu_*):
+    * live: mqviz's block loop (engine.rs:448-538) on the Fig-4 grid at a 0.02 V gate, two species, Poisson
+      injection at 5e8 ions/s, 128-step blocks of 100 ns steps; simulated time per wall second and ion-steps/s over
+      40 timed blocks after 200 warm ones.
+    * optimize_grid: one Optimize experiment (research.rs:133-163) on the corrected S14 channel: 4 AC amplitudes x
+      AC-frequency scans of 70-100 kHz at 0.5 kHz x 500 ions per point (+ the Na adduct's 500 for the leakage),
+      fits, rescans of narrow peaks, and the confirmation with 3x the ions.
+    * optimize_grad: the gradient search over f_exc on the differentiable form (16,384 ions, 2,000 steps), started
+      2 kHz from the optimum.
+    * adaptive_scan: scan::adaptive_scan (scan.rs:52-78) on the Fig-4 grid at a 0.02 V gate, 80-120 kHz at 1 kHz
+      coarse with 2,000 ions, refined at 0.25 kHz with 8,000 ions where T >= 0.35 Tmax and above the baseline, into
+      a fresh result cache."""
+    import tempfile
+    out = {}
+    # live population
+    g1 = synth.c1_grid()
+    e = mqhip.Engine(g1)
+    try:
+        ph = mqhip.Physics.correct(0.133322).copy(dt=1e-7, max_steps=20000, injection=mqhip.INJ_GAUSSIAN,
+                                                  inj_sigma_mm=0.1, legacy_force=1)
+        sp = [mqhip.Species(499.0, 1.0, 230.0), mqhip.Species(505.0, 1.0, 235.0)]
+        live = mqhip.Live(e, sp, [0.6, 0.4], mqhip.RfAc(100.0, 500e3, 0.5, 100e3), [0, 0.02, 0, 0], ph,
+                          block_steps=128, capacity=1 << 20, inj_rate=5e8, seed=5)
+        try:
+            live.run(200)                     # warm: the population approaches its steady size
+            s0 = live.run(0)
+            t0 = time.perf_counter()
+            s1 = live.run(40)
+            wall = time.perf_counter() - t0
+        finally:
+            live.close()
+        sim = (s1["step"] - s0["step"]) * ph.dt
+        out["live"] = {"blocks_per_s": 40 / wall, "simulated_s_per_wall_s": sim / wall,
+                       "ion_steps_per_s": (s1["ion_steps"] - s0["ion_steps"]) / wall, "alive_end": s1["alive"],
+                       "injected_in_timed_blocks": s1["injected"] - s0["injected"]}
+        # gradient search
+        phg = mqhip.Physics.correct(0.0).copy(dt=1e-7, injection=mqhip.INJ_GAUSSIAN, inj_sigma_mm=0.1,
+                                              legacy_force=1)
+        obj = mqhip.Objective(2000, 0, 18.6e-3, 2e-4, 0.5e-3, -1.0)
+        dcg = [0.0, 1.0, 0.0, 0.0]
+        jg = mqhip.job(499.0, 1.0, 0.0, 100.0, 500e3, 0.5, 94e3, 16384, 4499, dc_scale=0.5)
+        e.forward(jg, dcg, phg, obj)  # warm: tables of the differentiable form
+        t0 = time.perf_counter()
+        r = mqhip.optimize_grad(e, jg, dcg, phg, obj, mask=1 << 3, lo=[0, 0, 0, 90e3, 0],
+                                hi=[1e3, 1e7, 10, 104e3, 10], step0=[1, 1, 1, 1e3, 1], max_iter=40, tol=1e-3)
+        wall = time.perf_counter() - t0
+        out["optimize_grad"] = {"wall_s": wall, "evaluations": r["n_eval"], "s_per_evaluation": wall / r["n_eval"],
+                                "f_exc_khz": r["theta"][3] / 1e3, "J": r["J"]}
+        # adaptive scan into a fresh cache
+        with tempfile.TemporaryDirectory() as td:
+            cache = os.path.join(td, "scan.tsv")
+            t0 = time.perf_counter()
+            pts = mqhip.adaptive_scan(e, "bench-fig4-grid", mqhip.Species(499.0, 1.0, 230.0),
+                                      mqhip.RfAc(100.0, 500e3, 0.5, 100e3), [0, 0.02, 0, 0], ph, 80.0, 120.0, 1.0,
+                                      2000, 8000, -1.0, 100, cache)
+            wall = time.perf_counter() - t0
+        ions = int(sum(p[3] for p in pts))
+        out["adaptive_scan"] = {"wall_s": wall, "points": len(pts), "ions": ions,
+                                "trajectories_per_s": ions / wall}
+    finally:
+        e.close()
+    # one Optimize experiment on the S14 channel
+    g14 = mqhip.channel_grid(mqhip.Geom.s14(), tol=1e-7)
+    e = mqhip.Engine(g14)
+    try:
+        sp = mqhip.Species(2 * 863.473, 2.0, 425.0)
+        na = mqhip.Species(2 * 863.473 + 21.982, 2.0, 430.0)
+        rf = mqhip.RfAc(70.0, 500e3, 0.35, 84.25e3)
+        pho = mqhip.Physics.correct(0.1).copy(t_max=3e-3)
+        t0 = time.perf_counter()
+        res = mqhip.optimize_grid(e, sp, rf, [18, 24, 15, 20], pho, [0.35, 0.2625, 0.175, 0.0875], 70.0, 100.0, 0.5,
+                                  500, 5.0, 0xA1100101, leak=na, leak_n=500)
+        wall = time.perf_counter() - t0
+        rows = res["rows"] + [res["confirmation"]]
+        npts = sum(int(round((r["hi_khz"] - r["lo_khz"]) / r["step_khz"])) + 1 for r in rows)
+        out["optimize_grid"] = {"wall_s": wall, "rows": len(res["rows"]), "best_row": res["best"],
+                                "confirmed": bool(res["confirmed"]), "scan_points_approx": npts,
+                                "best_r": res["rows"][res["best"]]["r_mz"] if res["best"] >= 0 else None}
+    finally:
+        e.close()
+    log(f"drivers: {out}")
+    return out
+
+
 LEGACY_BYTES_PER_UPDATE = 9  # read one other-colour value, write one, read one electrode flag (split layout)
 
 
@@ -519,6 +604,7 @@ def main():
     ap.add_argument("--skip-tandem", action="store_true")
     ap.add_argument("--skip-refgeom", action="store_true")
     ap.add_argument("--skip-legacy", action="store_true")
+    ap.add_argument("--skip-drivers", action="store_true", help="skip the live / Optimize / adaptive-scan timings")
     ap.add_argument("--skip-c3", action="store_true", help="skip the 1M-trajectory sweep sub-object")
     ap.add_argument("--skip-transmit", action="store_true", help="skip the transmitting mass-scan sub-object")
     ap.add_argument("--legacy-only", action="store_true",
@@ -580,6 +666,8 @@ def main():
         result["reference_geometry"] = reference_geometry_bench()
     if not a.skip_legacy:
         result["legacy_potentials"] = legacy_potentials_bench()
+    if not a.skip_drivers:
+        result["drivers"] = drivers_bench()
     if not a.skip_cpu:
         result["cpu_baseline"] = cpu_baseline(grid, jobs, dc, ph, a.cpu_seconds)
     print(json.dumps(result), flush=True)
