@@ -271,7 +271,12 @@ int mq_autotune(mq_engine* eng, const mq_job* jobs, size_t njobs, const double d
  * for exactly nsteps FP64 steps (no termination), then
  *   J = mean_i [ sigmoid((z_i - z_exit) / scale)
  *               - w_leak * sigmoid((r_i - r_leak) / scale) ],
- * r_i = distance from the injection axis. ckpt_every = 0 picks ceil(sqrt(nsteps)).
+ * r_i = distance from the injection axis. Checkpoints: ckpt_every = 1 keeps
+ * every step's position (24 B per ion-step, 48 B with gas) so that mq_backward
+ * needs no recompute; 0 does so when that fits a quarter of the free device
+ * memory and otherwise checkpoints every ceil(sqrt(nsteps)) steps, recomputing
+ * each segment in mq_backward; K > 1 checkpoints every K steps. The results are
+ * the same bit for bit.
  * With no gas (pressure 0 or ccs_n2 0) the steps are collision-free. With gas
  * the thermal collisions of the plain path apply with common random numbers:
  * each ion's Philox draws depend on (seed, ion, step) only, the candidate and

@@ -503,9 +503,21 @@ struct mq_engine {
             throw Err(-1, "the differentiable form supports the thermal collision model only (legacy_collisions = 0)");
         if (ph.rng != MQ_RNG_PHILOX) throw Err(-1, "the differentiable form injects with the Philox stream (rng = 0)");
         const uint32_t n = (uint32_t)job.n;
-        const uint32_t K = o.ckpt_every ? o.ckpt_every : (uint32_t)std::ceil(std::sqrt((double)o.nsteps));
-        const uint32_t slots = (o.nsteps + K - 1) / K;
         JobP jp = job_params(job, ph, 0);
+        const int nseg = jp.nsig_dt > 0.0 ? 6 : 3;
+        // Checkpoints: 1 keeps the whole trajectory (the forward writes every step's position, and with gas its
+        // post-kick velocity; the backward walks them without a recompute); 0 does so when that fits a quarter of
+        // the free device memory, else checkpoints every ceil(sqrt(nsteps)) steps; K > 1 checkpoints every K.
+        bool full = o.ckpt_every == 1;
+        if (o.ckpt_every == 0) {
+            size_t free_b = 0, total_b = 0;
+            HIPCHK(hipMemGetInfo(&free_b, &total_b));
+            const double traj = (double)nseg * (double)o.nsteps * (double)n * sizeof(double);
+            full = traj <= 0.25 * (double)(free_b + adj_mem.bytes);
+        }
+        const uint32_t K = full ? o.nsteps
+                                : (o.ckpt_every ? o.ckpt_every : (uint32_t)std::ceil(std::sqrt((double)o.nsteps)));
+        const uint32_t slots = (o.nsteps + K - 1) / K;
         jobs.ensure(sizeof(JobP));
         HIPCHK(hipMemcpyAsync(jobs.p, &jp, sizeof(JobP), hipMemcpyHostToDevice, stream));
         prepare_dc(dc, false);
@@ -518,7 +530,6 @@ struct mq_engine {
         HIPCHK(hipGetLastError());
         // buffers: 6 slots of checkpoints, 3 x K segment (6 x K with gas: the post-kick velocities too),
         // 3 final, J, 5 + 6 gradients
-        const int nseg = jp.nsig_dt > 0.0 ? 6 : 3;
         const size_t per_ion = (size_t)6 * slots + (size_t)nseg * K + 3 + 1 + 5 + 6;
         adj_mem.ensure(per_ion * n * sizeof(double));
         double* p = adj_mem.as<double>();
@@ -533,6 +544,7 @@ struct mq_engine {
         O.z_exit = o.z_exit; O.scale = o.scale; O.r_leak = o.r_leak; O.w_leak = o.w_leak;
         O.axis_x = meta.axis_x; O.axis_y = meta.axis_y; O.nsteps = o.nsteps; O.ckpt = K;
         O.gas = nseg == 6 ? 1u : 0u;
+        O.full = full ? 1u : 0u;
         BatchP B = batch_params(ph, 1);
         const AdjGrids G = adj_grids();
         stats = mq_run_stats{};
@@ -579,7 +591,7 @@ struct mq_engine {
         stats.n_ions = n;
         stats.kernel_ms = ms;
         stats.launches = 1;
-        stats.ion_steps = 2ull * n * adj_obj.nsteps;  // recompute + backward
+        stats.ion_steps = (adj_obj.full ? 1ull : 2ull) * n * adj_obj.nsteps;  // (recompute +) backward
         stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
     }
 

@@ -102,10 +102,13 @@ struct AdjObj {
     double z_exit, scale, r_leak, w_leak, axis_x, axis_y;
     uint32_t nsteps, ckpt;
     uint32_t gas;  // 1: thermal collisions with frozen draws (seg[3..5] allocated)
+    uint32_t full; // 1: one segment of nsteps (ckpt = nsteps) that the forward itself fills, so the backward
+                   // walks the stored trajectory without recomputing it
 };
 struct AdjBuf {
     double* ck[6];   // checkpoints (x, y, z, vx, vy, vz) [slot * n + ion]
-    double* seg[6];  // recomputed positions of one segment [k * n + ion]; with gas also the post-kick velocities
+    double* seg[6];  // positions of one segment [k * n + ion] (recomputed, or with `full` the whole trajectory
+                     // written by the forward); with gas also the post-kick velocities
     double* fin[3];  // final position
     double* Ji;      // objective per ion
     double* gp[5];   // per-ion dJ/d(V_rf, f_rf, V_ac, f_ac, s_dc)

@@ -21,7 +21,10 @@
 // sqrt(N) checkpointing: the forward stores (x, v) every K steps; the backward
 // recomputes each K-step segment from its checkpoint into a per-ion buffer and
 // walks it backwards, so memory is O((N/K + K) ions) and work is 2 forwards
-// plus one backward.
+// plus one backward. Where the whole trajectory fits in device memory (24 B per
+// ion-step, 48 B with gas: 31 GB for 65,536 ions x 20,000 steps) the forward
+// writes every step's position itself and the backward walks it without a
+// recompute (AdjObj.full): one forward plus one backward.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -306,8 +309,21 @@ __global__ __launch_bounds__(256) void k_adj_forward(IonState S, const JobP* __r
             const size_t o = (size_t)(s / O.ckpt) * n + i;
             A.ck[0][o] = x; A.ck[1][o] = y; A.ck[2][o] = z; A.ck[3][o] = vx; A.ck[4][o] = vy; A.ck[5][o] = vz;
         }
+        // the trajectory slot of step s (with O.full), written with non-temporal stores: the stream of
+        // positions (31 GB for 65,536 ions x 20,000 steps) would otherwise evict the cell records from L2
+        const size_t o = (size_t)s * n + i;
+        if (O.full) {
+            __builtin_nontemporal_store(x, &A.seg[0][o]);
+            __builtin_nontemporal_store(y, &A.seg[1][o]);
+            __builtin_nontemporal_store(z, &A.seg[2][o]);
+        }
         step_fwd(B, J, G, K, phi, s, x, y, z, vx, vy, vz);
         if constexpr (GAS) {
+            if (O.full) {
+                __builtin_nontemporal_store(vx, &A.seg[3][o]);
+                __builtin_nontemporal_store(vy, &A.seg[4][o]);
+                __builtin_nontemporal_store(vz, &A.seg[5][o]);
+            }
             double gh[3], d[3];
             (void)collide_frozen(J, ion, s, vx, vy, vz, gh, d);
         }
@@ -386,8 +402,8 @@ __global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __
         const uint32_t s1 = min(O.nsteps, s0 + O.ckpt);
         const size_t oc = (size_t)seg * n + i;
         double x = A.ck[0][oc], y = A.ck[1][oc], z = A.ck[2][oc], vx = A.ck[3][oc], vy = A.ck[4][oc], vz = A.ck[5][oc];
-        // recompute the segment's positions x_s, s0 <= s < s1
-        for (uint32_t s = s0; s < s1; ++s) {
+        // recompute the segment's positions x_s, s0 <= s < s1 (with O.full the forward has stored them)
+        for (uint32_t s = s0; !O.full && s < s1; ++s) {
             const size_t o = (size_t)(s - s0) * n + i;
             A.seg[0][o] = x; A.seg[1][o] = y; A.seg[2][o] = z;
             step_fwd(B, J, G, K, phi, s, x, y, z, vx, vy, vz);
@@ -400,7 +416,9 @@ __global__ __launch_bounds__(256) void k_adj_backward(IonState S, const JobP* __
         // walk it backwards (the look-ahead index is clamped to s0: a redundant read in bounds)
         const auto pos = [&](int s, double& px, double& py, double& pz) {
             const size_t o = (size_t)((uint32_t)max(s, (int)s0) - s0) * n + i;
-            px = A.seg[0][o]; py = A.seg[1][o]; pz = A.seg[2][o];
+            px = __builtin_nontemporal_load(&A.seg[0][o]);
+            py = __builtin_nontemporal_load(&A.seg[1][o]);
+            pz = __builtin_nontemporal_load(&A.seg[2][o]);
         };
         double px, py, pz;
         pos((int)s1 - 1, px, py, pz);
