@@ -206,7 +206,8 @@ def test_batched_tail_switch_keeps_results(c1):
     assert sorted_run["steps"].max() > 64 * 8   # the run outlasted several re-sort intervals
 
 
-@pytest.mark.parametrize("case, n", [("half_thermal", 40000), ("legacy", 15000)])
+@pytest.mark.parametrize("case, n", [("half_thermal", 40000), ("legacy", 15000), ("half_thermal_nearest", 40000),
+                                     ("full_thermal", 40000)])
 def test_batched_quad_tail_keeps_results(c1, case, n):
     """The four-lane tail worker (k_tail4: each ion's step split over a quad of lanes, packed 16 ions per
     wavefront, 1024-step launches) takes over once at most 16 ions per wavefront are left. Per-ion results
@@ -240,6 +241,28 @@ def test_batched_quad_tail_keeps_results(c1, case, n):
         assert np.array_equal(quad[k], plain_launches[k]), k
     if case == "legacy":
         assert quad["ncoll"].sum() > 0
+
+
+def test_quad_tail_not_used_with_reference_streams(c1):
+    """The reference's per-ion streams draw sequentially, which the four-lane worker's split of the Philox blocks
+    does not apply to: with MQ_RNG_COMPAT the tail stays on one lane per ion, whatever tail_quads says."""
+    g, e = c1
+    ph = physics("half_thermal").copy(rng=mqhip.RNG_COMPAT)
+    jobs = jobs_for("half_thermal", 20000)
+    t0 = e.get_tune()
+    shape = dict(block=64, waves_per_cu=3, refill_period=8, chunk=64, chunk_cap=64, resort=1, sync_every=1,
+                 graph=0, tail_lanes=0)
+    try:
+        e.set_tune(tail_quads=1, **shape)
+        a = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+        st = e.stats()
+        e.set_tune(tail_quads=0, **shape)
+        b = e.run_batch_ions(jobs, [0, 5, 0, 0], ph)
+    finally:
+        e.set_tune(**t0)
+    assert st["quad_launches"] == 0 and st["launches"] >= 4   # re-sorted launches, then the tail
+    for k in KEYS + ("steps", "ncoll", "status"):
+        assert np.array_equal(a[k], b[k]), k
 
 
 def test_plain_independent_of_batch(c1):
