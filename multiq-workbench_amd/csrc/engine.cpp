@@ -804,9 +804,9 @@ struct mq_engine {
             uint32_t act = 64;
             // Once at most 16 ions per wavefront are left (Philox streams), each takes four lanes
             // (k_tail4): a wavefront's step then runs about half the instructions. Its wavefronts are
-            // packed with 16 ions each and only as many are launched as the ions need, so that below
-            // 16 per SIMD every SIMD steps one wavefront; a fixed chunk of QUAD_CHUNK steps repacks the
-            // shrinking population at every launch.
+            // packed with 16 ions each and only as many are launched as the ions fill, so that once they
+            // fill no more wavefronts than there are SIMDs each busy SIMD steps one; a fixed chunk of
+            // QUAD_CHUNK steps repacks the shrinking population at every launch.
             const bool quad = tail && tune.tail_quads && !B.compat && alive_last <= 16 * waves;
             uint32_t quad_grid = 0, qchunk = QUAD_CHUNK;
             if (quad) {
