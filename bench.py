@@ -15,8 +15,9 @@ object of the batched worker kernel and the CPU baseline (the FP64 oracle on
 the host's cores over a bounded sample of the same jobs). Sub-objects: the
 differentiable form (gradient evaluations/s), the tandem multi-domain batch
 (1M ions), the reference channel geometry solved on the GPU with the two
-numbers the reference recorded for it, and the legacy Java potentials
-regenerated on the GPU with the published Fig. 4 columns on them.
+numbers the reference recorded for it, the legacy Java potentials
+regenerated on the GPU with the published Fig. 4 columns on them, and the
+drivers around the path (live population, Optimize searches, adaptive scan).
 """
 from __future__ import annotations
 
