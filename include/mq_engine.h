@@ -133,12 +133,14 @@ typedef struct {
     double kernel_ms;        /* device time of the worker kernels (HIP events) */
     double wall_ms;          /* host wall time of the call */
     uint32_t quad_launches;  /* of the launches, those of the four-lane tail worker (mq_tune.tail_quads) */
+    uint32_t plain_recs;     /* plain path: 1 if its launches read the FP64 cell records (mq_tune.plain_recs) */
 } mq_run_stats;
 
-/* Tuning of the batched path (TileCfg, tiled.rs:12-29). mq_set_tune: zero
- * in block, waves_per_cu, refill_period, chunk, chunk_cap, sync_every and
- * coll_batch keeps the current value; resort, graph, tail_lanes and tail_quads
- * are always taken. Results per ion do not depend on any of these. */
+/* Tuning of the batched path (TileCfg, tiled.rs:12-29), and the plain path's
+ * field access. mq_set_tune: zero in block, waves_per_cu, refill_period,
+ * chunk, chunk_cap, sync_every and coll_batch keeps the current value; resort,
+ * graph, tail_lanes, tail_quads and plain_recs are always taken. Results per
+ * ion do not depend on any of these. */
 typedef struct {
     int32_t block;           /* threads per workgroup (64..256, multiple of 64) */
     int32_t waves_per_cu;    /* persistent waves launched per CU */
@@ -163,6 +165,11 @@ typedef struct {
                                 each on four lanes (the step's field components, drive sines
                                 and candidate draws split over the lanes; Philox runs only);
                                 0: one lane per ion throughout */
+    int32_t plain_recs;      /* plain path: 1 (default): per-cell FP64 records of the 8 corner
+                                values of the three bases (576 B per cell, built once per
+                                combined DC when device memory allows), kept per lane while
+                                its ion stays in the cell; 0: gather the node tables at
+                                every step. The same arithmetic either way */
 } mq_tune;
 
 typedef struct mq_engine mq_engine;

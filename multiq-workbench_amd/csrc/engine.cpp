@@ -130,7 +130,7 @@ struct mq_engine {
     // per-call pools
     DevBuf jobs, acc, list[3], ctr, bin_of, rank_of, hist, bsum;
     DevBuf sx, sy, sz, sphi, svx, svy, svz, sfx, sfy, sfz, ssteps, sncoll, scd, sjob, sstatus, srng;
-    mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0, 4, 0, 1};
+    mq_tune tune{256, 8, 8, 256, 256, 1, 16, 0, 4, 0, 1, 1};
     // no-alloc entry point: this device's scratch array and host staging
     // reserved at creation for MQ_NOALLOC_MAX_JOBS jobs
     void* scratch_dev = nullptr;
@@ -148,6 +148,9 @@ struct mq_engine {
     // exactly a float: -1 not checked yet, 0 not exact, 1 exact (records built)
     int rfac32_state = -1, dcc32_state = -1;
     DevBuf adj_rec, exact_flag;
+    // FP64 cell records of the plain path for the current combined DC
+    DevBuf plain_rec;
+    bool plain_rec_valid = false;
     std::vector<hipEvent_t> evs;
     mq_run_stats stats{};
 
@@ -235,6 +238,7 @@ struct mq_engine {
             HIPCHK(hipGetLastError());
             dc_valid = true;
             dcc32_state = -1;
+            plain_rec_valid = false;
             cells_valid = false;
         }
         if (want_cells && !cells_valid) {
@@ -431,8 +435,13 @@ struct mq_engine {
         stats.n_ions = n;
         float kms = 0.0f;
         if (n) {
-            if (path == MQ_PATH_PLAIN) run_plain(cb, B, n, kms);
-            else run_batched(cb, B, n, kms);
+            if (path == MQ_PATH_PLAIN) {
+                const double2* rec = plain_records(noalloc);
+                stats.plain_recs = rec ? 1u : 0u;
+                run_plain(cb, B, n, kms, rec);
+            } else {
+                run_batched(cb, B, n, kms);
+            }
         }
         launch_summarize(S, cb.acc, n, stream);
         HIPCHK(hipGetLastError());
@@ -453,7 +462,34 @@ struct mq_engine {
         return S;
     }
 
-    void run_plain(const CallBufs& cb, const BatchP& B, uint32_t n, float& kms)
+    // The plain path's FP64 cell records (72 doubles per cell, 576 B): built for
+    // the current combined DC when tune.plain_recs asks for them and device
+    // memory allows (at most half of what is free); with noalloc only in a buffer
+    // that an earlier call allocated. nullptr: the kernel gathers from the node
+    // tables instead, with the same results.
+    const double2* plain_records(bool noalloc)
+    {
+        if (!tune.plain_recs) return nullptr;
+        const uint64_t nc = (uint64_t)(nx - 1) * (ny - 1) * (nz - 1);
+        const size_t need = nc * PLAIN_REC_D2 * sizeof(double2);
+        if (plain_rec.bytes < need) {
+            if (noalloc) return nullptr;
+            size_t free_b = 0, total_b = 0;
+            HIPCHK(hipMemGetInfo(&free_b, &total_b));
+            if (need > free_b / 2) return nullptr;
+            plain_rec.ensure(need);
+            plain_rec_valid = false;
+        }
+        if (!plain_rec_valid) {
+            launch_build_plain_recs(plain_rec.as<double2>(), rf.as<double>(), dcc.as<double>(), ac.as<double>(), nx,
+                                    ny, nz, stream);
+            HIPCHK(hipGetLastError());
+            plain_rec_valid = true;
+        }
+        return plain_rec.as<double2>();
+    }
+
+    void run_plain(const CallBufs& cb, const BatchP& B, uint32_t n, float& kms, const double2* rec)
     {
         uint32_t nsteps = (uint32_t)std::max(1000.0, std::floor(2e-4 / B.dt));
         nsteps = (nsteps + 3u) & ~3u;
@@ -462,8 +498,8 @@ struct mq_engine {
         for (;;) {
             launch_fill_u32(live, 0u, 1, stream);
             HIPCHK(hipEventRecord(ev0, stream));
-            launch_plain(cb.S, cb.jobs, B, rf.as<double>(), dcc.as<double>(), ac.as<double>(), mask.as<uint8_t>(), n,
-                         nsteps, live, stream);
+            launch_plain(cb.S, cb.jobs, B, rf.as<double>(), dcc.as<double>(), ac.as<double>(), rec,
+                         mask.as<uint8_t>(), n, nsteps, live, stream);
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(ev1, stream));
             stats.launches += 1;
@@ -925,6 +961,7 @@ static void autotune(mq_engine& e, const mq_job* jobs, size_t njobs, const doubl
     const std::string key = tune_key(e, ph);
     best_rate = 0.0;
     if (!force && read_tune_cache(cache_path, key, best)) {
+        best.plain_recs = e.tune.plain_recs;  // the plain path's setting is not part of the entries
         e.tune = best;
         return;
     }
@@ -1274,6 +1311,8 @@ int mq_set_tune(mq_engine* eng, const mq_tune* t)
         n.tail_lanes = t->tail_lanes;
         if (t->tail_quads < 0 || t->tail_quads > 1) throw Err(-1, "tail_quads must be 0 or 1");
         n.tail_quads = t->tail_quads;
+        if (t->plain_recs < 0 || t->plain_recs > 1) throw Err(-1, "plain_recs must be 0 or 1");
+        n.plain_recs = t->plain_recs;
         n.graph = t->graph;
         if (n.graph && (n.sync_every % 2)) throw Err(-1, "graph mode needs an even sync_every");
         eng->tune = n;

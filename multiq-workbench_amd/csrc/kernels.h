@@ -9,8 +9,16 @@ namespace mq {
 
 void launch_inject(const IonState& S, const JobP* jobs, uint32_t njobs, const InjP& P, uint32_t n, hipStream_t st);
 void launch_combine_dc(double* out, const double* const d[4], const double v[4], uint32_t n3, hipStream_t st);
+// FP64 plain step. rec != nullptr: per-cell FP64 records of the 8 corner node
+// values of the three bases (PLAIN_REC_D2 double2 per cell, [basis][corner]
+// [component], built by launch_build_plain_recs from the same tables), cached
+// per lane; the results are those of the node tables bit for bit.
+constexpr int PLAIN_REC_D2 = 36;
 void launch_plain(const IonState& S, const JobP* jobs, const BatchP& B, const double* rf, const double* dc,
-                  const double* ac, const uint8_t* mask, uint32_t n, uint32_t nsteps, uint32_t* live, hipStream_t st);
+                  const double* ac, const double2* rec, const uint8_t* mask, uint32_t n, uint32_t nsteps,
+                  uint32_t* live, hipStream_t st);
+void launch_build_plain_recs(double2* rec, const double* rf, const double* dc, const double* ac, int nx, int ny,
+                             int nz, hipStream_t st);
 void launch_summarize(const IonState& S, unsigned long long* acc, uint32_t n, hipStream_t st);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, uint32_t n, hipStream_t st);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t st);

@@ -176,12 +176,62 @@ __global__ void k_combine_dc(double* __restrict__ out, const double* d0, const d
 }
 
 // ---------------------------------------------------- FP64 plain step
+// Per-cell FP64 records: the 8 corner node values (corner dx + 2 dy + 4 dz) of
+// the RF, combined DC and AC tables, [basis][corner][component], 72 doubles in
+// PLAIN_REC_D2 16-byte pieces.
+__global__ void k_build_plain_recs(double2* __restrict__ rec, const double* __restrict__ rf,
+                                   const double* __restrict__ dc, const double* __restrict__ ac, int nx, int ny, int nz)
+{
+    const uint32_t cx = (uint32_t)(nx - 1), cy = (uint32_t)(ny - 1), cz = (uint32_t)(nz - 1);
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (uint64_t)cx * cy * cz) return;
+    const uint32_t i = (uint32_t)(c % cx), j = (uint32_t)((c / cx) % cy), k = (uint32_t)(c / ((uint64_t)cx * cy));
+    const uint32_t sy = (uint32_t)nx, sz = (uint32_t)nx * (uint32_t)ny;
+    const uint32_t b = i + sy * (j + (uint32_t)ny * k);
+    const uint32_t node[8] = {b, b + 1, b + sy, b + 1 + sy, b + sz, b + 1 + sz, b + sy + sz, b + 1 + sy + sz};
+    const double* F[3] = {rf, dc, ac};
+    double v[72];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v[24 * q + 3 * kk + d] = F[q][3 * node[kk] + d];
+    double2* dst = rec + c * PLAIN_REC_D2;
+#pragma unroll
+    for (int q = 0; q < PLAIN_REC_D2; ++q) dst[q] = make_double2(v[2 * q], v[2 * q + 1]);
+}
+
+// The three basis fields at (ii, jj, kk; tx, ty, tz), in the corner order and
+// lerp chain of sample_d (interp, channel.metal:63-68), from a lane's cached
+// record of its cell.
+__device__ __forceinline__ void sample_rec(const double v[72], double tx, double ty, double tz, double er[3],
+                                           double ed[3], double ea[3])
+{
+    double* out[3] = {er, ed, ea};
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double* f = v + 24 * q + d;
+            const double c00 = lerp_d(f[0], f[3], tx), c10 = lerp_d(f[6], f[9], tx);
+            const double c01 = lerp_d(f[12], f[15], tx), c11 = lerp_d(f[18], f[21], tx);
+            out[q][d] = lerp_d(lerp_d(c00, c10, ty), lerp_d(c01, c11, ty), tz);
+        }
+}
+
 // channel.metal:75-123 in FP64: kick with E at the current position, drift,
 // advance the step counter, collide, classify (return, exit, box, crash, timeout).
+// REC: the field comes from per-cell records (k_build_plain_recs) that a lane
+// keeps in registers and reloads, 36 16-byte loads off one address, only when
+// its ion has entered another cell; otherwise it is gathered from the node
+// tables, 72 separately addressed values per step. The arithmetic is the same.
+template <bool REC>
 __global__ __launch_bounds__(256) void k_plain(IonState S, const JobP* __restrict__ jobs, BatchP B,
                                                const double* __restrict__ rf, const double* __restrict__ dc,
-                                               const double* __restrict__ ac, const uint8_t* __restrict__ mask,
-                                               uint32_t n, uint32_t nsteps, uint32_t* __restrict__ live)
+                                               const double* __restrict__ ac, const double2* __restrict__ rec,
+                                               const uint8_t* __restrict__ mask, uint32_t n, uint32_t nsteps,
+                                               uint32_t* __restrict__ live)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool alive = false;
@@ -200,17 +250,35 @@ __global__ __launch_bounds__(256) void k_plain(IonState S, const JobP* __restric
             s0 = S.rng[4 * (size_t)i]; s1 = S.rng[4 * (size_t)i + 1];
             s2 = S.rng[4 * (size_t)i + 2]; s3 = S.rng[4 * (size_t)i + 3];
         }
+        uint32_t kc = 0xFFFFFFFFu;  // the cell whose record kv holds
+        double kv[REC ? 72 : 1];
         for (uint32_t s = 0; s < nsteps; ++s) {
             int ii, jj, kk;
             double tx, ty, tz;
             loc_d(x * B.inv_h, B.nx, ii, tx, B.nearest);
             loc_d(y * B.inv_h, B.ny, jj, ty, B.nearest);
             loc_d(z * B.inv_h, B.nz, kk, tz, B.nearest);
-            const uint32_t b = (uint32_t)ii + sy * ((uint32_t)jj + (uint32_t)B.ny * (uint32_t)kk);
             double er[3], ed[3], ea[3];
-            sample_d(rf, b, sy, sz, tx, ty, tz, er);
-            sample_d(dc, b, sy, sz, tx, ty, tz, ed);
-            sample_d(ac, b, sy, sz, tx, ty, tz, ea);
+            if constexpr (REC) {
+                const uint32_t cell =
+                    (uint32_t)ii + (uint32_t)(B.nx - 1) * ((uint32_t)jj + (uint32_t)(B.ny - 1) * (uint32_t)kk);
+                if (cell != kc) {
+                    kc = cell;
+                    const double2* p = rec + (size_t)cell * PLAIN_REC_D2;
+#pragma unroll
+                    for (int q = 0; q < PLAIN_REC_D2; ++q) {
+                        const double2 t = p[q];
+                        kv[2 * q] = t.x;
+                        kv[2 * q + 1] = t.y;
+                    }
+                }
+                sample_rec(kv, tx, ty, tz, er, ed, ea);
+            } else {
+                const uint32_t b = (uint32_t)ii + sy * ((uint32_t)jj + (uint32_t)B.ny * (uint32_t)kk);
+                sample_d(rf, b, sy, sz, tx, ty, tz, er);
+                sample_d(dc, b, sy, sz, tx, ty, tz, ed);
+                sample_d(ac, b, sy, sz, tx, ty, tz, ea);
+            }
             const double a_rf = J.v_rf * sin2pi(frac_d(J.w_rf * (double)st));
             const double a_ac = J.v_ac * sin2pi(frac_d(fma(J.w_ac, (double)st, phi)));
             double E;
@@ -432,9 +500,17 @@ void launch_combine_dc(double* out, const double* const d[4], const double v[4],
 }
 
 void launch_plain(const IonState& S, const JobP* jobs, const BatchP& B, const double* rf, const double* dc,
-                  const double* ac, const uint8_t* mask, uint32_t n, uint32_t nsteps, uint32_t* live, hipStream_t st)
+                  const double* ac, const double2* rec, const uint8_t* mask, uint32_t n, uint32_t nsteps,
+                  uint32_t* live, hipStream_t st)
 {
-    k_plain<<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, rf, dc, ac, mask, n, nsteps, live);
+    if (rec) k_plain<true><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, rf, dc, ac, rec, mask, n, nsteps, live);
+    else k_plain<false><<<nblk(n, 256), 256, 0, st>>>(S, jobs, B, rf, dc, ac, rec, mask, n, nsteps, live);
+}
+void launch_build_plain_recs(double2* rec, const double* rf, const double* dc, const double* ac, int nx, int ny,
+                             int nz, hipStream_t st)
+{
+    const uint64_t nc = (uint64_t)(nx - 1) * (ny - 1) * (nz - 1);
+    if (nc) k_build_plain_recs<<<(uint32_t)((nc + 127) / 128), 128, 0, st>>>(rec, rf, dc, ac, nx, ny, nz);
 }
 
 void launch_summarize(const IonState& S, unsigned long long* acc, uint32_t n, hipStream_t st)

@@ -181,14 +181,15 @@ POS_SCALE = 1.6e10
 
 class RunStats(C.Structure):
     _fields_ = [("ion_steps", C.c_uint64), ("n_ions", C.c_uint64), ("launches", C.c_uint32),
-                ("kernel_ms", C.c_double), ("wall_ms", C.c_double), ("quad_launches", C.c_uint32)]
+                ("kernel_ms", C.c_double), ("wall_ms", C.c_double), ("quad_launches", C.c_uint32),
+                ("plain_recs", C.c_uint32)]
 
 
 class Tune(C.Structure):
     _fields_ = [("block", C.c_int32), ("waves_per_cu", C.c_int32), ("refill_period", C.c_int32),
                 ("chunk", C.c_int32), ("chunk_cap", C.c_int32), ("resort", C.c_int32), ("sync_every", C.c_int32),
                 ("graph", C.c_int32), ("coll_batch", C.c_int32), ("tail_lanes", C.c_int32),
-                ("tail_quads", C.c_int32)]
+                ("tail_quads", C.c_int32), ("plain_recs", C.c_int32)]
 
 
 class Objective(C.Structure):

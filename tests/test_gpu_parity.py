@@ -65,6 +65,47 @@ def jobs_for(case, n):
 CASES = ["full_thermal", "half_thermal", "half_thermal_nearest", "thermal_disc", "legacy", "legacy_trilinear"]
 
 
+@pytest.fixture(scope="module")
+def c1_push():
+    """The C1 grid with a uniform axial push added to its DC basis, so that every trajectory depends on the DC
+    weight (the gate alone acts only near z = 16.8 mm)."""
+    g = synth.c1_grid()
+    push = np.zeros((g["nx"] * g["ny"] * g["nz"], 3))
+    push[:, 2] = 2.0
+    g["dc"] = [None, g["dc"][1] + push.reshape(-1), None, None]
+    say("[gpu] creating engine on the 64x64x256 grid with an axial DC push")
+    e = mqhip.Engine(g)
+    yield g, e
+    e.close()
+
+
+@pytest.mark.parametrize("case", CASES + ["compat"])
+def test_plain_cell_records_keep_results(c1_push, case):
+    """The plain path reads the field from per-cell FP64 records of the corner node values, kept per lane while
+    its ion stays in a cell (mq_tune.plain_recs = 1, the default), or gathers the node tables at every step (0).
+    Per-ion results are the same bit for bit, for every collision model and for the reference's own streams, and
+    after a change of the DC set (the records are rebuilt for the new combined DC)."""
+    g, e = c1_push
+    ph = physics("half_thermal").copy(rng=mqhip.RNG_COMPAT) if case == "compat" else physics(case)
+    jobs = jobs_for("half_thermal" if case == "compat" else case, 2000)
+    t0 = e.get_tune()
+    runs = {}
+    try:
+        for dc in ([0, 5, 0, 0], [0, 3.5, 0, 0]):
+            for recs in (1, 0):
+                say(f"[gpu] plain path, {case}, dc {dc}: plain_recs={recs}")
+                e.set_tune(plain_recs=recs)
+                runs[(tuple(dc), recs)] = e.run_batch_ions(jobs, dc, ph, mqhip.PATH_PLAIN)
+                assert e.stats()["plain_recs"] == recs
+    finally:
+        e.set_tune(**t0)
+    for dc in ([0, 5, 0, 0], [0, 3.5, 0, 0]):
+        a, b = runs[(tuple(dc), 1)], runs[(tuple(dc), 0)]
+        for k in KEYS + ("steps", "ncoll", "status"):
+            assert np.array_equal(a[k], b[k]), (dc, k)
+    assert not np.array_equal(runs[((0, 5, 0, 0), 1)]["z"], runs[((0, 3.5, 0, 0), 1)]["z"])
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_plain_fp64_matches_oracle(c1, oracle_mod, case):
     g, e = c1
