@@ -123,6 +123,13 @@ int mq_tandem_run_ions(mq_tandem* t, const mq_job* jobs, size_t njobs, const mq_
 /* Statistics of the last run (ion-steps, launches, device and wall time). */
 int mq_tandem_last_stats(const mq_tandem* t, mq_run_stats* out);
 
+/* Field access of the FP64 plain path (no counterpart in trapsim, whose
+ * transport is a CPU loop): 1 (default) reads per-cell FP64 records of the four
+ * field sets (576 B per cell each, built once per drive's DC sets when device
+ * memory allows), 0 gathers the node tables at every step. Results are the
+ * same; mq_run_stats.plain_recs reports which one the last run used. */
+int mq_tandem_set_plain_recs(mq_tandem* t, int32_t on);
+
 #ifdef __cplusplus
 }
 #endif

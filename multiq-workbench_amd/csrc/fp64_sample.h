@@ -38,4 +38,22 @@ __device__ __forceinline__ void sample_d(const double* __restrict__ F, uint32_t 
     }
 }
 
+// The three basis fields at (tx, ty, tz) in a cell, in the corner order and
+// lerp chain of sample_d, from a per-cell FP64 record of the 8 corner node
+// values, [basis][corner][component] (72 doubles).
+__device__ __forceinline__ void sample_rec(const double v[72], double tx, double ty, double tz, double er[3],
+                                           double ed[3], double ea[3])
+{
+    double* out[3] = {er, ed, ea};
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double* f = v + 24 * q + d;
+            const double c00 = lerp_d(f[0], f[3], tx), c10 = lerp_d(f[6], f[9], tx);
+            const double c01 = lerp_d(f[12], f[15], tx), c11 = lerp_d(f[18], f[21], tx);
+            out[q][d] = lerp_d(lerp_d(c00, c10, ty), lerp_d(c01, c11, ty), tz);
+        }
+}
+
 }  // namespace mq

@@ -202,24 +202,6 @@ __global__ void k_build_plain_recs(double2* __restrict__ rec, const double* __re
     for (int q = 0; q < PLAIN_REC_D2; ++q) dst[q] = make_double2(v[2 * q], v[2 * q + 1]);
 }
 
-// The three basis fields at (ii, jj, kk; tx, ty, tz), in the corner order and
-// lerp chain of sample_d (interp, channel.metal:63-68), from a lane's cached
-// record of its cell.
-__device__ __forceinline__ void sample_rec(const double v[72], double tx, double ty, double tz, double er[3],
-                                           double ed[3], double ea[3])
-{
-    double* out[3] = {er, ed, ea};
-#pragma unroll
-    for (int q = 0; q < 3; ++q)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double* f = v + 24 * q + d;
-            const double c00 = lerp_d(f[0], f[3], tx), c10 = lerp_d(f[6], f[9], tx);
-            const double c01 = lerp_d(f[12], f[15], tx), c11 = lerp_d(f[18], f[21], tx);
-            out[q][d] = lerp_d(lerp_d(c00, c10, ty), lerp_d(c01, c11, ty), tz);
-        }
-}
-
 // channel.metal:75-123 in FP64: kick with E at the current position, drift,
 // advance the step counter, collide, classify (return, exit, box, crash, timeout).
 // REC: the field comes from per-cell records (k_build_plain_recs) that a lane

@@ -131,11 +131,18 @@ struct TPlainArgs {
     TandemP T;
     TGrid cube, port;
     const double* dc[4];  // combined DC: splitter, combiner, driven port, idle port
+    // with REC: per-cell FP64 records of the same four field sets (k_build_plain_recs; the cube's
+    // records carry an unused AC slot)
+    const double2* rec[4];
     unsigned long long* ctr;
     uint32_t n, nsteps;
     uint32_t* live;
 };
 
+// REC: the field comes from the per-cell records of the ion's field set, kept
+// per lane while the ion stays in one cell of one set; otherwise from the node
+// tables. The lerp chain is the same, so the results are too.
+template <bool REC>
 __global__ __launch_bounds__(256) void k_tandem_plain(TPlainArgs A)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -152,6 +159,8 @@ __global__ __launch_bounds__(256) void k_tandem_plain(TPlainArgs A)
         const double dt = T.dt;
         int status = 0;
         u32x4 cw = {0, 0, 0, 0};
+        uint32_t kc = 0xFFFFFFFFu;  // 4 x cell + field set of the record in kv
+        double kv[REC ? 72 : 1];
         for (uint32_t s = 0; s < A.nsteps; ++s) {
             const bool cube = reg == REG_SPLIT || reg == REG_COMBINE;
             const TGrid& G = cube ? A.cube : A.port;
@@ -162,15 +171,35 @@ __global__ __launch_bounds__(256) void k_tandem_plain(TPlainArgs A)
             loc_d(x * G.inv_h, G.nx, ii, tx, (int)T.nearest);
             loc_d(y * G.inv_h, G.ny, jj, ty, (int)T.nearest);
             loc_d(z * G.inv_h, G.nz, kk, tz, (int)T.nearest);
-            const uint32_t sy = (uint32_t)G.nx, sz = (uint32_t)G.nx * (uint32_t)G.ny;
-            const uint32_t b = (uint32_t)ii + sy * ((uint32_t)jj + (uint32_t)G.ny * (uint32_t)kk);
             double er[3], ed[3], ea[3] = {0.0, 0.0, 0.0};
-            sample_d(G.rf, b, sy, sz, tx, ty, tz, er);
-            sample_d(dcp, b, sy, sz, tx, ty, tz, ed);
+            if constexpr (REC) {
+                const uint32_t set = reg == REG_SPLIT ? 0u : reg == REG_COMBINE ? 1u : port_driven(T, reg) ? 2u : 3u;
+                const uint32_t cell =
+                    (uint32_t)ii + (uint32_t)(G.nx - 1) * ((uint32_t)jj + (uint32_t)(G.ny - 1) * (uint32_t)kk);
+                if (4u * cell + set != kc) {
+                    kc = 4u * cell + set;
+                    const double2* p = (set == 0u ? A.rec[0] : set == 1u ? A.rec[1] : set == 2u ? A.rec[2] : A.rec[3]) +
+                                       (size_t)cell * PLAIN_REC_D2;
+#pragma unroll
+                    for (int q = 0; q < PLAIN_REC_D2; ++q) {
+                        const double2 t = p[q];
+                        kv[2 * q] = t.x;
+                        kv[2 * q + 1] = t.y;
+                    }
+                }
+                double eac[3];
+                sample_rec(kv, tx, ty, tz, er, ed, eac);
+                if (!cube) { ea[0] = eac[0]; ea[1] = eac[1]; ea[2] = eac[2]; }
+            } else {
+                const uint32_t sy = (uint32_t)G.nx, sz = (uint32_t)G.nx * (uint32_t)G.ny;
+                const uint32_t b = (uint32_t)ii + sy * ((uint32_t)jj + (uint32_t)G.ny * (uint32_t)kk);
+                sample_d(G.rf, b, sy, sz, tx, ty, tz, er);
+                sample_d(dcp, b, sy, sz, tx, ty, tz, ed);
+                if (!cube) sample_d(G.ac, b, sy, sz, tx, ty, tz, ea);
+            }
             const double a_rf = T.v_rf * sin2pi(frac_d(T.w_rf * (double)st));
             double a_ac = 0.0;
             if (!cube) {
-                sample_d(G.ac, b, sy, sz, tx, ty, tz, ea);
                 const double wa = w_ac_of(T, reg);
                 if (wa > 0.0) a_ac = T.v_ac * sin2pi(frac_d(wa * (double)st));
             }
@@ -537,14 +566,19 @@ void launch_tandem_init(const IonState& S, uint32_t n, hipStream_t st)
 }
 
 void launch_tandem_plain(const IonState& S, const JobP* jobs, const TandemP& T, const TGrid& cube, const TGrid& port,
-                         const double* const dc[4], unsigned long long* ctr, uint32_t n, uint32_t nsteps,
-                         uint32_t* live, hipStream_t st)
+                         const double* const dc[4], const double2* const rec[4], unsigned long long* ctr, uint32_t n,
+                         uint32_t nsteps, uint32_t* live, hipStream_t st)
 {
     TPlainArgs a;
     a.S = S; a.jobs = jobs; a.T = T; a.cube = cube; a.port = port;
-    for (int k = 0; k < 4; ++k) a.dc[k] = dc[k];
+    for (int k = 0; k < 4; ++k) {
+        a.dc[k] = dc[k];
+        a.rec[k] = rec ? rec[k] : nullptr;
+    }
     a.ctr = ctr; a.n = n; a.nsteps = nsteps; a.live = live;
-    if (n) k_tandem_plain<<<(n + 255) / 256, 256, 0, st>>>(a);
+    if (!n) return;
+    if (rec) k_tandem_plain<true><<<(n + 255) / 256, 256, 0, st>>>(a);
+    else k_tandem_plain<false><<<(n + 255) / 256, 256, 0, st>>>(a);
 }
 
 void launch_tandem_batched(const IonState& S, const JobP* jobs, const TandemP& T, const TGrid& cube,

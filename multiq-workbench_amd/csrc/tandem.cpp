@@ -27,9 +27,11 @@ void mq_set_last_error(const std::string& msg);  // engine.cpp
 
 namespace mq {
 void launch_tandem_init(const IonState& S, uint32_t n, hipStream_t st);
+// rec: per-cell FP64 records of the four field sets (launch_build_plain_recs), or
+// nullptr for the node tables.
 void launch_tandem_plain(const IonState& S, const JobP* jobs, const TandemP& T, const TGrid& cube, const TGrid& port,
-                         const double* const dc[4], unsigned long long* ctr, uint32_t n, uint32_t nsteps,
-                         uint32_t* live, hipStream_t st);
+                         const double* const dc[4], const double2* const rec[4], unsigned long long* ctr, uint32_t n,
+                         uint32_t nsteps, uint32_t* live, hipStream_t st);
 void launch_tandem_batched(const IonState& S, const JobP* jobs, const TandemP& T, const TGrid& cube,
                            const TGrid& port, const float4* const cells[4], unsigned long long* ctr,
                            const uint32_t* list_in, uint32_t* list_out, uint32_t* ctr_in, uint32_t* ctr_out,
@@ -99,9 +101,10 @@ struct Grid {
 
 // Combined DC field of one table and its FP32 cell records, cached by voltages.
 struct Table {
-    bool valid = false, cells_valid = false;
+    bool valid = false, cells_valid = false, prec_valid = false;
     double key[4] = {0, 0, 0, 0};
     DevBuf dcc, cells;
+    DevBuf prec;  // FP64 cell records of the plain path
 };
 
 }  // namespace
@@ -116,6 +119,7 @@ struct mq_tandem {
     DevBuf jobs, ctr, tot, list[3], lctr, bin_of, rank_of, hist, bsum;
     DevBuf sx, sy, sz, sphi, svx, svy, svz, sfx, sfy, sfz, ssteps, sncoll, scd, sjob, sstatus, sregion, slast;
     mq_run_stats stats{};
+    bool plain_recs = true;  // mq_tandem_set_plain_recs
 
     void create(const mq_fields* c, const mq_fields* p, const mq_tandem_geom* g)
     {
@@ -187,6 +191,7 @@ struct mq_tandem {
             HIPCHK(hipGetLastError());
             T.valid = true;
             T.cells_valid = false;
+            T.prec_valid = false;
         }
         if (want_cells && !T.cells_valid) {
             const size_t nc = (size_t)(G.meta.nx - 1) * (G.meta.ny - 1) * (G.meta.nz - 1);
@@ -313,8 +318,49 @@ struct mq_tandem {
         return S;
     }
 
+    // FP64 cell records of the four field sets for the plain path (as the single-channel engine's), when
+    // plain_recs is on and all four fit in half of the free device memory; false: node tables.
+    bool plain_records(const double2* rec[4])
+    {
+        if (!plain_recs) return false;
+        size_t need[4], total = 0;
+        for (int t = 0; t < 4; ++t) {
+            const Grid& G = t < 2 ? cube : port;
+            const uint64_t nc = (uint64_t)(G.meta.nx - 1) * (G.meta.ny - 1) * (G.meta.nz - 1);
+            if (nc >= (1ull << 30)) return false;  // the kernel's cache key is 4 x cell + set
+            need[t] = nc * PLAIN_REC_D2 * sizeof(double2);
+            if (tab[t].prec.bytes < need[t]) total += need[t];
+        }
+        if (total) {
+            size_t free_b = 0, total_b = 0;
+            HIPCHK(hipMemGetInfo(&free_b, &total_b));
+            if (total > free_b / 2) return false;
+        }
+        for (int t = 0; t < 4; ++t) {
+            const Grid& G = t < 2 ? cube : port;
+            Table& T = tab[t];
+            if (T.prec.bytes < need[t]) {
+                T.prec.ensure(need[t]);
+                T.prec_valid = false;
+            }
+            if (!T.prec_valid) {
+                // the cube has no excitation: its AC slot is never used
+                const double* acp = G.has_ac ? G.ac.as<double>() : G.rf.as<double>();
+                launch_build_plain_recs(T.prec.as<double2>(), G.rf.as<double>(), T.dcc.as<double>(), acp, G.meta.nx,
+                                        G.meta.ny, G.meta.nz, stream);
+                HIPCHK(hipGetLastError());
+                T.prec_valid = true;
+            }
+            rec[t] = T.prec.as<double2>();
+        }
+        return true;
+    }
+
     void run_plain(const IonState& S, const TandemP& T, uint32_t n, float& kms)
     {
+        const double2* rec[4] = {nullptr, nullptr, nullptr, nullptr};
+        const bool use_rec = plain_records(rec);
+        stats.plain_recs = use_rec ? 1u : 0u;
         uint32_t nsteps = (uint32_t)std::max(1000.0, std::floor(2e-4 / T.dt));
         nsteps = (nsteps + 3u) & ~3u;
         lctr.ensure(256);
@@ -325,8 +371,8 @@ struct mq_tandem {
         for (;;) {
             launch_fill_u32(live, 0u, 1, stream);
             HIPCHK(hipEventRecord(ev0, stream));
-            launch_tandem_plain(S, jobs.as<JobP>(), T, cg, pg, dc, ctr.as<unsigned long long>(), n, nsteps, live,
-                                stream);
+            launch_tandem_plain(S, jobs.as<JobP>(), T, cg, pg, dc, use_rec ? rec : nullptr,
+                                ctr.as<unsigned long long>(), n, nsteps, live, stream);
             HIPCHK(hipEventRecord(ev1, stream));
             HIPCHK(hipGetLastError());
             uint32_t alive = 0;
@@ -531,6 +577,15 @@ int mq_tandem_last_stats(const mq_tandem* t, mq_run_stats* out)
 {
     if (!t || !out) { mq_set_last_error("NULL argument"); return -1; }
     *out = t->stats;
+    return 0;
+}
+
+int mq_tandem_set_plain_recs(mq_tandem* t, int32_t on)
+{
+    if (!t) { mq_set_last_error("tandem is NULL"); return -1; }
+    if (on < 0 || on > 1) { mq_set_last_error("plain_recs must be 0 or 1"); return -1; }
+    t->plain_recs = on != 0;
+    mq_set_last_error("");
     return 0;
 }
 

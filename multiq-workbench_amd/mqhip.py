@@ -379,6 +379,7 @@ SIGNATURES = {
                                      C.POINTER(IonSoA), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
                                      C.POINTER(C.c_uint32)]),
     "mq_tandem_last_stats": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
+    "mq_tandem_set_plain_recs": (C.c_int, [C.c_void_p, C.c_int32]),
     "mq_legacy_default": (None, [C.POINTER(LegacyOpts)]),
     "mq_legacy_dims": (C.c_int, [C.POINTER(LegacyOpts), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
@@ -1033,6 +1034,10 @@ class Tandem:
         s = RunStats()
         _check(load().mq_tandem_last_stats(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def set_plain_recs(self, on: bool) -> None:
+        """mq_tandem_set_plain_recs: the FP64 path reads per-cell records (True) or the node tables."""
+        _check(load().mq_tandem_set_plain_recs(self._h, int(bool(on))))
 
 
 # ------------------------------------------------------------------ scan API

@@ -51,6 +51,29 @@ def test_plain_matches_oracle_exactly(small, oracle_mod):
     assert sum(p["forwarded"] for p in c) > 0 and tt["detected"] > 0
 
 
+def test_plain_cell_records_keep_results(small):
+    """The FP64 path reads per-cell records of the four field sets (splitter, combiner, driven and idle port;
+    mq_tandem_set_plain_recs(1), the default) or gathers the node tables (0): the counters and every ion's final
+    state are the same bit for bit."""
+    t, (cube, port, geom, d, jobs, ph) = small
+    big = [mqhip.job(j.sp.mass, j.sp.z, j.sp.ccs_n2, j.drive.v_rf, j.drive.f_rf, j.drive.v_ac, j.drive.f_ac, 1024,
+                     j.seed) for j in jobs]
+    out = {}
+    try:
+        for on in (True, False):
+            say(f"[gpu] tandem FP64, 4 x 1024 ions, cell records {on}")
+            t.set_plain_recs(on)
+            out[on] = t.run_ions(big, d, ph, mqhip.PATH_PLAIN)
+            assert t.stats()["plain_recs"] == int(on)
+    finally:
+        t.set_plain_recs(True)
+    (ca, ta, sa), (cb, tb, sb) = out[True], out[False]
+    assert ta == tb and ca == cb
+    for k in ("status", "region", "last", "steps", "ncoll", "job", "x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(sa[k], sb[k]), k
+    assert sum(p["entered"] for p in ca) > 0 and ta["detected"] > 0
+
+
 @pytest.mark.parametrize("case", ["entry", "bounce", "forward_floor", "reverse", "from_other", "board_top",
                                   "board_mouth", "back", "wall", "detect"])
 def test_closed_form_cases_match_oracle(oracle_mod, case):
